@@ -1,0 +1,91 @@
+"""ctypes binding of libgpusharesat.so (the C ABI in include/gss.h and
+include/gss_sharer.h).
+
+The library is built in-tree by gpusharesat_amd.build.  Importing this module
+without it raises ImportError: there is no CPU fallback for the check path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libgpusharesat.so")
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_u64 = ctypes.c_uint64
+c_p = ctypes.c_void_p
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+c_char_p = ctypes.c_char_p
+
+
+class GssError(RuntimeError):
+    def __init__(self, func: str, code: int, msg: str):
+        super().__init__(f"{func} failed ({code}): {msg}")
+        self.code = code
+
+
+# Every symbol exported by include/gss.h and include/gss_sharer.h, with its
+# argument types (all return int status unless listed in _RESTYPES).
+SIGNATURES: dict[str, list] = {
+    "gss_last_error": [],
+    "gss_engine_create": [c_int, c_p],
+    "gss_engine_destroy": [c_p],
+    "gss_engine_cu_count": [c_p, c_p],
+    "gss_engine_sync": [c_p],
+    "gss_db_append": [c_p, c_i64, c_p, c_p, c_p],
+    "gss_db_set_bucket": [c_p, c_int, c_i64, c_p],
+    "gss_db_bucket_count": [c_p, c_int, c_p],
+    "gss_db_totals": [c_p, c_p, c_p],
+    "gss_db_read": [c_p, c_int, c_i64, c_i64, c_p],
+    "gss_db_clear": [c_p],
+    "gss_assig_reset": [c_p, c_i64, c_int],
+    "gss_assig_grow_vars": [c_p, c_i64],
+    "gss_assig_upload": [c_p, c_p, c_p],
+    "gss_assig_download": [c_p, c_p, c_p],
+    "gss_assig_apply": [c_p, c_i64, c_p],
+    "gss_assig_roll_forward": [c_p, c_i64, c_p, c_p],
+    "gss_sweep_async": [c_p, c_p],
+    "gss_sweep_wait": [c_p, c_p],
+    "gss_sweep": [c_p, c_p, c_p],
+    "gss_hits_copy": [c_p, c_p, c_i64, c_p],
+    "gss_sweep_kernel_ms": [c_p, c_p],
+    "gss_sweep_kernel_stats": [c_p, c_p, c_p, c_int],
+    "gss_set_tuning": [c_p, c_int, c_int],
+}
+_RESTYPES = {"gss_last_error": c_char_p}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the library once; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} not found: build it with `python -m gpusharesat_amd.build` "
+            "(there is no CPU fallback for the check path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        try:
+            f = getattr(lib, name)
+        except AttributeError as exc:
+            raise ImportError(f"{LIB_PATH} does not export {name}") from exc
+        f.argtypes = args
+        f.restype = _RESTYPES.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def check(name: str, rc: int) -> None:
+    if rc != 0:
+        msg = load().gss_last_error().decode(errors="replace")
+        raise GssError(name, rc, msg)
+
+
+def call(name: str, *args) -> None:
+    check(name, getattr(load(), name)(*args))

@@ -1,0 +1,778 @@
+// Host side of the check engine: device memory for the clause buckets, the
+// assignment table and the hit chunks, and the C ABI declared in gss.h.
+//
+// Layout in HBM
+//   clause bucket `len`: uint32 literals, row-major (row r at r*len), capacity
+//       doubling; replaces the reference's 32-interleaved per-size arrays
+//       (gpuShareLib/Clauses.cu:47-55) because a clause is read by a whole lane
+//       group at once here, not by one lane.
+//   assignment table: uint64 [2*vars][words]; row 2v = slots where v is false,
+//       row 2v+1 = slots where v is true, i.e. row l = slots where literal l is
+//       false.  Replaces per-solver MultiLBool arrays + MultiAgg aggregates
+//       (gpuShareLib/Assigs.cuh:39-74): one row holds every solver's slots, so
+//       the reference's aggregate prefilter and agg-bit -> solver walk
+//       (GpuRunner.cu:92-131) are not needed.
+//   hits: chunks of kHitChunk 24-byte records + a fill count per chunk.
+//
+// Threading: one engine is driven by one host thread at a time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gss.h"
+#include "kernels.h"
+
+using namespace gss;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return set_err(_e == hipErrorOutOfMemory ? GSS_E_NOMEM : GSS_E_HIP, "%s: %s", \
+                           #expr, hipGetErrorString(_e));                                  \
+    } while (0)
+
+struct Bucket {
+    uint32_t *d = nullptr;
+    int64_t count = 0;
+    int64_t cap = 0;  // rows
+};
+
+template <typename T>
+struct Pinned {
+    T *p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t n) {
+        if (n <= cap) return GSS_OK;
+        size_t nc = std::max(n, cap * 2);
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc((void **)&p, nc * sizeof(T), hipHostMallocDefault) != hipSuccess)
+            return set_err(GSS_E_NOMEM, "pinned host allocation of %zu bytes failed",
+                           nc * sizeof(T));
+        cap = nc;
+        return GSS_OK;
+    }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    int reserve(size_t n, bool keep = false, hipStream_t s = nullptr) {
+        if (n <= cap) return GSS_OK;
+        size_t nc = std::max(n, cap + cap / 2);
+        T *np = nullptr;
+        hipError_t e = hipMalloc((void **)&np, nc * sizeof(T));
+        if (e != hipSuccess)
+            return set_err(GSS_E_NOMEM, "device allocation of %zu bytes failed: %s",
+                           nc * sizeof(T), hipGetErrorString(e));
+        if (p) {
+            if (keep && cap) {
+                HIP_TRY(hipMemcpyAsync(np, p, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
+            }
+            HIP_TRY(hipStreamSynchronize(s));
+            (void)hipFree(p);
+        }
+        p = np;
+        cap = nc;
+        return GSS_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct gss_engine {
+    int device = 0;
+    int cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    std::vector<Bucket> buckets;  // index = clause length
+    int64_t total_clauses = 0, total_lits = 0;
+    // Largest literal ever stored (running max, never lowered); a sweep is
+    // refused unless the table has a row for it, so the kernel's table reads
+    // stay in bounds.
+    int64_t max_lit = -1;
+
+    // assignment table
+    DevBuf<uint64_t> table;
+    int64_t vars = 0;
+    int words = 0;
+
+    // sweep state
+    DevBuf<BucketDesc> d_desc;
+    Pinned<BucketDesc> h_desc;
+    uint64_t *d_valid = nullptr;
+    Pinned<uint64_t> h_valid;
+    DevBuf<gss_hit> d_hits;
+    DevBuf<uint32_t> d_fills;
+    uint32_t *d_alloc = nullptr;
+    Pinned<uint32_t> h_small;
+    std::vector<uint32_t> fills;
+    uint32_t chunk_cap = 0;
+    int nbuckets = 0, ntasks = 0, task_clauses = 0, blocks = 0;
+    bool pending = false, waited = false, have_sweep = false;
+    int64_t last_hits = 0;
+    uint32_t last_chunks = 0;
+    float last_ms = 0.f;
+    double total_ms = 0.0;
+    int64_t nsweeps = 0;
+    int occ[7] = {0, 0, 0, 0, 0, 0, 0};
+
+    // tuning
+    int tune_waves_per_cu = 0;
+    int tune_task_clauses = 0;
+    int tune_max_chunks = 0;
+
+    // staging
+    Pinned<uint32_t> h_stage;
+    DevBuf<uint32_t> d_stage;
+    Pinned<ScatterSeg> h_segs;
+    DevBuf<ScatterSeg> d_segs;
+    Pinned<gss_var_update> h_ups;
+    DevBuf<gss_var_update> d_ups;
+    int8_t *d_lastpos = nullptr;
+    Pinned<int8_t> h_lastpos;
+    DevBuf<uint32_t> d_offsets;
+    DevBuf<gss_hit> d_dense;
+    Pinned<gss_hit> h_hits;
+};
+
+namespace {
+
+int check_engine(gss_engine *e) {
+    if (!e) return set_err(GSS_E_INVAL, "null engine");
+    return hipSetDevice(e->device) == hipSuccess ? GSS_OK
+                                                 : set_err(GSS_E_HIP, "hipSetDevice failed");
+}
+
+int log2_exact(int w) {
+    int l = 0;
+    while ((1 << l) < w) l++;
+    return (1 << l) == w ? l : -1;
+}
+
+int ensure_bucket_cap(gss_engine *e, int len, int64_t need) {
+    if ((int)e->buckets.size() <= len) e->buckets.resize(len + 1);
+    Bucket &b = e->buckets[len];
+    if (need <= b.cap) return GSS_OK;
+    int64_t nc = std::max<int64_t>(std::max<int64_t>(need, b.cap * 2), 256);
+    uint32_t *np = nullptr;
+    hipError_t he = hipMalloc((void **)&np, (size_t)nc * len * sizeof(uint32_t));
+    if (he != hipSuccess)
+        return set_err(GSS_E_NOMEM, "bucket %d: device allocation of %lld rows failed: %s", len,
+                       (long long)nc, hipGetErrorString(he));
+    if (b.d) {
+        if (b.count)
+            HIP_TRY(hipMemcpyAsync(np, b.d, (size_t)b.count * len * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        (void)hipFree(b.d);
+    }
+    b.d = np;
+    b.cap = nc;
+    return GSS_OK;
+}
+
+// Launches the sweep kernel for the current descriptors (already uploaded).
+int launch_current_sweep(gss_engine *e) {
+    HIP_TRY(launch_reset_u32(e->d_alloc, 1, e->stream));
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    if (e->ntasks > 0) {
+        SweepArgs a;
+        a.buckets = e->d_desc.p;
+        a.nbuckets = e->nbuckets;
+        a.ntasks = e->ntasks;
+        a.task_clauses = e->task_clauses;
+        a.table = e->table.p;
+        a.valid = e->d_valid;
+        a.hits = e->d_hits.p;
+        a.fills = e->d_fills.p;
+        a.alloc = e->d_alloc;
+        a.chunk_cap = e->chunk_cap;
+        HIP_TRY(launch_sweep(a, e->words, e->blocks, e->stream));
+    }
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    return GSS_OK;
+}
+
+int reserve_chunks(gss_engine *e, uint64_t chunks) {
+    if (chunks > 0xFFFFFFF0ull) return set_err(GSS_E_NOMEM, "hit buffer too large");
+    int rc;
+    if ((rc = e->d_hits.reserve((size_t)chunks * kHitChunk))) return rc;
+    if ((rc = e->d_fills.reserve((size_t)chunks))) return rc;
+    e->chunk_cap = (uint32_t)std::min<size_t>(e->d_fills.cap, e->d_hits.cap / kHitChunk);
+    return GSS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gss_last_error(void) { return g_err.c_str(); }
+
+int gss_engine_create(int device, gss_engine **out) {
+    if (!out) return set_err(GSS_E_INVAL, "null out");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return set_err(GSS_E_INVAL, "device %d out of range (%d devices)", device, ndev);
+    HIP_TRY(hipSetDevice(device));
+    gss_engine *e = new (std::nothrow) gss_engine();
+    if (!e) return set_err(GSS_E_NOMEM, "engine allocation failed");
+    e->device = device;
+    int rc = GSS_OK;
+    auto fail = [&](int code) {
+        gss_engine_destroy(e);
+        return code;
+    };
+    if (hipDeviceGetAttribute(&e->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return fail(set_err(GSS_E_HIP, "cannot query CU count"));
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(GSS_E_HIP, "stream creation failed"));
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return fail(set_err(GSS_E_HIP, "event creation failed"));
+    if (hipMalloc((void **)&e->d_valid, GSS_MAX_WORDS * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc((void **)&e->d_alloc, 64) != hipSuccess ||
+        hipMalloc((void **)&e->d_lastpos, GSS_MAX_WORDS) != hipSuccess)
+        return fail(set_err(GSS_E_NOMEM, "device allocation failed"));
+    if ((rc = e->h_valid.reserve(GSS_MAX_WORDS)) || (rc = e->h_small.reserve(16)) ||
+        (rc = e->h_lastpos.reserve(GSS_MAX_WORDS)))
+        return fail(rc);
+    if ((rc = gss_assig_reset(e, 0, 1))) return fail(rc);
+    *out = e;
+    return GSS_OK;
+}
+
+int gss_engine_destroy(gss_engine *e) {
+    if (!e) return GSS_OK;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (Bucket &b : e->buckets)
+        if (b.d) (void)hipFree(b.d);
+    e->table.release();
+    e->d_desc.release();
+    e->d_hits.release();
+    e->d_fills.release();
+    e->d_stage.release();
+    e->d_segs.release();
+    e->d_ups.release();
+    e->d_offsets.release();
+    e->d_dense.release();
+    if (e->d_valid) (void)hipFree(e->d_valid);
+    if (e->d_alloc) (void)hipFree(e->d_alloc);
+    if (e->d_lastpos) (void)hipFree(e->d_lastpos);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return GSS_OK;
+}
+
+int gss_engine_cu_count(gss_engine *e, int *out) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!out) return set_err(GSS_E_INVAL, "null out");
+    *out = e->cu;
+    return GSS_OK;
+}
+
+int gss_engine_sync(gss_engine *e) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+/* ---- clause DB ------------------------------------------------------------ */
+
+int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                  uint32_t *out_rows) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!lits || !lens))) return set_err(GSS_E_INVAL, "bad clause batch");
+    if (n == 0) return GSS_OK;
+    // Group by length (stable), so that each bucket's new rows are one
+    // contiguous segment of the staging buffer.
+    int maxlen = 0;
+    std::vector<int64_t> start(n);
+    int64_t pos = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
+            return set_err(GSS_E_INVAL, "clause %lld: length %d outside [1, %d]", (long long)i,
+                           lens[i], GSS_MAX_CLAUSE_LEN);
+        maxlen = std::max(maxlen, (int)lens[i]);
+        start[i] = pos;
+        pos += lens[i];
+    }
+    const int64_t total = pos;
+    uint32_t mx = 0;
+    for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
+    std::vector<int64_t> per(maxlen + 1, 0);
+    for (int64_t i = 0; i < n; i++) per[lens[i]]++;
+    if ((int)e->buckets.size() <= maxlen) e->buckets.resize(maxlen + 1);
+    for (int len = 1; len <= maxlen; len++)
+        if (per[len] && (rc = ensure_bucket_cap(e, len, e->buckets[len].count + per[len])))
+            return rc;
+    // staging offsets (in literals) of each bucket's segment
+    std::vector<int64_t> seg_off(maxlen + 1, 0), seg_fill(maxlen + 1, 0);
+    int64_t acc = 0;
+    int nseg = 0;
+    for (int len = 1; len <= maxlen; len++) {
+        seg_off[len] = acc;
+        acc += per[len] * len;
+        if (per[len]) nseg++;
+    }
+    if ((rc = e->h_stage.reserve((size_t)total)) || (rc = e->d_stage.reserve((size_t)total)) ||
+        (rc = e->h_segs.reserve(nseg)) || (rc = e->d_segs.reserve(nseg)))
+        return rc;
+    // any previous async user of the staging buffers must be done
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int64_t i = 0; i < n; i++) {
+        const int len = lens[i];
+        const int64_t k = seg_fill[len]++;
+        std::memcpy(e->h_stage.p + seg_off[len] + k * len, lits + start[i], len * sizeof(uint32_t));
+        if (out_rows) out_rows[i] = (uint32_t)(e->buckets[len].count + k);
+    }
+    int s = 0;
+    for (int len = 1; len <= maxlen; len++) {
+        if (!per[len]) continue;
+        Bucket &b = e->buckets[len];
+        ScatterSeg &sg = e->h_segs.p[s++];
+        sg.dst = b.d + (size_t)b.count * len;
+        sg.src = (uint64_t)seg_off[len];
+        sg.count = (uint64_t)per[len] * len;
+        b.count += per[len];
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_stage.p, e->h_stage.p, (size_t)total * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, (size_t)nseg * sizeof(ScatterSeg),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(launch_scatter(e->d_segs.p, nseg, e->d_stage.p, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->total_clauses += n;
+    e->total_lits += total;
+    e->max_lit = std::max<int64_t>(e->max_lit, mx);
+    return GSS_OK;
+}
+
+int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (len < 1 || len > GSS_MAX_CLAUSE_LEN || n < 0 || (n > 0 && !lits))
+        return set_err(GSS_E_INVAL, "bad bucket replacement");
+    uint32_t mx = 0;
+    for (int64_t i = 0; i < n * len; i++) mx = std::max(mx, lits[i]);
+    if ((int)e->buckets.size() <= len) e->buckets.resize(len + 1);
+    Bucket &b = e->buckets[len];
+    const int64_t old = b.count;
+    b.count = 0;  // nothing to preserve when growing
+    if ((rc = ensure_bucket_cap(e, len, n))) {
+        b.count = old;
+        return rc;
+    }
+    if (n)
+        HIP_TRY(hipMemcpyAsync(b.d, lits, (size_t)n * len * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    b.count = n;
+    e->total_clauses += n - old;
+    e->total_lits += (n - old) * len;
+    if (n) e->max_lit = std::max<int64_t>(e->max_lit, mx);
+    return GSS_OK;
+}
+
+int gss_db_bucket_count(gss_engine *e, int len, int64_t *out) {
+    if (!e || !out) return set_err(GSS_E_INVAL, "null argument");
+    *out = (len >= 1 && len < (int)e->buckets.size()) ? e->buckets[len].count : 0;
+    return GSS_OK;
+}
+
+int gss_db_totals(gss_engine *e, int64_t *clauses, int64_t *literals) {
+    if (!e) return set_err(GSS_E_INVAL, "null engine");
+    if (clauses) *clauses = e->total_clauses;
+    if (literals) *literals = e->total_lits;
+    return GSS_OK;
+}
+
+int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (len < 1 || len >= (int)e->buckets.size() || row0 < 0 || n < 0 ||
+        row0 + n > e->buckets[len].count || (n && !out))
+        return set_err(GSS_E_INVAL, "bad bucket read");
+    if (n)
+        HIP_TRY(hipMemcpyAsync(out, e->buckets[len].d + (size_t)row0 * len,
+                               (size_t)n * len * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_db_clear(gss_engine *e) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    for (Bucket &b : e->buckets) b.count = 0;
+    e->total_clauses = e->total_lits = 0;
+    e->max_lit = -1;
+    return GSS_OK;
+}
+
+/* ---- assignment table ----------------------------------------------------- */
+
+int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (var_count < 0 || var_count > (1ll << 31) - 1 || words < 1 || words > GSS_MAX_WORDS ||
+        log2_exact(words) < 0)
+        return set_err(GSS_E_INVAL, "bad table shape (%lld vars, %d words; words must be a power "
+                                    "of two <= %d)",
+                       (long long)var_count, words, GSS_MAX_WORDS);
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const size_t n = (size_t)std::max<int64_t>(var_count, 1) * 2 * words;
+    if (n > e->table.cap) {
+        e->table.release();
+        if ((rc = e->table.reserve(n))) return rc;
+    }
+    HIP_TRY(launch_fill_u64(e->table.p, (int64_t)n, 0ull, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->vars = var_count;
+    e->words = words;
+    return GSS_OK;
+}
+
+int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (var_count < e->vars || var_count > (1ll << 31) - 1)
+        return set_err(GSS_E_INVAL, "variable count can only grow");
+    const size_t old_n = (size_t)e->vars * 2 * e->words;
+    const size_t n = (size_t)std::max<int64_t>(var_count, 1) * 2 * e->words;
+    if ((rc = e->table.reserve(n, true, e->stream))) return rc;
+    HIP_TRY(launch_fill_u64(e->table.p + old_n, (int64_t)(n - old_n), 0ull, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->vars = var_count;
+    return GSS_OK;
+}
+
+int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_true) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!is_def || !is_true) return set_err(GSS_E_INVAL, "null plane");
+    const int W = e->words;
+    const int64_t V = e->vars;
+    // stage per block of variables
+    const int64_t blk = 1 << 16;
+    Pinned<uint64_t> st;
+    if ((rc = st.reserve((size_t)blk * 2 * W))) return rc;
+    for (int64_t v0 = 0; v0 < V; v0 += blk) {
+        const int64_t nv = std::min(blk, V - v0);
+        for (int64_t v = 0; v < nv; v++) {
+            for (int w = 0; w < W; w++) {
+                const uint64_t d = is_def[(v0 + v) * W + w], t = is_true[(v0 + v) * W + w];
+                st.p[(v * 2) * W + w] = d & ~t;
+                st.p[(v * 2 + 1) * W + w] = d & t;
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(e->table.p + (size_t)v0 * 2 * W, st.p,
+                               (size_t)nv * 2 * W * sizeof(uint64_t), hipMemcpyHostToDevice,
+                               e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return GSS_OK;
+}
+
+int gss_assig_download(gss_engine *e, uint64_t *is_def, uint64_t *is_true) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!is_def || !is_true) return set_err(GSS_E_INVAL, "null plane");
+    const int W = e->words;
+    const int64_t V = e->vars;
+    std::vector<uint64_t> t((size_t)V * 2 * W);
+    if (V)
+        HIP_TRY(hipMemcpyAsync(t.data(), e->table.p, t.size() * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int64_t v = 0; v < V; v++)
+        for (int w = 0; w < W; w++) {
+            const uint64_t f = t[(v * 2) * W + w], tr = t[(v * 2 + 1) * W + w];
+            is_def[v * W + w] = f | tr;
+            is_true[v * W + w] = tr;
+        }
+    return GSS_OK;
+}
+
+static int upload_updates(gss_engine *e, int64_t n, const gss_var_update *ups) {
+    int rc;
+    for (int64_t i = 0; i < n; i++)
+        if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)e->words)
+            return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
+                           (long long)i, ups[i].var, ups[i].word);
+    if ((rc = e->h_ups.reserve((size_t)n)) || (rc = e->d_ups.reserve((size_t)n))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::memcpy(e->h_ups.p, ups, (size_t)n * sizeof(gss_var_update));
+    HIP_TRY(hipMemcpyAsync(e->d_ups.p, e->h_ups.p, (size_t)n * sizeof(gss_var_update),
+                           hipMemcpyHostToDevice, e->stream));
+    return GSS_OK;
+}
+
+int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (n < 0 || (n && !ups)) return set_err(GSS_E_INVAL, "bad update batch");
+    if (!n) return GSS_OK;
+    if ((rc = upload_updates(e, n, ups))) return rc;
+    HIP_TRY(launch_assig_apply(e->d_ups.p, n, e->table.p, e->words, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_assig_roll_forward(gss_engine *e, int64_t n, const gss_var_update *ups,
+                           const int8_t *last_pos) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (n < 0 || (n && (!ups || !last_pos))) return set_err(GSS_E_INVAL, "bad roll-forward");
+    if (!n) return GSS_OK;
+    for (int w = 0; w < e->words; w++)
+        if (last_pos[w] > 63) return set_err(GSS_E_INVAL, "last_pos[%d] = %d > 63", w, last_pos[w]);
+    if ((rc = upload_updates(e, n, ups))) return rc;
+    std::memcpy(e->h_lastpos.p, last_pos, e->words);
+    HIP_TRY(hipMemcpyAsync(e->d_lastpos, e->h_lastpos.p, e->words, hipMemcpyHostToDevice,
+                           e->stream));
+    HIP_TRY(launch_assig_roll(e->d_ups.p, n, e->table.p, e->words, e->d_lastpos, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+/* ---- sweep ---------------------------------------------------------------- */
+
+int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!valid) return set_err(GSS_E_INVAL, "null valid mask");
+    if (e->pending) return set_err(GSS_E_STATE, "previous sweep not waited for");
+    if (e->total_clauses > 0 && e->max_lit >= 2 * e->vars)
+        return set_err(GSS_E_STATE, "clause literal %lld needs variable %lld but the table has "
+                                    "%lld variables",
+                       (long long)e->max_lit, (long long)(e->max_lit >> 1), (long long)e->vars);
+    const int W = e->words;
+    const int cpp = sweep_clauses_per_pass(W);
+    int tc = e->tune_task_clauses > 0 ? e->tune_task_clauses : 16 * cpp;
+    tc = std::max(cpp, (tc / cpp) * cpp);
+    e->task_clauses = tc;
+    // descriptors of non-empty buckets
+    int nb = 0;
+    for (size_t len = 1; len < e->buckets.size(); len++)
+        if (e->buckets[len].count) nb++;
+    if ((rc = e->h_desc.reserve(std::max(nb, 1))) || (rc = e->d_desc.reserve(std::max(nb, 1))))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));  // pinned descriptors / valid reused
+    int64_t tasks = 0;
+    int k = 0;
+    for (size_t len = 1; len < e->buckets.size(); len++) {
+        const Bucket &b = e->buckets[len];
+        if (!b.count) continue;
+        if (b.count > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
+        BucketDesc &d = e->h_desc.p[k++];
+        d.lits = b.d;
+        d.len = (uint32_t)len;
+        d.count = (uint32_t)b.count;
+        d.task_begin = (uint32_t)tasks;
+        d.pad = 0;
+        tasks += (b.count + tc - 1) / tc;
+    }
+    if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
+    e->nbuckets = nb;
+    e->ntasks = (int)tasks;
+    int active_words = 0;
+    for (int w = 0; w < W; w++) {
+        e->h_valid.p[w] = valid[w];
+        if (valid[w]) active_words++;
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_valid, e->h_valid.p, W * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           e->stream));
+    if (nb)
+        HIP_TRY(hipMemcpyAsync(e->d_desc.p, e->h_desc.p, nb * sizeof(BucketDesc),
+                               hipMemcpyHostToDevice, e->stream));
+    // persistent grid
+    const int li = log2_exact(W);
+    if (!e->occ[li]) e->occ[li] = sweep_occupancy(W);
+    int per_cu = e->occ[li];
+    if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
+    int64_t blocks = (int64_t)e->cu * per_cu;
+    const int64_t need_blocks = (tasks + 3) / 4;
+    blocks = std::max<int64_t>(1, std::min(blocks, need_blocks));
+    e->blocks = (int)blocks;
+    // Hit capacity: an abandoned chunk holds more than kHitChunk - 64 records
+    // and each wave leaves at most one partial chunk, so this bound is never
+    // exceeded while records <= clauses * active words.
+    const uint64_t ub = (uint64_t)e->total_clauses * (uint64_t)active_words;
+    uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        // never take more than a quarter of the free memory; beyond that the
+        // wait re-runs with exactly what the claims asked for
+        const uint64_t budget =
+            e->chunk_cap + (uint64_t)(free_b / 4) / (kHitChunk * sizeof(gss_hit));
+        chunks = std::min(chunks, std::max<uint64_t>(budget, 1));
+    }
+    if (e->tune_max_chunks > 0) chunks = std::min<uint64_t>(chunks, e->tune_max_chunks);
+    if (chunks > e->chunk_cap && (rc = reserve_chunks(e, chunks))) return rc;
+    if ((rc = launch_current_sweep(e))) return rc;
+    e->pending = true;
+    e->waited = false;
+    e->have_sweep = true;
+    return GSS_OK;
+}
+
+int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->pending) return set_err(GSS_E_STATE, "no sweep in flight");
+    for (;;) {
+        HIP_TRY(hipMemcpyAsync(e->h_small.p, e->d_alloc, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        const uint32_t used = e->h_small.p[0];
+        if (used <= e->chunk_cap) {
+            e->last_chunks = used;
+            break;
+        }
+        // The buffer was too small: grow to what the claims asked for, re-run.
+        if ((rc = reserve_chunks(e, (uint64_t)used + used / 8 + 16))) {
+            e->pending = false;
+            return rc;
+        }
+        if ((rc = launch_current_sweep(e))) {
+            e->pending = false;
+            return rc;
+        }
+    }
+    e->pending = false;
+    e->waited = true;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms = ms;
+    e->total_ms += ms;
+    e->nsweeps++;
+    e->fills.resize(e->last_chunks);
+    if (e->last_chunks)
+        HIP_TRY(hipMemcpy(e->fills.data(), e->d_fills.p, e->last_chunks * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (uint32_t f : e->fills) tot += f;
+    e->last_hits = tot;
+    if (n_hits) *n_hits = tot;
+    return GSS_OK;
+}
+
+int gss_sweep(gss_engine *e, const uint64_t *valid, int64_t *n_hits) {
+    int rc = gss_sweep_async(e, valid);
+    if (rc) return rc;
+    return gss_sweep_wait(e, n_hits);
+}
+
+int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->waited) return set_err(GSS_E_STATE, "no waited sweep");
+    if (n) *n = e->last_hits;
+    const int64_t want = std::min<int64_t>(cap, e->last_hits);
+    if (want <= 0) return GSS_OK;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    // Compact the claimed chunks into a dense device buffer, then one D2H.
+    const uint32_t nc = e->last_chunks;
+    std::vector<uint64_t> offs(nc);
+    int64_t acc = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        offs[c] = (uint64_t)acc;
+        acc += e->fills[c];
+    }
+    if ((rc = e->d_dense.reserve((size_t)e->last_hits))) return rc;
+    // segment copies as uint32 words through the scatter kernel
+    if ((rc = e->h_segs.reserve(nc)) || (rc = e->d_segs.reserve(nc))) return rc;
+    int ns = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        if (!e->fills[c]) continue;
+        ScatterSeg &sg = e->h_segs.p[ns++];
+        sg.dst = (uint32_t *)(e->d_dense.p + offs[c]);
+        sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
+        sg.count = (uint64_t)e->fills[c] * (sizeof(gss_hit) / 4);
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, ns * sizeof(ScatterSeg),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(launch_scatter(e->d_segs.p, ns, (const uint32_t *)e->d_hits.p, e->stream));
+    HIP_TRY(hipMemcpyAsync(out, e->d_dense.p, (size_t)want * sizeof(gss_hit),
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_sweep_kernel_ms(gss_engine *e, float *ms) {
+    if (!e || !ms) return set_err(GSS_E_INVAL, "null argument");
+    *ms = e->last_ms;
+    return GSS_OK;
+}
+
+int gss_sweep_kernel_stats(gss_engine *e, double *total_ms, int64_t *count, int reset) {
+    if (!e) return set_err(GSS_E_INVAL, "null engine");
+    if (total_ms) *total_ms = e->total_ms;
+    if (count) *count = e->nsweeps;
+    if (reset) {
+        e->total_ms = 0;
+        e->nsweeps = 0;
+    }
+    return GSS_OK;
+}
+
+int gss_set_tuning(gss_engine *e, int key, int value) {
+    if (!e) return set_err(GSS_E_INVAL, "null engine");
+    if (value < 0) return set_err(GSS_E_INVAL, "negative tuning value");
+    switch (key) {
+        case GSS_TUNE_WAVES_PER_CU: e->tune_waves_per_cu = value; return GSS_OK;
+        case GSS_TUNE_TASK_CLAUSES: e->tune_task_clauses = value; return GSS_OK;
+        case GSS_TUNE_MAX_HIT_CHUNKS:
+            e->tune_max_chunks = value;
+            if (value > 0) {
+                // drop the current buffer so the next sweep starts this small
+                e->d_hits.release();
+                e->d_fills.release();
+                e->chunk_cap = 0;
+            }
+            return GSS_OK;
+        default: return set_err(GSS_E_INVAL, "unknown tuning key %d", key);
+    }
+}
+
+}  // extern "C"

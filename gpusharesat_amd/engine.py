@@ -1,0 +1,219 @@
+"""Python handle on the device check engine (include/gss.h).
+
+This is the thin host-side mirror used by the tests and the benchmark; the
+solver-facing interface is the C++ GpuShare::GpuClauseSharer in
+include/gpuShareLib/GpuClauseSharer.h (Python mirror: gpusharesat_amd.sharer).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+HIT_DTYPE = np.dtype([("row", "<u4"), ("len", "<u2"), ("word", "<u2"),
+                      ("false_mask", "<u8"), ("unit_mask", "<u8")])
+VAR_UPDATE_DTYPE = np.dtype([("var", "<u4"), ("word", "<u4"),
+                             ("is_def", "<u8"), ("is_true", "<u8")])
+assert HIT_DTYPE.itemsize == 24 and VAR_UPDATE_DTYPE.itemsize == 24
+
+TUNE_WAVES_PER_CU = 1
+TUNE_TASK_CLAUSES = 2
+TUNE_MAX_HIT_CHUNKS = 3
+MAX_WORDS = 64
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def round_words(words: int) -> int:
+    """Table row width used for `words` assignment words (a power of two)."""
+    w = 1
+    while w < words:
+        w *= 2
+    if w > MAX_WORDS:
+        raise ValueError(f"at most {MAX_WORDS * 64} assignment slots per sweep")
+    return w
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        _lib.call("gss_engine_create", device, ctypes.byref(self._h))
+        self.words = 1
+        self.vars = 0
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            _lib.call("gss_engine_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def cu_count(self) -> int:
+        v = ctypes.c_int()
+        _lib.call("gss_engine_cu_count", self._h, ctypes.byref(v))
+        return v.value
+
+    def sync(self) -> None:
+        _lib.call("gss_engine_sync", self._h)
+
+    # -- clause DB ------------------------------------------------------------
+    def append(self, lits: np.ndarray, lens: np.ndarray) -> np.ndarray:
+        """Appends clauses (CSR-free: lens[i] literals each, concatenated).
+        Returns each clause's row inside its length bucket."""
+        lits = np.ascontiguousarray(lits, dtype=np.uint32)
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        if int(lens.sum(dtype=np.int64)) != lits.size:
+            raise ValueError("sum(lens) != len(lits)")
+        rows = np.empty(lens.size, dtype=np.uint32)
+        _lib.call("gss_db_append", self._h, lens.size, _ptr(lits), _ptr(lens), _ptr(rows))
+        return rows
+
+    def set_bucket(self, length: int, lits: np.ndarray) -> None:
+        lits = np.ascontiguousarray(lits, dtype=np.uint32)
+        n = lits.size // length
+        _lib.call("gss_db_set_bucket", self._h, length, n, _ptr(lits))
+
+    def bucket_count(self, length: int) -> int:
+        v = ctypes.c_int64()
+        _lib.call("gss_db_bucket_count", self._h, length, ctypes.byref(v))
+        return v.value
+
+    def totals(self) -> tuple[int, int]:
+        c, l = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("gss_db_totals", self._h, ctypes.byref(c), ctypes.byref(l))
+        return c.value, l.value
+
+    def read_bucket(self, length: int, row0: int = 0, n: int | None = None) -> np.ndarray:
+        if n is None:
+            n = self.bucket_count(length) - row0
+        out = np.empty(n * length, dtype=np.uint32)
+        _lib.call("gss_db_read", self._h, length, row0, n, _ptr(out))
+        return out.reshape(n, length)
+
+    def clear(self) -> None:
+        _lib.call("gss_db_clear", self._h)
+
+    # -- assignment table -----------------------------------------------------
+    def reset_assignments(self, var_count: int, words: int) -> None:
+        w = round_words(words)
+        _lib.call("gss_assig_reset", self._h, var_count, w)
+        self.words, self.vars = w, var_count
+
+    def grow_vars(self, var_count: int) -> None:
+        _lib.call("gss_assig_grow_vars", self._h, var_count)
+        self.vars = var_count
+
+    def upload(self, is_def: np.ndarray, is_true: np.ndarray) -> None:
+        """Planes are [vars, words] uint64 (words <= table width; missing words
+        are undefined)."""
+        d = self._pad(is_def)
+        t = self._pad(is_true)
+        _lib.call("gss_assig_upload", self._h, _ptr(d), _ptr(t))
+
+    def _pad(self, plane: np.ndarray) -> np.ndarray:
+        plane = np.asarray(plane, dtype=np.uint64)
+        if plane.shape[0] != self.vars:
+            raise ValueError("plane has wrong variable count")
+        if plane.shape[1] == self.words:
+            return np.ascontiguousarray(plane)
+        out = np.zeros((self.vars, self.words), dtype=np.uint64)
+        out[:, :plane.shape[1]] = plane
+        return out
+
+    def download(self) -> tuple[np.ndarray, np.ndarray]:
+        d = np.empty((self.vars, self.words), dtype=np.uint64)
+        t = np.empty((self.vars, self.words), dtype=np.uint64)
+        _lib.call("gss_assig_download", self._h, _ptr(d), _ptr(t))
+        return d, t
+
+    def apply(self, ups: np.ndarray) -> None:
+        ups = np.ascontiguousarray(ups, dtype=VAR_UPDATE_DTYPE)
+        _lib.call("gss_assig_apply", self._h, ups.size, _ptr(ups))
+
+    def roll_forward(self, ups: np.ndarray, last_pos) -> None:
+        ups = np.ascontiguousarray(ups, dtype=VAR_UPDATE_DTYPE)
+        lp = np.full(self.words, -1, dtype=np.int8)
+        lp[:len(last_pos)] = last_pos
+        _lib.call("gss_assig_roll_forward", self._h, ups.size, _ptr(ups), _ptr(lp))
+
+    # -- sweep ----------------------------------------------------------------
+    def _valid(self, valid) -> np.ndarray:
+        v = np.zeros(self.words, dtype=np.uint64)
+        valid = np.asarray(valid, dtype=np.uint64).ravel()
+        if valid.size > self.words:
+            raise ValueError("more valid words than table words")
+        v[:valid.size] = valid
+        return v
+
+    def sweep_async(self, valid) -> None:
+        self._valid_buf = self._valid(valid)
+        _lib.call("gss_sweep_async", self._h, _ptr(self._valid_buf))
+
+    def sweep_wait(self) -> int:
+        n = ctypes.c_int64()
+        _lib.call("gss_sweep_wait", self._h, ctypes.byref(n))
+        return n.value
+
+    def sweep(self, valid) -> int:
+        v = self._valid(valid)
+        n = ctypes.c_int64()
+        _lib.call("gss_sweep", self._h, _ptr(v), ctypes.byref(n))
+        return n.value
+
+    def hits(self) -> np.ndarray:
+        n = ctypes.c_int64()
+        _lib.call("gss_hits_copy", self._h, None, 0, ctypes.byref(n))
+        out = np.empty(n.value, dtype=HIT_DTYPE)
+        if n.value:
+            _lib.call("gss_hits_copy", self._h, _ptr(out), n.value, ctypes.byref(n))
+        return out
+
+    def kernel_ms(self) -> float:
+        v = ctypes.c_float()
+        _lib.call("gss_sweep_kernel_ms", self._h, ctypes.byref(v))
+        return v.value
+
+    def kernel_stats(self, reset: bool = False) -> tuple[float, int]:
+        t, c = ctypes.c_double(), ctypes.c_int64()
+        _lib.call("gss_sweep_kernel_stats", self._h, ctypes.byref(t), ctypes.byref(c), int(reset))
+        return t.value, c.value
+
+    def set_tuning(self, key: int, value: int) -> None:
+        _lib.call("gss_set_tuning", self._h, key, value)
+
+
+class ClauseMap:
+    """Maps (bucket length, row) of hit records back to caller clause ids."""
+
+    def __init__(self, lens: np.ndarray, rows: np.ndarray, ids: np.ndarray | None = None):
+        lens = np.asarray(lens, dtype=np.int64)
+        rows = np.asarray(rows, dtype=np.int64)
+        ids = np.arange(lens.size, dtype=np.int64) if ids is None else np.asarray(ids, np.int64)
+        maxlen = int(lens.max()) if lens.size else 0
+        counts = np.bincount(lens, minlength=maxlen + 2)
+        self.base = np.zeros(maxlen + 2, dtype=np.int64)
+        self.base[1:] = np.cumsum(counts)[:-1]
+        self.table = np.full(lens.size, -1, dtype=np.int64)
+        self.table[self.base[lens] + rows] = ids
+
+    def ids(self, hits: np.ndarray) -> np.ndarray:
+        return self.table[self.base[hits["len"].astype(np.int64)] + hits["row"].astype(np.int64)]

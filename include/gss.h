@@ -1,0 +1,156 @@
+/*
+ * gss.h -- C ABI of the MI355X clause-vs-assignment check engine.
+ *
+ * This is the thin, torch-free device-module layer under the C++
+ * GpuShare::GpuClauseSharer implementation (include/gpuShareLib/GpuClauseSharer.h).
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (0 = ok, negative = error, see GSS_E_*), and never throws.  The message of the
+ * last error on the calling thread is available from gss_last_error().
+ *
+ * Reference interfaces each group replaces (GpuShareSat tree, file:line):
+ *   engine / stream     gpuShareLib/GpuUtils.cuh:33-67 (StreamPointer/EventPointer),
+ *                       gpuShareLib/GpuClauseSharerImpl.cu:40-65 (device set-up)
+ *   clause DB           gpuShareLib/Clauses.cu:128-145,200-217,318-356
+ *                       (PerSizeKeeper / DClauses::update / initClauses),
+ *                       Clauses.cu:249-282 (full bucket re-upload after reduceDb)
+ *   assignment table    gpuShareLib/Assigs.cu:100-116 (dUpdateAssigs),
+ *                       Assigs.cu:127-141 (dSetAllAssigsToLast),
+ *                       Assigs.cu:326-372 (fillAssigsAsync)
+ *   sweep               gpuShareLib/GpuRunner.cu:34-177 (ReportComputer,
+ *                       dFindClauses), GpuRunner.cu:323-345 (launchGpu)
+ *   hits                gpuShareLib/Reporter.cuh:41-58,77-127 (DReporter /
+ *                       Reporter::getCopiedToHost), GpuRunner.cu:347-383
+ *
+ * Data model
+ *   literal    uint32 2*var + sign (sign 1 = negative), as the reference's Lit.
+ *   clause DB  clauses live in length buckets; a clause is addressed by
+ *              (len, row) where row is its position in bucket `len`.
+ *   assignment table
+ *              `words` uint64 words per literal; bit b of word w is assignment
+ *              slot 64*w + b.  Internally the row of literal l holds the mask of
+ *              slots where l is false; gss_assig_* take the reference's
+ *              per-variable isDef/isTrue view.
+ *   hit        one record per (clause, word) whose false or unit mask is
+ *              nonzero; nothing is ever dropped.
+ */
+#ifndef GSS_H
+#define GSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSS_OK 0
+#define GSS_E_INVAL (-1)    /* invalid argument */
+#define GSS_E_NOMEM (-2)    /* device or host allocation failed / over budget */
+#define GSS_E_HIP (-3)      /* HIP runtime error */
+#define GSS_E_STATE (-4)    /* call not valid in the current state */
+
+#define GSS_MAX_WORDS 64    /* max uint64 words per literal row (4096 slots) */
+#define GSS_MAX_CLAUSE_LEN 65535
+
+typedef struct gss_engine gss_engine;
+
+typedef struct gss_hit {
+    uint32_t row;        /* row within the length bucket */
+    uint16_t len;        /* clause length = bucket */
+    uint16_t word;       /* assignment word */
+    uint64_t false_mask; /* slots where every literal is false */
+    uint64_t unit_mask;  /* slots where exactly one literal is undefined, rest false */
+} gss_hit;
+
+typedef struct gss_var_update {
+    uint32_t var;
+    uint32_t word;
+    uint64_t is_def;     /* slots where the variable is assigned */
+    uint64_t is_true;    /* slots where it is true (ignored where !is_def) */
+} gss_var_update;
+
+/* Last error message of the calling thread ("" if none). */
+const char *gss_last_error(void);
+
+/* ---- engine ---------------------------------------------------------------
+ * Replaces the reference's device set-up + single stream
+ * (GpuClauseSharerImpl.cu:40-65, GpuClauseSharerImpl.cuh:34). */
+int gss_engine_create(int device, gss_engine **out);
+int gss_engine_destroy(gss_engine *e);
+/* Number of compute units of the engine's device. */
+int gss_engine_cu_count(gss_engine *e, int *out);
+/* Blocks until all work queued on the engine's stream is done. */
+int gss_engine_sync(gss_engine *e);
+
+/* ---- clause DB -------------------------------------------------------------
+ * Append n clauses (lens[i] literals each, concatenated in lits).  Rows are
+ * assigned in order within each length bucket and returned in out_rows (may be
+ * NULL).  Replaces HostClauses::getUpdatesForDevice + initClauses
+ * (Clauses.cu:318-346, GpuRunner.cu:64-66,314-316): one H2D copy of the packed
+ * batch and one scatter kernel. */
+int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits,
+                  const int32_t *lens, uint32_t *out_rows);
+/* Replace the whole content of bucket `len` by n rows (n*len literals);
+ * replaces the full re-upload after compaction (Clauses.cu:249-282). */
+int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits);
+/* Number of rows in bucket `len`. */
+int gss_db_bucket_count(gss_engine *e, int len, int64_t *out);
+/* Totals over all buckets. */
+int gss_db_totals(gss_engine *e, int64_t *clauses, int64_t *literals);
+/* Copy rows [row0, row0+n) of bucket len to host (n*len literals). */
+int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out);
+/* Remove every clause. */
+int gss_db_clear(gss_engine *e);
+
+/* ---- assignment table ------------------------------------------------------
+ * (Re)shape the table to var_count variables x words words; every slot of
+ * every variable becomes undefined.  words must be in [1, GSS_MAX_WORDS]. */
+int gss_assig_reset(gss_engine *e, int64_t var_count, int words);
+/* Grow to var_count variables keeping existing rows (new ones undefined). */
+int gss_assig_grow_vars(gss_engine *e, int64_t var_count);
+/* Upload whole planes, var-major: is_def[var*words + w], is_true[...]. */
+int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_true);
+/* Download whole planes (same layout).  Synchronous. */
+int gss_assig_download(gss_engine *e, uint64_t *is_def, uint64_t *is_true);
+/* Apply n per-(var, word) updates (replaces dUpdateAssigs, Assigs.cu:100-116).
+ * At most one update per (var, word) per call. */
+int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups);
+/* For each (var, word) of ups, set every slot of that word to the value held
+ * by slot last_pos[word] (replaces dSetAllAssigsToLast, Assigs.cu:127-141).
+ * Words with last_pos < 0 are left alone. */
+int gss_assig_roll_forward(gss_engine *e, int64_t n, const gss_var_update *ups,
+                           const int8_t *last_pos);
+
+/* ---- sweep -----------------------------------------------------------------
+ * Check every clause against every slot set in valid[w] (w < words) and
+ * record the hits on the device.  Replaces launchGpu/dFindClauses
+ * (GpuRunner.cu:134-177,323-345).  Asynchronous on the engine's stream. */
+int gss_sweep_async(gss_engine *e, const uint64_t *valid);
+/* Wait for the last sweep; returns its number of hit records.  If the hit
+ * buffer was too small the sweep is re-run with a larger one (no drops). */
+int gss_sweep_wait(gss_engine *e, int64_t *n_hits);
+/* gss_sweep_async + gss_sweep_wait. */
+int gss_sweep(gss_engine *e, const uint64_t *valid, int64_t *n_hits);
+/* Copy the hit records of the last (waited) sweep to host: up to cap records,
+ * *n = total.  Record order is unspecified. */
+int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
+/* Duration in ms of the sweep kernel of the last waited sweep, measured with
+ * HIP events recorded on the engine's stream around the kernel launch. */
+int gss_sweep_kernel_ms(gss_engine *e, float *ms);
+/* Sum of sweep-kernel durations (ms) and count since the last reset. */
+int gss_sweep_kernel_stats(gss_engine *e, double *total_ms, int64_t *count, int reset);
+
+/* Tuning knobs (0 = default).  Exposed for the autotuner and the tests;
+ * results never depend on them. */
+int gss_set_tuning(gss_engine *e, int key, int value);
+#define GSS_TUNE_WAVES_PER_CU 1   /* persistent waves per CU (0 = occupancy) */
+#define GSS_TUNE_TASK_CLAUSES 2   /* clauses per task */
+#define GSS_TUNE_MAX_HIT_CHUNKS 3 /* cap on the hit buffer a sweep starts with
+                                     (0 = sized to the worst case); a sweep
+                                     that needs more re-runs with a larger one */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSS_H */
