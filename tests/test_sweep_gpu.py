@@ -265,8 +265,13 @@ def test_assignment_apply_and_roll_forward(engine_factory):
 def test_invalid_arguments_are_errors(engine_factory):
     from gpusharesat_amd._lib import GssError
     e = engine_factory()
-    with pytest.raises(GssError):
-        e.append(np.array([0], np.uint32), np.array([0], np.int32))
+    with pytest.raises(GssError):  # empty clause
+        e.append(np.array([], np.uint32), np.array([0], np.int32))
+    with pytest.raises(GssError):  # literal beyond the table's variables
+        e.reset_assignments(4, 1)
+        e.append(np.array([8, 9], np.uint32), np.array([2], np.int32))
+        e.sweep([ALL])
+    e.clear()
     from gpusharesat_amd import _lib
     with pytest.raises(GssError):
         _lib.call("gss_assig_reset", e.handle, 10, 3)  # not a power of two
