@@ -47,6 +47,8 @@ SIGNATURES: dict[str, list] = {
     "gss_assig_download": [c_p, c_p, c_p],
     "gss_assig_apply": [c_p, c_i64, c_p],
     "gss_assig_roll_forward": [c_p, c_i64, c_p, c_p],
+    "gss_assig_roll_forward_applied": [c_p, c_p],
+    "gss_engine_mem_info": [c_p, c_p, c_p],
     "gss_sweep_async": [c_p, c_p],
     "gss_sweep_wait": [c_p, c_p],
     "gss_sweep": [c_p, c_p, c_p],
@@ -54,6 +56,34 @@ SIGNATURES: dict[str, list] = {
     "gss_sweep_kernel_ms": [c_p, c_p],
     "gss_sweep_kernel_stats": [c_p, c_p, c_p, c_int],
     "gss_set_tuning": [c_p, c_int, c_int],
+    # include/gss_sharer.h
+    "gss_sharer_default_options": [c_p],
+    "gss_sharer_create": [c_p, c_p, c_p, c_p],
+    "gss_sharer_destroy": [c_p],
+    "gss_sharer_gpu_run": [c_p],
+    "gss_sharer_reduce_db": [c_p],
+    "gss_sharer_get_added_clause_count": [c_p, c_p],
+    "gss_sharer_get_added_clause_count_at_last_reduce_db": [c_p, c_p],
+    "gss_sharer_has_run_out_of_gpu_memory_once": [c_p, c_p],
+    "gss_sharer_get_gpu_mem_info": [c_p, c_p, c_p],
+    "gss_sharer_get_global_stat_count": [c_p, c_p],
+    "gss_sharer_get_global_stat": [c_p, c_int, c_p],
+    "gss_sharer_write_clauses_in_cnf": [c_p, c_char_p],
+    "gss_sharer_set_var_count": [c_p, c_int],
+    "gss_sharer_add_clause": [c_p, c_int, c_p, c_int, c_p],
+    "gss_sharer_set_cpu_solver_count": [c_p, c_int],
+    "gss_sharer_get_one_solver_stat_name": [c_p, c_int, c_p],
+    "gss_sharer_get_global_stat_name": [c_p, c_int, c_p],
+    "gss_sharer_get_one_solver_stat_count": [c_p, c_p],
+    "gss_sharer_try_set_solver_values": [c_p, c_int, c_p, c_int, c_p],
+    "gss_sharer_unset_solver_values": [c_p, c_int, c_p, c_int],
+    "gss_sharer_try_send_assignment": [c_p, c_int, c_p],
+    "gss_sharer_pop_reported_clause": [c_p, c_int, c_p, c_p, c_p, c_p],
+    "gss_sharer_get_last_assig_all_reported": [c_p, c_int, c_p],
+    "gss_sharer_get_current_assignment": [c_p, c_int, c_p],
+    "gss_sharer_get_one_solver_stat": [c_p, c_int, c_int, c_p],
+    "gss_sharer_limits": [c_p, c_p, c_p],
+    "gss_sharer_debug_clause": [c_p, c_i64, c_p, c_p, c_p],
 }
 _RESTYPES = {"gss_last_error": c_char_p}
 

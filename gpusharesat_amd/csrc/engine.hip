@@ -25,12 +25,17 @@
 #include <string>
 #include <vector>
 
+#include "errors.h"
 #include "gss.h"
 #include "kernels.h"
 
 using namespace gss;
 
 namespace {
+
+// Clauses of at most this many literals read both rows of each literal in one
+// pass (see kernels.hip).
+constexpr int kDefaultFuseLen = 4;
 
 thread_local std::string g_err;
 
@@ -95,7 +100,8 @@ struct DevBuf {
             if (keep && cap) {
                 HIP_TRY(hipMemcpyAsync(np, p, cap * sizeof(T), hipMemcpyDeviceToDevice, s));
             }
-            HIP_TRY(hipStreamSynchronize(s));
+            // the old buffer may still be read by queued work on any stream
+            HIP_TRY(hipDeviceSynchronize());
             (void)hipFree(p);
         }
         p = np;
@@ -110,6 +116,11 @@ struct DevBuf {
 };
 
 }  // namespace
+
+int gss::record_error(int code, const char *msg) {
+    g_err = msg ? msg : "";
+    return code;
+}
 
 struct gss_engine {
     int device = 0;
@@ -147,12 +158,14 @@ struct gss_engine {
     float last_ms = 0.f;
     double total_ms = 0.0;
     int64_t nsweeps = 0;
-    int occ[7] = {0, 0, 0, 0, 0, 0, 0};
+    int occ[7][9] = {};  // [log2 words][unroll] resident blocks per CU
 
     // tuning
     int tune_waves_per_cu = 0;
     int tune_task_clauses = 0;
     int tune_max_chunks = 0;
+    int tune_unroll = 0;
+    int tune_fuse_len = 0;
 
     // staging
     Pinned<uint32_t> h_stage;
@@ -163,6 +176,7 @@ struct gss_engine {
     DevBuf<gss_var_update> d_ups;
     int8_t *d_lastpos = nullptr;
     Pinned<int8_t> h_lastpos;
+    int64_t applied = 0;  // updates of the last gss_assig_apply, still in d_ups
     DevBuf<uint32_t> d_offsets;
     DevBuf<gss_hit> d_dense;
     Pinned<gss_hit> h_hits;
@@ -215,12 +229,14 @@ int launch_current_sweep(gss_engine *e) {
         a.ntasks = e->ntasks;
         a.task_clauses = e->task_clauses;
         a.table = e->table.p;
+        a.sentinel = (uint32_t)(2 * e->vars);
         a.valid = e->d_valid;
         a.hits = e->d_hits.p;
         a.fills = e->d_fills.p;
         a.alloc = e->d_alloc;
         a.chunk_cap = e->chunk_cap;
-        HIP_TRY(launch_sweep(a, e->words, e->blocks, e->stream));
+        a.fuse_len = e->tune_fuse_len > 0 ? e->tune_fuse_len : kDefaultFuseLen;
+        HIP_TRY(launch_sweep(a, e->words, e->tune_unroll, e->blocks, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     return GSS_OK;
@@ -312,6 +328,16 @@ int gss_engine_sync(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_engine_mem_info(gss_engine *e, size_t *free_bytes, size_t *total_bytes) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
     return GSS_OK;
 }
 
@@ -451,38 +477,65 @@ int gss_db_clear(gss_engine *e) {
 
 /* ---- assignment table ----------------------------------------------------- */
 
+// The table has 2 * (vars + 1) rows: rows 2v, 2v+1 for each variable, then
+// two sentinel rows.  Row 2*vars is all ones ("false in every slot") and row
+// 2*vars + 1 is all zeros ("true in no slot"); the sweep reads them for
+// padding lanes past the end of a clause, so those lanes need no masking.
+static size_t table_rows(int64_t vars) { return (size_t)(vars + 1) * 2; }
+
+static int table_fits(int64_t vars, int words) {
+    // the sweep addresses the table with 32-bit byte offsets
+    if ((uint64_t)table_rows(vars) * (uint64_t)words * 8ull > 0xFFFFFFFFull)
+        return set_err(GSS_E_INVAL, "assignment table of %lld vars x %d words exceeds 4 GiB",
+                       (long long)vars, words);
+    return GSS_OK;
+}
+
+static int write_sentinels(gss_engine *e) {
+    HIP_TRY(launch_fill_u64(e->table.p + (size_t)e->vars * 2 * e->words, e->words, ~0ull,
+                            e->stream));
+    HIP_TRY(launch_fill_u64(e->table.p + ((size_t)e->vars * 2 + 1) * e->words, e->words, 0ull,
+                            e->stream));
+    return GSS_OK;
+}
+
 int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
     int rc = check_engine(e);
     if (rc) return rc;
-    if (var_count < 0 || var_count > (1ll << 31) - 1 || words < 1 || words > GSS_MAX_WORDS ||
+    if (var_count < 0 || var_count > (1ll << 31) - 2 || words < 1 || words > GSS_MAX_WORDS ||
         log2_exact(words) < 0)
         return set_err(GSS_E_INVAL, "bad table shape (%lld vars, %d words; words must be a power "
                                     "of two <= %d)",
                        (long long)var_count, words, GSS_MAX_WORDS);
+    if ((rc = table_fits(var_count, words))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    const size_t n = (size_t)std::max<int64_t>(var_count, 1) * 2 * words;
+    const size_t n = table_rows(var_count) * words;
     if (n > e->table.cap) {
         e->table.release();
         if ((rc = e->table.reserve(n))) return rc;
     }
     HIP_TRY(launch_fill_u64(e->table.p, (int64_t)n, 0ull, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
     e->vars = var_count;
     e->words = words;
+    if ((rc = write_sentinels(e))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }
 
 int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
     int rc = check_engine(e);
     if (rc) return rc;
-    if (var_count < e->vars || var_count > (1ll << 31) - 1)
+    if (var_count < e->vars || var_count > (1ll << 31) - 2)
         return set_err(GSS_E_INVAL, "variable count can only grow");
+    if ((rc = table_fits(var_count, e->words))) return rc;
+    // rows of the old sentinels become the first new variable: clear from there
     const size_t old_n = (size_t)e->vars * 2 * e->words;
-    const size_t n = (size_t)std::max<int64_t>(var_count, 1) * 2 * e->words;
+    const size_t n = table_rows(var_count) * e->words;
     if ((rc = e->table.reserve(n, true, e->stream))) return rc;
     HIP_TRY(launch_fill_u64(e->table.p + old_n, (int64_t)(n - old_n), 0ull, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
     e->vars = var_count;
+    if ((rc = write_sentinels(e))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }
 
@@ -539,8 +592,9 @@ static int upload_updates(gss_engine *e, int64_t n, const gss_var_update *ups) {
         if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)e->words)
             return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
                            (long long)i, ups[i].var, ups[i].word);
-    if ((rc = e->h_ups.reserve((size_t)n)) || (rc = e->d_ups.reserve((size_t)n))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->applied = 0;
+    if ((rc = e->h_ups.reserve((size_t)n)) || (rc = e->d_ups.reserve((size_t)n))) return rc;
     std::memcpy(e->h_ups.p, ups, (size_t)n * sizeof(gss_var_update));
     HIP_TRY(hipMemcpyAsync(e->d_ups.p, e->h_ups.p, (size_t)n * sizeof(gss_var_update),
                            hipMemcpyHostToDevice, e->stream));
@@ -551,10 +605,31 @@ int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups) {
     int rc = check_engine(e);
     if (rc) return rc;
     if (n < 0 || (n && !ups)) return set_err(GSS_E_INVAL, "bad update batch");
-    if (!n) return GSS_OK;
+    if (!n) {
+        e->applied = 0;
+        return GSS_OK;
+    }
     if ((rc = upload_updates(e, n, ups))) return rc;
     HIP_TRY(launch_assig_apply(e->d_ups.p, n, e->table.p, e->words, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->applied = n;
+    return GSS_OK;
+}
+
+int gss_assig_roll_forward_applied(gss_engine *e, const int8_t *last_pos) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!last_pos) return set_err(GSS_E_INVAL, "null last_pos");
+    if (!e->applied) return GSS_OK;
+    for (int w = 0; w < e->words; w++)
+        if (last_pos[w] > 63) return set_err(GSS_E_INVAL, "last_pos[%d] = %d > 63", w, last_pos[w]);
+    // h_lastpos is only rewritten after the stream has drained (sweep wait or
+    // the synchronisation in upload_updates), so the async copy is safe
+    std::memcpy(e->h_lastpos.p, last_pos, e->words);
+    HIP_TRY(hipMemcpyAsync(e->d_lastpos, e->h_lastpos.p, e->words, hipMemcpyHostToDevice,
+                           e->stream));
+    HIP_TRY(launch_assig_roll(e->d_ups.p, e->applied, e->table.p, e->words, e->d_lastpos,
+                              e->stream));
+    e->applied = 0;
     return GSS_OK;
 }
 
@@ -627,8 +702,9 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
                                hipMemcpyHostToDevice, e->stream));
     // persistent grid
     const int li = log2_exact(W);
-    if (!e->occ[li]) e->occ[li] = sweep_occupancy(W);
-    int per_cu = e->occ[li];
+    const int ui = std::min(e->tune_unroll, 8);
+    if (!e->occ[li][ui]) e->occ[li][ui] = sweep_occupancy(W, e->tune_unroll);
+    int per_cu = e->occ[li][ui];
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
     int64_t blocks = (int64_t)e->cu * per_cu;
     const int64_t need_blocks = (tasks + 3) / 4;
@@ -762,6 +838,12 @@ int gss_set_tuning(gss_engine *e, int key, int value) {
     switch (key) {
         case GSS_TUNE_WAVES_PER_CU: e->tune_waves_per_cu = value; return GSS_OK;
         case GSS_TUNE_TASK_CLAUSES: e->tune_task_clauses = value; return GSS_OK;
+        case GSS_TUNE_UNROLL:
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+                return set_err(GSS_E_INVAL, "unroll must be 0, 1, 2, 4 or 8");
+            e->tune_unroll = value;
+            return GSS_OK;
+        case GSS_TUNE_FUSE_LEN: e->tune_fuse_len = value; return GSS_OK;
         case GSS_TUNE_MAX_HIT_CHUNKS:
             e->tune_max_chunks = value;
             if (value > 0) {

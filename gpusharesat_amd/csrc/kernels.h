@@ -24,12 +24,15 @@ struct SweepArgs {
     int nbuckets;
     int ntasks;
     int task_clauses;        // clauses per task (multiple of clauses per wave pass)
-    const uint64_t *table;   // [2 * vars][words]: row l = slots where literal l is false
+    const uint64_t *table;   // [2 * vars + 2][words]: row l = slots where literal l is false;
+                             // rows 2*vars (all ones), 2*vars+1 (zeros) are sentinels
     const uint64_t *valid;   // [words]
     gss_hit *hits;           // chunk_cap * kHitChunk records
     uint32_t *fills;         // [chunk_cap] records used in each claimed chunk
     uint32_t *alloc;         // number of chunks claimed (may exceed chunk_cap)
     uint32_t chunk_cap;
+    int fuse_len;            // clauses of at most this many literals use the fused schedule
+    uint32_t sentinel;       // literal whose row is all ones and whose negation's row is zero
 };
 
 struct ScatterSeg {
@@ -38,11 +41,12 @@ struct ScatterSeg {
     uint64_t count;  // uint32 elements
 };
 
-// words is the table row width (power of two, 1..64).
-hipError_t launch_sweep(const SweepArgs &a, int words, int blocks, hipStream_t s);
+// words is the table row width (power of two, 1..64); unroll selects the
+// number of rows in flight per lane for 16 words (0 = default).
+hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
-// Max resident 256-thread blocks per CU for the sweep kernel of this width.
-int sweep_occupancy(int words);
+// Max resident 256-thread blocks per CU for the sweep kernel variant.
+int sweep_occupancy(int words, int unroll);
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, hipStream_t s);

@@ -3,30 +3,41 @@
 // The usefulness rule of the reference (gpuShareLib/GpuRunner.cu:34-57,
 // ReportComputer) reports slot s of a clause when every literal is false
 // ("falsified") or when exactly one literal occurrence is undefined and every
-// other one is false ("unit").  The sweep evaluates it in an equivalent two-pass
-// form that reads one table row per literal:
+// other one is false ("unit").  The sweep evaluates it in an equivalent form
+// that reads table rows, where row l holds the slots in which literal l is
+// false (so literal l is true exactly where row l^1 is set):
 //
-//   pass 1  with f_l = slots where literal l is false, count the non-false
-//           occurrences per slot, saturating at 2:
-//               c2 |= c1 & ~f_l;  c1 |= ~f_l
-//           falsified = ~c1;  one_nonfalse = c1 & ~c2.  A slot with c2 set can
-//           never report, so the clause stops once every slot of every word of
-//           its group has c2 set.
-//   pass 2  (only when some slot has exactly one non-false occurrence) that
-//           occurrence is undefined iff no literal of the clause is true:
-//               unit = one_nonfalse & ~OR_l f_{l^1}
-//           (literal l is true exactly where its negation l^1 is false).
+//   count   c1/c2 = "at least one / at least two non-false occurrences",
+//           saturating:  c2 |= c1 & ~f_l;  c1 |= ~f_l
+//   true    t = OR over the clause of f_{l^1} (some literal is true)
+//   report  falsified = ~c1;  unit = c1 & ~c2 & ~t
 //
-// Duplicate literals are separate occurrences, as in the reference
-// (GpuRunner.cu:68-89 walks the literal list), so "x, x" with x undefined is
-// not unit.  tests/test_oracle.py checks the forms agree by brute force.
+// A slot with c2 set can never report, nor one with t set, so a clause stops
+// once every valid slot of every word of its group is dead.  Two schedules
+// give the same answer:
+//   two-pass  pass 1 reads only rows f_l (1 bit per slot per literal) and
+//             stops on c2; pass 2 reads rows f_{l^1} only if some slot has
+//             exactly one non-false occurrence, and stops when those slots
+//             are resolved;
+//   fused     one pass reads both rows of each literal (adjacent in memory,
+//             rows 2v and 2v+1) and stops on c2 | t.
+// A clause of at most `fuse_len` literals uses the fused schedule, longer
+// ones the two-pass one.  Duplicate literals are separate occurrences, as in
+// the reference (GpuRunner.cu:68-89 walks the literal list), so "x, x" with x
+// undefined is not unit.  tests/test_oracle.py and the GPU parity tests check
+// the forms agree.
 //
 // Work mapping.  A group of G = W*S lanes owns one clause: lane (s, w) handles
-// table word w and literal sub-lane s (S literals are read per step and
-// combined with a butterfly over lanes).  With W = 16 (1024 slots) each
-// literal step is one coalesced 128-byte row read by 16 lanes.  A wave holds
-// 64/G groups; a task is a run of clauses of one length bucket; waves are
-// persistent and take tasks round-robin.
+// table word w and literal sub-lane s.  The group loads G literals of the
+// clause at once (lane q holds literal q of the chunk; the first chunk of the
+// next clause is prefetched while the current one is checked) and hands them
+// out with DPP row broadcasts (G = 16: a group is one 16-lane DPP row) or
+// shuffles; each step a lane issues U row reads (U independent loads in
+// flight), the S sub-lanes' counts are merged with a butterfly, then the group
+// votes on early exit.  With W = 16 (1024 slots) a row read by the 16 lanes of
+// a group is one coalesced 128-byte line.  A wave holds 64/G groups; a task is
+// a run of clauses of one length bucket; waves are persistent and take tasks
+// round-robin.
 //
 // Hits.  A clause/word with a report becomes one 24-byte record.  Each wave
 // compacts its records with a ballot + mbcnt prefix and appends them to a
@@ -37,13 +48,13 @@
 // counter still counts and the host re-runs the sweep.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "kernels.h"
 
 namespace gss {
 
 namespace {
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ uint32_t prefix_count(unsigned long long bal) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
@@ -60,47 +71,154 @@ __device__ __forceinline__ bool group_any(bool pred, int g) {
     }
 }
 
-// Evaluates one clause for this lane's word.  All G lanes of the group call it
-// together (group-uniform control flow).
-template <int W, int S>
-__device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, int len,
-                                             const uint64_t *__restrict__ table, int w, int sub,
-                                             int g, uint64_t valid, uint64_t &fm, uint64_t &um) {
+// Literal J of the group's preloaded chunk, for every lane of the group.
+// With G = 16 a group is one DPP row, so this is a row broadcast (no LDS).
+template <int G, int J>
+__device__ __forceinline__ uint32_t chunk_lit(uint32_t litreg, int gbase, int sub) {
+    if constexpr (G == 16) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)litreg, 0x150 + J, 0xF, 0xF, false);
+    } else {
+        return (uint32_t)__shfl((int)litreg, gbase + J);
+    }
+}
+
+// Row of literal `lit` in word w (32-bit byte offsets: the host keeps the
+// table under 4 GiB).
+template <int W>
+__device__ __forceinline__ uint64_t row_word(const uint64_t *__restrict__ table, uint32_t lit,
+                                             int w) {
+    const uint32_t off = lit * (uint32_t)(W * 8) + (uint32_t)w * 8u;
+    return *(const uint64_t *)((const char *)table + off);
+}
+
+// Clause state of one lane: saturating count of non-false occurrences per slot
+// (c1: >= 1, c2: >= 2) and the slots where some literal is true.
+struct ClauseState {
+    uint64_t c1, c2, any_t;
+};
+
+// One step: U literals per lane.  The chunk holds the group's literals
+// [base, base + G) with the sentinel literal past the clause end, whose rows
+// contribute nothing; this step covers chunk positions [ST, ST + S*U).  The
+// U row reads are issued back to back, folded into the state, and for S > 1
+// the S sub-lanes are merged with a butterfly.
+template <int W, int S, int U, int ST, bool ROWS_F, bool ROWS_T>
+__device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub,
+                                          const uint64_t *__restrict__ table, int w,
+                                          ClauseState &cs) {
     constexpr int G = W * S;
-    // Invalid slots start "dead" (two non-false occurrences).
-    uint64_t c1 = ~valid, c2 = ~valid;
-    for (int i0 = 0; i0 < len; i0 += S) {
-        const int i = i0 + sub;
-        uint64_t m1 = 0;
-        if (i < len) m1 = ~table[(size_t)cl[i] * W + w];
-        uint64_t m2 = 0;
+    uint64_t f[U], t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        uint32_t lit;
+        if constexpr (S == 1 && U <= 4) {
+            lit = u == 0 ? chunk_lit<G, ST + 0>(litreg, gbase, sub)
+                : u == 1 ? chunk_lit<G, (ST + 1) % G>(litreg, gbase, sub)
+                : u == 2 ? chunk_lit<G, (ST + 2) % G>(litreg, gbase, sub)
+                         : chunk_lit<G, (ST + 3) % G>(litreg, gbase, sub);
+        } else if constexpr (S == 1) {
+            lit = (uint32_t)__shfl((int)litreg, gbase + ST + u);
+        } else {
+            lit = (uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u);
+        }
+        if (ROWS_F) f[u] = row_word<W>(table, lit, w);
+        if (ROWS_T) t[u] = row_word<W>(table, lit ^ 1u, w);
+    }
+    if constexpr (S == 1) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (ROWS_F) {
+                cs.c2 |= cs.c1 & ~f[u];
+                cs.c1 |= ~f[u];
+            }
+            if (ROWS_T) cs.any_t |= t[u];
+        }
+    } else {
+        uint64_t m1 = 0, m2 = 0, tt = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (ROWS_F) {
+                m2 |= m1 & ~f[u];
+                m1 |= ~f[u];
+            }
+            if (ROWS_T) tt |= t[u];
+        }
 #pragma unroll
         for (int d = W; d < G; d <<= 1) {
-            const uint64_t p1 = __shfl_xor(m1, d);
-            const uint64_t p2 = __shfl_xor(m2, d);
-            m2 = m2 | p2 | (m1 & p1);
-            m1 |= p1;
+            if (ROWS_F) {
+                const uint64_t p1 = __shfl_xor(m1, d);
+                const uint64_t p2 = __shfl_xor(m2, d);
+                m2 = m2 | p2 | (m1 & p1);
+                m1 |= p1;
+            }
+            if (ROWS_T) tt |= __shfl_xor(tt, d);
         }
-        c2 |= m2 | (c1 & m1);
-        c1 |= m1;
-        if (!group_any<G>(~c2 != 0, g)) break;
-    }
-    fm = ~c1;
-    const uint64_t cand = c1 & ~c2;
-    um = 0;
-    if (group_any<G>(cand != 0, g)) {
-        uint64_t any_true = 0;
-        for (int i0 = 0; i0 < len; i0 += S) {
-            const int i = i0 + sub;
-            uint64_t t = 0;
-            if (i < len) t = table[(size_t)(cl[i] ^ 1u) * W + w];
-#pragma unroll
-            for (int d = W; d < G; d <<= 1) t |= __shfl_xor(t, d);
-            any_true |= t;
-            if (!group_any<G>((cand & ~any_true) != 0, g)) break;
+        if (ROWS_F) {
+            cs.c2 |= m2 | (cs.c1 & m1);
+            cs.c1 |= m1;
         }
-        um = cand & ~any_true;
+        if (ROWS_T) cs.any_t |= tt;
     }
+}
+
+// Runs the steps of one preloaded chunk while they start before the clause
+// end and `more(cs)` holds.  Returns false if it stopped early.
+template <int W, int S, int U, bool ROWS_F, bool ROWS_T, typename More>
+__device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub, int base,
+                                            int len, const uint64_t *__restrict__ table, int w,
+                                            ClauseState &cs, More &&more) {
+    constexpr int G = W * S;
+    constexpr int LPS = S * U;
+    bool go = true;
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        ((go = go && (base + I * LPS < len) &&
+               [&] {
+                   step_rows<W, S, U, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, table, w, cs);
+                   return more(cs);
+               }()),
+         ...);
+    }(std::make_integer_sequence<int, G / LPS>{});
+    return go;
+}
+
+// Evaluates one clause for this lane's word.  All G lanes of the group call it
+// together (group-uniform control flow).  first_chunk holds literal q of the
+// clause in lane q of the group (the sentinel past the end), already loaded by
+// the caller.
+template <int W, int S, int U>
+__device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, int len,
+                                             uint32_t first_chunk, uint32_t sentinel,
+                                             const uint64_t *__restrict__ table, int q, int g,
+                                             int w, int sub, uint64_t valid, int fuse_len,
+                                             uint64_t &fm, uint64_t &um) {
+    constexpr int G = W * S;
+    const int gbase = g * G;
+    // invalid slots start dead
+    ClauseState cs{~valid, ~valid, 0};
+    const bool fused = len <= fuse_len;
+    auto alive = [&](const ClauseState &c) { return group_any<G>(~(c.c2 | c.any_t) != 0, g); };
+    uint32_t litreg = first_chunk;
+    for (int base = 0; base < len; base += G) {
+        if (base) litreg = base + q < len ? cl[base + q] : sentinel;
+        const bool go = fused ? chunk_steps<W, S, U, true, true>(litreg, gbase, sub, base, len,
+                                                                 table, w, cs, alive)
+                              : chunk_steps<W, S, U, true, false>(litreg, gbase, sub, base, len,
+                                                                  table, w, cs, alive);
+        if (!go) break;  // dead, or the clause ended inside this chunk
+    }
+    fm = ~cs.c1;
+    const uint64_t cand = cs.c1 & ~cs.c2 & ~cs.any_t;
+    if (!fused && group_any<G>(cand != 0, g)) {
+        auto open = [&](const ClauseState &c) { return group_any<G>((cand & ~c.any_t) != 0, g); };
+        litreg = first_chunk;
+        for (int base = 0; base < len; base += G) {
+            if (base) litreg = base + q < len ? cl[base + q] : sentinel;
+            if (!chunk_steps<W, S, U, false, true>(litreg, gbase, sub, base, len, table, w, cs,
+                                                   open))
+                break;
+        }
+    }
+    um = cand & ~cs.any_t;
 }
 
 __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int nb, int task) {
@@ -113,13 +231,14 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
     return lo;
 }
 
-template <int W, int S>
+template <int W, int S, int U>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     constexpr int G = W * S;
     constexpr int CPW = 64 / G;  // clauses per wave pass
-    const int lane = (int)lane_id();
+    const int lane = (int)__lane_id();
     const int g = lane / G;
-    const int sub = (lane % G) / W;
+    const int q = lane % G;
+    const int sub = q / W;
     const int w = lane % W;
     const uint64_t valid = a.valid[w];
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -134,13 +253,20 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
         const int64_t row0 = (int64_t)(task - (int)bd.task_begin) * a.task_clauses;
         const int64_t rem = (int64_t)bd.count - row0;
         const int nrows = rem < a.task_clauses ? (int)rem : a.task_clauses;
+        const uint32_t *rows_base = bd.lits + (size_t)row0 * len;
+        // literal q of the group's first clause chunk, prefetched one pass ahead
+        uint32_t next_chunk =
+            (g < nrows && q < len) ? rows_base[(size_t)g * len + q] : a.sentinel;
         for (int it = 0; it < nrows; it += CPW) {
             const bool active = it + g < nrows;
             const int64_t row = row0 + it + g;
+            const uint32_t chunk0 = next_chunk;
+            const int nxt = it + CPW + g;
+            next_chunk = (nxt < nrows && q < len) ? rows_base[(size_t)nxt * len + q] : a.sentinel;
             uint64_t fm = 0, um = 0;
             if (active) {
-                check_clause<W, S>(bd.lits + (size_t)row * len, len, a.table, w, sub, g, valid, fm,
-                                   um);
+                check_clause<W, S, U>(bd.lits + (size_t)row * len, len, chunk0, a.sentinel,
+                                      a.table, q, g, w, sub, valid, a.fuse_len, fm, um);
             }
             const bool hit = active && sub == 0 && (fm | um) != 0;
             const unsigned long long bal = __ballot(hit);
@@ -215,48 +341,73 @@ __global__ void k_assig_roll(const gss_var_update *__restrict__ ups, int64_t n,
     table[base + words] = ((t >> p) & 1ull) ? ~0ull : 0ull;
 }
 
-template <int W, int S>
+template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_sweep<W, S>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<W, S, U>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-template <int W, int S>
+template <int W, int S, int U>
 int occupancy_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U>, 256, 0) !=
+        hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
+}
+
+// Kernel variants: (W, S, U).  `unroll` selects U for W = 16 (0 = default).
+#define GSS_SWEEP_VARIANTS(X)                  \
+    X(1, 16, 1)                                \
+    X(2, 8, 2)                                 \
+    X(4, 4, 4)                                 \
+    X(8, 2, 4)                                 \
+    X(16, 1, 4)                                \
+    X(16, 1, 1)                                \
+    X(16, 1, 2)                                \
+    X(16, 1, 8)                                \
+    X(32, 1, 4)                                \
+    X(64, 1, 2)
+
+struct Variant {
+    int w, s, u;
+};
+
+Variant pick(int words, int unroll) {
+    switch (words) {
+        case 1: return {1, 16, 1};
+        case 2: return {2, 8, 2};
+        case 4: return {4, 4, 4};
+        case 8: return {8, 2, 4};
+        case 16:
+            if (unroll == 1 || unroll == 2 || unroll == 8) return {16, 1, unroll};
+            return {16, 1, 4};
+        case 32: return {32, 1, 4};
+        case 64: return {64, 1, 2};
+        default: return {0, 0, 0};
+    }
 }
 
 }  // namespace
 
 int sweep_clauses_per_pass(int words) { return words <= 16 ? 4 : 64 / words; }
 
-hipError_t launch_sweep(const SweepArgs &a, int words, int blocks, hipStream_t s) {
-    switch (words) {
-        case 1: return launch_sweep_t<1, 16>(a, blocks, s);
-        case 2: return launch_sweep_t<2, 8>(a, blocks, s);
-        case 4: return launch_sweep_t<4, 4>(a, blocks, s);
-        case 8: return launch_sweep_t<8, 2>(a, blocks, s);
-        case 16: return launch_sweep_t<16, 1>(a, blocks, s);
-        case 32: return launch_sweep_t<32, 1>(a, blocks, s);
-        case 64: return launch_sweep_t<64, 1>(a, blocks, s);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s) {
+    const Variant v = pick(words, unroll);
+#define X(W_, S_, U_) \
+    if (v.w == W_ && v.s == S_ && v.u == U_) return launch_sweep_t<W_, S_, U_>(a, blocks, s);
+    GSS_SWEEP_VARIANTS(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
-int sweep_occupancy(int words) {
-    switch (words) {
-        case 1: return occupancy_t<1, 16>();
-        case 2: return occupancy_t<2, 8>();
-        case 4: return occupancy_t<4, 4>();
-        case 8: return occupancy_t<8, 2>();
-        case 16: return occupancy_t<16, 1>();
-        case 32: return occupancy_t<32, 1>();
-        case 64: return occupancy_t<64, 1>();
-        default: return 1;
-    }
+int sweep_occupancy(int words, int unroll) {
+    const Variant v = pick(words, unroll);
+#define X(W_, S_, U_) \
+    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>();
+    GSS_SWEEP_VARIANTS(X)
+#undef X
+    return 1;
 }
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s) {

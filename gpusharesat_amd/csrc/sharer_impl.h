@@ -83,6 +83,7 @@ struct ClauseMeta {
 class ClauseMirror {
 public:
     explicit ClauseMirror(float decay) : decay_(decay) {}
+    void set_decay(float decay) { decay_ = decay; }
     // any thread
     long stage(const int *lits, int count, int lbd);
     // GPU thread: moves staged clauses into the buckets; returns them grouped
@@ -96,7 +97,7 @@ public:
     void commit_staged(Batch &b);
     void bump(int len, uint32_t row);
     // keeps clauses per the reference policy; returns the buckets changed
-    std::vector<int> reduce(std::vector<unsigned long> *stats);
+    std::vector<int> reduce(int max_lbd);
     float approx_nth_activity(int min_lbd, int max_lbd, long n) const;
 
     long clause_count() const { return count_; }
@@ -111,6 +112,16 @@ public:
     const std::vector<uint32_t> &bucket_lits(int len) const { return lits_[len]; }
     bool has_staged();
     float act_incr() const { return act_incr_; }
+    bool find(long id, float &activity, int &lbd) const {
+        for (const auto &bucket : meta_)
+            for (const ClauseMeta &m : bucket)
+                if (m.id == id) {
+                    activity = m.activity;
+                    lbd = m.lbd;
+                    return true;
+                }
+        return false;
+    }
 
 private:
     void rescale();
@@ -203,6 +214,9 @@ public:
     int max_clause_size() const { return limits_.max_clause_size; }
     gss_engine *engine() { return engine_; }
     int solver_count() const { return (int)windows_.size(); }
+    bool debug_clause(long id, float &activity, int &lbd) const {
+        return clauses_.find(id, activity, lbd);
+    }
 
 private:
     struct InFlight {
@@ -210,15 +224,16 @@ private:
         std::vector<SolverWindow::Taken> taken;  // per solver
         std::vector<gss_var_update> updates;
         std::vector<int8_t> last_pos;
+        long clause_count = 0;
     };
 
     void run_once(bool can_start);
+    void reduce_db_now();
     bool start_run(InFlight &run);
     void collect(InFlight &run, std::vector<gss_hit> &hits);
     void deliver(InFlight &run, std::vector<gss_hit> &hits);
     void unset_pending_locked(int solver);
     void ensure_table();
-    void fail(const std::string &what, int rc);
     void log(int level, const std::string &msg);
     int words_for(int solvers) const;
 
@@ -229,7 +244,7 @@ private:
     int var_count_ = 0;
     int64_t table_vars_ = 0;
     int table_words_ = 0;
-    bool table_dirty_shape_ = true;
+    int64_t clause_vars_ = 0;  // 1 + largest variable of any clause on the device
 
     ClauseMirror clauses_;
     std::vector<std::unique_ptr<SolverWindow>> windows_;

@@ -21,6 +21,8 @@ assert HIT_DTYPE.itemsize == 24 and VAR_UPDATE_DTYPE.itemsize == 24
 TUNE_WAVES_PER_CU = 1
 TUNE_TASK_CLAUSES = 2
 TUNE_MAX_HIT_CHUNKS = 3
+TUNE_UNROLL = 4
+TUNE_FUSE_LEN = 5
 MAX_WORDS = 64
 
 

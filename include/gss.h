@@ -81,6 +81,9 @@ int gss_engine_destroy(gss_engine *e);
 int gss_engine_cu_count(gss_engine *e, int *out);
 /* Blocks until all work queued on the engine's stream is done. */
 int gss_engine_sync(gss_engine *e);
+/* Free and total device memory (replaces cudaMemGetInfo in
+ * GpuClauseSharerImpl::getGpuMemInfo, GpuClauseSharerImpl.cu:205-207). */
+int gss_engine_mem_info(gss_engine *e, size_t *free_bytes, size_t *total_bytes);
 
 /* ---- clause DB -------------------------------------------------------------
  * Append n clauses (lens[i] literals each, concatenated in lits).  Rows are
@@ -113,8 +116,14 @@ int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_t
 /* Download whole planes (same layout).  Synchronous. */
 int gss_assig_download(gss_engine *e, uint64_t *is_def, uint64_t *is_true);
 /* Apply n per-(var, word) updates (replaces dUpdateAssigs, Assigs.cu:100-116).
- * At most one update per (var, word) per call. */
+ * At most one update per (var, word) per call.  Asynchronous on the engine's
+ * stream (the updates are copied before the call returns). */
 int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups);
+/* Roll forward the (var, word) pairs of the last gss_assig_apply, as
+ * gss_assig_roll_forward does, without uploading them again; asynchronous.
+ * A run enqueues apply -> sweep -> this, so the roll-forward follows the
+ * check as dSetAllAssigsToLast follows dFindClauses (GpuRunner.cu:339-344). */
+int gss_assig_roll_forward_applied(gss_engine *e, const int8_t *last_pos);
 /* For each (var, word) of ups, set every slot of that word to the value held
  * by slot last_pos[word] (replaces dSetAllAssigsToLast, Assigs.cu:127-141).
  * Words with last_pos < 0 are left alone. */
@@ -148,6 +157,11 @@ int gss_set_tuning(gss_engine *e, int key, int value);
 #define GSS_TUNE_MAX_HIT_CHUNKS 3 /* cap on the hit buffer a sweep starts with
                                      (0 = sized to the worst case); a sweep
                                      that needs more re-runs with a larger one */
+#define GSS_TUNE_UNROLL 4         /* table rows in flight per lane at 16 words:
+                                     1, 2, 4 or 8 (0 = default 4) */
+#define GSS_TUNE_FUSE_LEN 5       /* clauses of at most this many literals read
+                                     both rows of a literal in one pass
+                                     (0 = default 4) */
 
 #ifdef __cplusplus
 }

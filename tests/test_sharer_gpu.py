@@ -1,0 +1,351 @@
+"""GpuClauseSharer API on the GPU: the reference's library-level tests restated
+through the Python mirror of include/gpuShareLib/GpuClauseSharer.h, plus a
+randomised end-to-end check of the assignment snapshot semantics against the
+CPU oracle.
+
+Reference tests (glucose-syrup/test/GpuSolverTest.cu unless noted):
+  testReported :285-328, testClausesAssigsReported :343-391,
+  testFindClausesMultiThread :394-454, testOneAssignmentThenTwo :486-508,
+  SolverDoesntImportSameClauseTwice(+OnSuccessiveGpuExecutions) :511-563,
+  SolverHasManyClausesReportedAllAtOnce :678-724, testReduceDb :889-935,
+  testSolverPassesManyAssignments :949-981, testManyAssignments :157-181,
+  SolverCanReimportClause (rel-newtech/test/GpuSolverTest.cu:577-596),
+  ClauseActivityLbdTest.cu:59-95 (activity init / bump / decay).
+"""
+import numpy as np
+import pytest
+
+import _oracle as orc
+from gpusharesat_amd.sharer import GpuClauseSharer, default_options, mkLit
+
+pytestmark = pytest.mark.gpu
+
+
+def sharer(var_count, solvers, **opts):
+    s = GpuClauseSharer(default_options(min_gpu_latency_micros=0, verbosity=0, **opts))
+    s.setVarCount(var_count)
+    s.setCpuSolverCount(solvers)
+    return s
+
+
+def T(v):
+    return mkLit(v)
+
+
+def F(v):
+    return mkLit(v, True)
+
+
+def test_reported_unit_clause():
+    with sharer(10, 3) as s:
+        cid = s.addClause(-1, [mkLit(3)])
+        assert s.trySetSolverValues(0, [F(3)])
+        assert s.trySendAssignment(0) == 0
+        s.execute()
+        assert s.popReportedClause(0) == ([mkLit(3)], cid)
+        assert s.popReportedClause(0) is None
+        assert cid == 0
+
+
+def test_clauses_assigs_reported():
+    with sharer(3, 3) as s:
+        for v in range(3):
+            s.addClause(-1, [mkLit(v)])
+        assert s.trySetSolverValues(0, [F(0), T(1), T(2)])
+        s.trySendAssignment(0)
+        assert s.trySetSolverValues(0, [T(0), F(1), F(2)])
+        s.trySendAssignment(0)
+        assert s.trySetSolverValues(1, [T(0), F(1), T(2)])
+        s.trySendAssignment(1)
+        s.execute()
+        assert [len(s.popAll(i)) for i in range(3)] == [3, 1, 0]
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert len(s.popAll(0)) == 1
+        assert len(s.popAll(1)) == 0
+
+
+def test_find_clauses():
+    with sharer(3, 1) as s:
+        assert s.trySetSolverValues(0, [F(0), T(1)])  # var 2 stays undefined
+        s.trySendAssignment(0)
+        s.addClause(-1, [mkLit(0), mkLit(1)])
+        s.addClause(-1, [mkLit(0), F(1)])
+        s.addClause(-1, [F(1), mkLit(2)])
+        s.execute()
+        got = sorted(tuple(l) for l, _ in s.popAll(0))
+        assert got == sorted([(mkLit(0), F(1)), (F(1), mkLit(2))])
+
+
+def test_one_assignment_then_two():
+    with sharer(4, 1) as s:
+        s.addClause(-1, [F(0), F(1), mkLit(2)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert s.trySetSolverValues(0, [T(3)])
+        s.trySendAssignment(0)
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert len(s.popAll(0)) == 1
+
+
+def test_same_clause_imported_once():
+    with sharer(3, 1) as s:
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.unsetSolverValues(0, [T(0)])
+        assert s.trySetSolverValues(0, [F(1)])
+        s.trySendAssignment(0)
+        s.addClause(-1, [F(0), mkLit(1)])
+        s.execute()
+        assert len(s.popAll(0)) == 1
+        assert s.getOneSolverStat(0, "reportedClauses") == 1
+
+
+def test_same_clause_imported_once_across_runs():
+    with sharer(3, 1) as s:
+        s.addClause(-1, [F(0), mkLit(1)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.gpuRun()
+        s.unsetSolverValues(0, [T(0)])
+        assert s.trySetSolverValues(0, [F(1)])
+        s.trySendAssignment(0)
+        s.gpuRun()
+        n = len(s.popAll(0))
+        s.gpuRun()
+        n += len(s.popAll(0))
+        assert n == 1
+        assert s.getOneSolverStat(0, "reportedClauses") == 1
+
+
+def test_same_assignment_resent_reports_again():
+    with sharer(5, 1) as s:
+        s.addClause(-1, [F(0), mkLit(1)])
+        assert s.trySetSolverValues(0, [T(0)])
+        assert s.trySendAssignment(0) >= 0
+        s.execute()
+        assert len(s.popAll(0)) == 1
+        assert s.trySendAssignment(0) >= 0
+        s.execute()
+        assert len(s.popAll(0)) == 1
+
+
+def test_many_clauses_reported_all_at_once():
+    n = 4000
+    with sharer(n, 1) as s:
+        assert s.trySetSolverValues(0, [T(i) for i in range(0, n, 3)])
+        for i in range(n):
+            s.addClause(-1, [mkLit(i)])
+        s.trySendAssignment(0)
+        s.execute()
+        got = s.popAll(0)
+        assert len(got) == n - (n + 2) // 3
+        assert all(l[0] >> 1 != 0 and (l[0] >> 1) % 3 != 0 for l, _ in got)
+        assert s.getOneSolverStat(0, "reportedClausesUnit") == len(got)
+
+
+def test_solver_passes_many_assignments():
+    with sharer(64, 3) as s:
+        for i in range(32):
+            s.addClause(-1, [F(2 * i), mkLit(2 * i + 1)])
+            if i:
+                s.unsetSolverValues(0, [T(2 * i - 2)])
+            assert s.trySetSolverValues(0, [T(2 * i)])
+            assert s.trySendAssignment(0) == i
+        s.execute()
+        got = s.popAll(0)
+        assert len(got) == 32
+        assert s.getOneSolverStat(0, "failuresToFindAssig") == 0
+        assert s.getOneSolverStat(0, "reportedClausesBinary") == 32
+
+
+def test_slot_window_exhaustion_and_deferred_unsets():
+    with sharer(4, 1) as s:
+        slots = s.slots_per_solver
+        s.addClause(-1, [mkLit(3)])
+        for i in range(slots):
+            assert s.trySetSolverValues(0, [T(i % 3)])
+            assert s.trySendAssignment(0) == i
+        assert s.trySendAssignment(0) == -1
+        assert not s.trySetSolverValues(0, [T(3)])
+        assert s.getOneSolverStat(0, "failuresToFindAssig") == 2
+        # an unset while full is deferred, and visible in the current assignment
+        s.unsetSolverValues(0, [T(0)])
+        cur = s.getCurrentAssignment(0, 4)
+        assert list(cur) == [2, 0, 0, 2]
+        s.execute()
+        assert s.trySendAssignment(0) == slots
+        assert len(s.popAll(0)) == 1  # {x3} unit in every assignment
+
+
+def test_current_assignment_values():
+    with sharer(5, 2) as s:
+        s.trySetSolverValues(1, [T(0), F(2), T(4)])
+        assert list(s.getCurrentAssignment(1, 5)) == [0, 2, 1, 2, 0]
+        s.unsetSolverValues(1, [F(2)])
+        assert list(s.getCurrentAssignment(1, 5)) == [0, 2, 2, 2, 0]
+        assert list(s.getCurrentAssignment(0, 5)) == [2] * 5
+
+
+def test_own_clause_not_reported_back():
+    with sharer(3, 2) as s:
+        for solver in (0, 1):
+            assert s.trySetSolverValues(solver, [T(0)])
+            s.trySendAssignment(solver)
+        cid = s.addClause(0, [F(0), mkLit(1)])
+        s.execute()
+        assert s.popAll(0) == []
+        assert s.popAll(1) == [([F(0), mkLit(1)], cid)]
+
+
+def test_reduce_db_keeps_active_clauses():
+    with sharer(5, 1) as s:
+        s.addClause(-1, [F(1), mkLit(3), mkLit(4)])
+        used = s.addClause(-1, [F(0), mkLit(2), mkLit(4)])
+        assert s.trySetSolverValues(0, [T(0), F(4)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert s.getGlobalStat("gpuClauses") == 2
+        assert s.getGlobalStat("gpuClauseLengthSum") == 6
+        assert [c for _, c in s.popAll(0)] == [used]
+        s.reduceDb()
+        assert s.getGlobalStat("gpuClauses") == 1
+        assert s.getGlobalStat("gpuClauseLengthSum") == 3
+        assert s.getGlobalStat("gpuReduceDbs") == 1
+        assert s.getAddedClauseCountAtLastReduceDb() == 2
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert all(c == used for _, c in s.popAll(0))
+        rep = s.getOneSolverStat(0, "reportedClauses")
+        s.addClause(-1, [F(0), F(3)])
+        s.trySendAssignment(0)
+        s.execute()
+        s.popAll(0)
+        assert s.getOneSolverStat(0, "reportedClauses") >= rep + 1
+        assert s.getAddedClauseCount() == 3
+
+
+def test_activity_init_bump_decay():
+    # ClauseActivityLbdTest.cu:59-95 with decay 0.5: a new clause starts at the
+    # increment after one decay (2.0); a report bumps it by the increment.
+    with sharer(2, 1, clause_activity_decay=0.5) as s:
+        cid = s.addClause(-1, [mkLit(0)])
+        s.gpuRun()
+        act, lbd = s.debug_clause(cid)
+        assert abs(act - 2.0) < 1e-6 and lbd == 1
+        s.trySendAssignment(0)  # x0 undefined: {x0} is unit
+        s.execute()
+        act, _ = s.debug_clause(cid)
+        assert abs(act - 4.0) < 1e-6
+        s.addClause(-1, [mkLit(1)])  # one more decay: increment 4.0
+        s.trySendAssignment(0)
+        s.execute()
+        act, _ = s.debug_clause(cid)
+        assert abs(act - 8.0) < 1e-6
+
+
+def test_stat_names_and_counts():
+    with sharer(3, 1) as s:
+        assert s.getGlobalStatCount() == 13
+        assert s.getOneSolverStatCount() == 6
+        assert s.getGlobalStatName(0) == "gpuClauses"
+        assert s.getGlobalStatName(12) == "timeSpentReduceGpuDb"
+        assert s.getOneSolverStatName(5) == "reportedClausesBinary"
+        free, total = s.getGpuMemInfo()
+        assert 0 < free <= total
+
+
+def test_write_clauses_in_cnf(tmp_path):
+    with sharer(4, 1) as s:
+        s.addClause(-1, [F(0), mkLit(3)])
+        s.addClause(-1, [mkLit(1)])
+        s.gpuRun()
+        p = tmp_path / "db.cnf"
+        s.writeClausesInCnf(str(p))
+        assert p.read_text().splitlines() == ["p cnf 4 2", "2 0", "-1 4 0"]
+
+
+def test_rejected_clauses_and_bad_ids():
+    from gpusharesat_amd._lib import GssError
+    with sharer(4, 1) as s:
+        assert s.addClause(-1, []) == -1
+        assert s.addClause(-1, [0] * (s.max_clause_size + 1)) == -1
+        with pytest.raises(GssError):
+            s.trySendAssignment(5)
+        with pytest.raises(GssError):
+            s.addClause(3, [0])
+
+
+def _clause_hits(clauses, snapshots):
+    """Oracle: for each snapshot (var values 0/1/2), the set of clause indices
+    that are falsified or unit."""
+    V = len(snapshots[0])
+    A = len(snapshots)
+    W = (A + 63) // 64
+    d = np.zeros((V, W), np.uint64)
+    t = np.zeros((V, W), np.uint64)
+    for a, vals in enumerate(snapshots):
+        bit = np.uint64(1 << (a % 64))
+        d[vals != 2, a // 64] |= bit
+        t[vals == 0, a // 64] |= bit
+    off = np.zeros(len(clauses) + 1, np.int64)
+    off[1:] = np.cumsum([len(c) for c in clauses])
+    lits = np.concatenate([np.asarray(c, np.uint32) for c in clauses])
+    valid = np.zeros(W, np.uint64)
+    for a in range(A):
+        valid[a // 64] |= np.uint64(1 << (a % 64))
+    out = [set() for _ in range(A)]
+    for h in orc.sweep(off, lits, d, t, valid):
+        m = int(h["false_mask"]) | int(h["unit_mask"])
+        for b in range(64):
+            if m >> b & 1:
+                out[int(h["word"]) * 64 + b].add(int(h["clause"]))
+    return out
+
+
+@pytest.mark.parametrize("solvers,seed", [(1, 0), (3, 1), (5, 2), (16, 3)])
+def test_snapshot_semantics_match_oracle(solvers, seed):
+    """Each solver mutates its assignment (set / unset), sends 0-3 snapshots
+    per run, and pops everything after two runs; the popped clause ids must be
+    exactly the union of the oracle's hits over its snapshots of that run."""
+    rng = np.random.default_rng(seed)
+    V = 60
+    with sharer(V, solvers) as s:
+        clauses = []
+        ids = []
+        state = [np.full(V, 2, np.int64) for _ in range(solvers)]
+        for rnd in range(12):
+            for _ in range(int(rng.integers(5, 40))):
+                L = int(rng.integers(1, 6))
+                c = [int(x) for x in rng.integers(0, 2 * V, size=L)]
+                clauses.append(c)
+                ids.append(s.addClause(-1, c))
+            snaps = [[] for _ in range(solvers)]
+            for so in range(solvers):
+                for _ in range(int(rng.integers(0, 4))):
+                    setv = rng.choice(V, size=int(rng.integers(0, 12)), replace=False)
+                    lits = [mkLit(int(v), bool(rng.integers(0, 2))) for v in setv]
+                    unset = [int(v) for v in rng.choice(V, size=int(rng.integers(0, 8)),
+                                                         replace=False)]
+                    s.unsetSolverValues(so, [mkLit(v) for v in unset])
+                    for v in unset:
+                        state[so][v] = 2
+                    assert s.trySetSolverValues(so, lits)
+                    for l in lits:
+                        state[so][l >> 1] = 1 if l & 1 else 0
+                    assert list(s.getCurrentAssignment(so, V)) == list(state[so])
+                    assert s.trySendAssignment(so) >= 0
+                    snaps[so].append(state[so].copy())
+            s.execute()
+            for so in range(solvers):
+                got = sorted(cid for _, cid in s.popAll(so))
+                exp = set()
+                if snaps[so]:
+                    for hs in _clause_hits(clauses, snaps[so]):
+                        exp |= hs
+                assert got == sorted(ids[i] for i in exp), (rnd, so)

@@ -11,8 +11,9 @@ import pytest
 
 import _oracle as orc
 from gpusharesat_amd import workloads
-from gpusharesat_amd.engine import (ClauseMap, Engine, VAR_UPDATE_DTYPE, TUNE_MAX_HIT_CHUNKS,
-                                    TUNE_TASK_CLAUSES, TUNE_WAVES_PER_CU)
+from gpusharesat_amd.engine import (ClauseMap, Engine, VAR_UPDATE_DTYPE, TUNE_FUSE_LEN,
+                                    TUNE_MAX_HIT_CHUNKS, TUNE_TASK_CLAUSES, TUNE_UNROLL,
+                                    TUNE_WAVES_PER_CU)
 
 pytestmark = pytest.mark.gpu
 
@@ -184,10 +185,23 @@ def test_tuning_never_changes_results(engine_factory):
     e.sweep(inst.valid)
     ref = gpu_records(e, inst.lens, rows)
     for key, val in ((TUNE_WAVES_PER_CU, 4), (TUNE_TASK_CLAUSES, 4), (TUNE_TASK_CLAUSES, 1000),
-                     (TUNE_WAVES_PER_CU, 0)):
+                     (TUNE_WAVES_PER_CU, 0), (TUNE_TASK_CLAUSES, 0), (TUNE_FUSE_LEN, 1),
+                     (TUNE_FUSE_LEN, 1000), (TUNE_FUSE_LEN, 0)):
         e.set_tuning(key, val)
         e.sweep(inst.valid)
         assert_same(gpu_records(e, inst.lens, rows), ref)
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 8])
+@pytest.mark.parametrize("fuse", [1, 3, 1000])
+def test_parity_unroll_and_fuse_at_16_words(engine_factory, unroll, fuse):
+    lits, lens, d, t, valid = random_case(300 + unroll * 7 + fuse, 400, 4000, 30, 16, p_def=0.9,
+                                          p_true=0.25)
+    e = engine_factory()
+    e.set_tuning(TUNE_UNROLL, unroll)
+    e.set_tuning(TUNE_FUSE_LEN, fuse)
+    got, exp = run_both(e, lits, lens, 400, d, t, valid)
+    assert_same(got, exp)
 
 
 def test_incremental_appends_and_bucket_replace(engine_factory):
