@@ -2,13 +2,14 @@
 // assignment table and the hit chunks, and the C ABI declared in gss.h.
 //
 // Layout in HBM
-//   clause bucket `len`: uint32 literals, row-major (row r at r*len), capacity
-//       doubling; replaces the reference's 32-interleaved per-size arrays
+//   clause bucket `len`: uint32 row-major (row r at r*len), capacity doubling;
+//       each literal l is stored as l << 7, the byte offset of its table row
+//       at 16 words.  Replaces the reference's 32-interleaved per-size arrays
 //       (gpuShareLib/Clauses.cu:47-55) because a clause is read by a whole lane
 //       group at once here, not by one lane.
-//   assignment table: uint64 [2*vars][words]; row 2v = slots where v is false,
-//       row 2v+1 = slots where v is true, i.e. row l = slots where literal l is
-//       false.  Replaces per-solver MultiLBool arrays + MultiAgg aggregates
+//   assignment table: uint64 [2*vars + 2][words]; row l = slots where literal
+//       l is NOT false (row 2v: v not false, row 2v+1: v not true), then two
+//       sentinel rows.  Replaces per-solver MultiLBool arrays + MultiAgg aggregates
 //       (gpuShareLib/Assigs.cuh:39-74): one row holds every solver's slots, so
 //       the reference's aggregate prefilter and agg-bit -> solver walk
 //       (GpuRunner.cu:92-131) are not needed.
@@ -36,6 +37,11 @@ namespace {
 // Clauses of at most this many literals read both rows of each literal in one
 // pass (see kernels.hip).
 constexpr int kDefaultFuseLen = 4;
+
+// Clause literals are stored on the device as row offsets, literal << 7 (the
+// byte offset of the literal's row in a 16-word table); see kernels.hip.
+constexpr int kLitShift = 7;
+constexpr uint32_t kMaxLit = (1u << (32 - kLitShift)) - 1;
 
 thread_local std::string g_err;
 
@@ -229,7 +235,7 @@ int launch_current_sweep(gss_engine *e) {
         a.ntasks = e->ntasks;
         a.task_clauses = e->task_clauses;
         a.table = e->table.p;
-        a.sentinel = (uint32_t)(2 * e->vars);
+        a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
         a.valid = e->d_valid;
         a.hits = e->d_hits.p;
         a.fills = e->d_fills.p;
@@ -365,6 +371,7 @@ int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t 
     const int64_t total = pos;
     uint32_t mx = 0;
     for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
+    if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
     std::vector<int64_t> per(maxlen + 1, 0);
     for (int64_t i = 0; i < n; i++) per[lens[i]]++;
     if ((int)e->buckets.size() <= maxlen) e->buckets.resize(maxlen + 1);
@@ -405,7 +412,7 @@ int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t 
                            hipMemcpyHostToDevice, e->stream));
     HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, (size_t)nseg * sizeof(ScatterSeg),
                            hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(launch_scatter(e->d_segs.p, nseg, e->d_stage.p, e->stream));
+    HIP_TRY(launch_scatter(e->d_segs.p, nseg, e->d_stage.p, kLitShift, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->total_clauses += n;
     e->total_lits += total;
@@ -420,6 +427,7 @@ int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
         return set_err(GSS_E_INVAL, "bad bucket replacement");
     uint32_t mx = 0;
     for (int64_t i = 0; i < n * len; i++) mx = std::max(mx, lits[i]);
+    if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
     if ((int)e->buckets.size() <= len) e->buckets.resize(len + 1);
     Bucket &b = e->buckets[len];
     const int64_t old = b.count;
@@ -428,10 +436,13 @@ int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
         b.count = old;
         return rc;
     }
-    if (n)
-        HIP_TRY(hipMemcpyAsync(b.d, lits, (size_t)n * len * sizeof(uint32_t),
+    if (n) {
+        std::vector<uint32_t> stored((size_t)n * len);
+        for (size_t i = 0; i < stored.size(); i++) stored[i] = lits[i] << kLitShift;
+        HIP_TRY(hipMemcpyAsync(b.d, stored.data(), stored.size() * sizeof(uint32_t),
                                hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
     b.count = n;
     e->total_clauses += n - old;
     e->total_lits += (n - old) * len;
@@ -463,6 +474,7 @@ int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) 
                                (size_t)n * len * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int64_t i = 0; i < n * len; i++) out[i] >>= kLitShift;  // stored offsets -> literals
     return GSS_OK;
 }
 
@@ -477,10 +489,12 @@ int gss_db_clear(gss_engine *e) {
 
 /* ---- assignment table ----------------------------------------------------- */
 
-// The table has 2 * (vars + 1) rows: rows 2v, 2v+1 for each variable, then
-// two sentinel rows.  Row 2*vars is all ones ("false in every slot") and row
-// 2*vars + 1 is all zeros ("true in no slot"); the sweep reads them for
-// padding lanes past the end of a clause, so those lanes need no masking.
+// The table has 2 * (vars + 1) rows.  Row l holds the slots where literal l
+// is NOT false (all ones = undefined everywhere): row 2v = v not false, row
+// 2v+1 = v not true.  Two sentinel rows follow: row 2*vars is all zeros
+// ("false in every slot") and row 2*vars + 1 all ones ("true in no slot");
+// the sweep reads them for padding lanes past the end of a clause, so those
+// lanes need no masking.
 static size_t table_rows(int64_t vars) { return (size_t)(vars + 1) * 2; }
 
 static int table_fits(int64_t vars, int words) {
@@ -492,9 +506,9 @@ static int table_fits(int64_t vars, int words) {
 }
 
 static int write_sentinels(gss_engine *e) {
-    HIP_TRY(launch_fill_u64(e->table.p + (size_t)e->vars * 2 * e->words, e->words, ~0ull,
+    HIP_TRY(launch_fill_u64(e->table.p + (size_t)e->vars * 2 * e->words, e->words, 0ull,
                             e->stream));
-    HIP_TRY(launch_fill_u64(e->table.p + ((size_t)e->vars * 2 + 1) * e->words, e->words, 0ull,
+    HIP_TRY(launch_fill_u64(e->table.p + ((size_t)e->vars * 2 + 1) * e->words, e->words, ~0ull,
                             e->stream));
     return GSS_OK;
 }
@@ -514,7 +528,7 @@ int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
         e->table.release();
         if ((rc = e->table.reserve(n))) return rc;
     }
-    HIP_TRY(launch_fill_u64(e->table.p, (int64_t)n, 0ull, e->stream));
+    HIP_TRY(launch_fill_u64(e->table.p, (int64_t)n, ~0ull, e->stream));  // all undefined
     e->vars = var_count;
     e->words = words;
     if ((rc = write_sentinels(e))) return rc;
@@ -532,7 +546,7 @@ int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
     const size_t old_n = (size_t)e->vars * 2 * e->words;
     const size_t n = table_rows(var_count) * e->words;
     if ((rc = e->table.reserve(n, true, e->stream))) return rc;
-    HIP_TRY(launch_fill_u64(e->table.p + old_n, (int64_t)(n - old_n), 0ull, e->stream));
+    HIP_TRY(launch_fill_u64(e->table.p + old_n, (int64_t)(n - old_n), ~0ull, e->stream));
     e->vars = var_count;
     if ((rc = write_sentinels(e))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -554,8 +568,8 @@ int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_t
         for (int64_t v = 0; v < nv; v++) {
             for (int w = 0; w < W; w++) {
                 const uint64_t d = is_def[(v0 + v) * W + w], t = is_true[(v0 + v) * W + w];
-                st.p[(v * 2) * W + w] = d & ~t;
-                st.p[(v * 2 + 1) * W + w] = d & t;
+                st.p[(v * 2) * W + w] = ~(d & ~t);
+                st.p[(v * 2 + 1) * W + w] = ~(d & t);
             }
         }
         HIP_TRY(hipMemcpyAsync(e->table.p + (size_t)v0 * 2 * W, st.p,
@@ -579,7 +593,7 @@ int gss_assig_download(gss_engine *e, uint64_t *is_def, uint64_t *is_true) {
     HIP_TRY(hipStreamSynchronize(e->stream));
     for (int64_t v = 0; v < V; v++)
         for (int w = 0; w < W; w++) {
-            const uint64_t f = t[(v * 2) * W + w], tr = t[(v * 2 + 1) * W + w];
+            const uint64_t f = ~t[(v * 2) * W + w], tr = ~t[(v * 2 + 1) * W + w];
             is_def[v * W + w] = f | tr;
             is_true[v * W + w] = tr;
         }
@@ -808,7 +822,7 @@ int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
     }
     HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, ns * sizeof(ScatterSeg),
                            hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(launch_scatter(e->d_segs.p, ns, (const uint32_t *)e->d_hits.p, e->stream));
+    HIP_TRY(launch_scatter(e->d_segs.p, ns, (const uint32_t *)e->d_hits.p, 0, e->stream));
     HIP_TRY(hipMemcpyAsync(out, e->d_dense.p, (size_t)want * sizeof(gss_hit),
                            hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));

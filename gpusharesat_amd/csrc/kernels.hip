@@ -4,28 +4,35 @@
 // ReportComputer) reports slot s of a clause when every literal is false
 // ("falsified") or when exactly one literal occurrence is undefined and every
 // other one is false ("unit").  The sweep evaluates it in an equivalent form
-// that reads table rows, where row l holds the slots in which literal l is
-// false (so literal l is true exactly where row l^1 is set):
+// over table rows, where row l holds nf_l = the slots in which literal l is
+// NOT false (true or undefined); literal l is true exactly where its negation
+// l^1 is false, i.e. outside nf_{l^1}:
 //
 //   count   c1/c2 = "at least one / at least two non-false occurrences",
-//           saturating:  c2 |= c1 & ~f_l;  c1 |= ~f_l
-//   true    t = OR over the clause of f_{l^1} (some literal is true)
-//   report  falsified = ~c1;  unit = c1 & ~c2 & ~t
+//           saturating:  c2 |= c1 & nf_l;  c1 |= nf_l
+//   true    nt = AND over the clause of nf_{l^1} (no literal is true)
+//   report  falsified = ~c1;  unit = c1 & ~c2 & nt
 //
-// A slot with c2 set can never report, nor one with t set, so a clause stops
-// once every valid slot of every word of its group is dead.  Two schedules
-// give the same answer:
-//   two-pass  pass 1 reads only rows f_l (1 bit per slot per literal) and
-//             stops on c2; pass 2 reads rows f_{l^1} only if some slot has
-//             exactly one non-false occurrence, and stops when those slots
-//             are resolved;
-//   fused     one pass reads both rows of each literal (adjacent in memory,
-//             rows 2v and 2v+1) and stops on c2 | t.
+// A slot with c2 set, or outside nt, can never report, so a clause stops once
+// every valid slot of every word of its group is dead.  Two schedules give the
+// same answer:
+//   two-pass  pass 1 reads only rows nf_l (1 bit per slot per literal) and
+//             stops on c2; pass 2 reads rows nf_{l^1} only if some slot has
+//             exactly one non-false occurrence, and stops once those slots are
+//             resolved;
+//   fused     one pass reads both rows of each literal (adjacent in memory)
+//             and stops on c2 | ~nt.
 // A clause of at most `fuse_len` literals uses the fused schedule, longer
 // ones the two-pass one.  Duplicate literals are separate occurrences, as in
 // the reference (GpuRunner.cu:68-89 walks the literal list), so "x, x" with x
 // undefined is not unit.  tests/test_oracle.py and the GPU parity tests check
 // the forms agree.
+//
+// Clause literals are stored on the device as row offsets: literal l as
+// l << 7, the byte offset of its row in a table of 16 words (scaled by the
+// actual width at compile time); negation is "xor 128".  Padding lanes past
+// the end of a clause hold the offset of a sentinel row pair (nf = 0, and
+// all ones for its negation) that contributes nothing, so they need no mask.
 //
 // Work mapping.  A group of G = W*S lanes owns one clause: lane (s, w) handles
 // table word w and literal sub-lane s.  The group loads G literals of the
@@ -36,8 +43,8 @@
 // flight), the S sub-lanes' counts are merged with a butterfly, then the group
 // votes on early exit.  With W = 16 (1024 slots) a row read by the 16 lanes of
 // a group is one coalesced 128-byte line.  A wave holds 64/G groups; a task is
-// a run of clauses of one length bucket; waves are persistent and take tasks
-// round-robin.
+// a run of clauses of one length bucket (its parameters are wave-uniform and
+// live in scalar registers); waves are persistent and take tasks round-robin.
 //
 // Hits.  A clause/word with a report becomes one 24-byte record.  Each wave
 // compacts its records with a ballot + mbcnt prefix and appends them to a
@@ -61,87 +68,103 @@ __device__ __forceinline__ uint32_t prefix_count(unsigned long long bal) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
 }
 
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 template <int G>
-__device__ __forceinline__ bool group_any(bool pred, int g) {
-    unsigned long long b = __ballot(pred);
+__device__ __forceinline__ bool group_any(bool pred, uint64_t gmask) {
+    const unsigned long long b = __ballot(pred);
     if constexpr (G == 64) {
         return b != 0;
     } else {
-        return ((b >> (g * G)) & ((1ull << G) - 1ull)) != 0;
+        return (b & gmask) != 0;
     }
 }
 
-// Literal J of the group's preloaded chunk, for every lane of the group.
-// With G = 16 a group is one DPP row, so this is a row broadcast (no LDS).
-template <int G, int J>
-__device__ __forceinline__ uint32_t chunk_lit(uint32_t litreg, int gbase, int sub) {
-    if constexpr (G == 16) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)litreg, 0x150 + J, 0xF, 0xF, false);
-    } else {
-        return (uint32_t)__shfl((int)litreg, gbase + J);
-    }
-}
-
-// Row of literal `lit` in word w (32-bit byte offsets: the host keeps the
-// table under 4 GiB).
+// Byte offset inside the table of row word w, from a stored row offset
+// (literal << 7, i.e. the offset for a 16-word table).
 template <int W>
-__device__ __forceinline__ uint64_t row_word(const uint64_t *__restrict__ table, uint32_t lit,
-                                             int w) {
-    const uint32_t off = lit * (uint32_t)(W * 8) + (uint32_t)w * 8u;
+__device__ __forceinline__ uint32_t scale_offset(uint32_t stored) {
+    if constexpr (W == 16) return stored;
+    else if constexpr (W > 16) return stored << (W == 32 ? 1 : 2);
+    else return stored >> (W == 8 ? 1 : W == 4 ? 2 : W == 2 ? 3 : 4);
+}
+
+// Address of row word w of the literal at chunk position J, for every lane of
+// the group.  With G = 16 a group is one DPP row, so the literal is a row
+// broadcast (no LDS), fused with the add of the lane's word offset.
+template <int W, int G, int J>
+__device__ __forceinline__ uint32_t chunk_addr(uint32_t litreg, int gbase, uint32_t w8) {
+    if constexpr (G == 16 && W == 16) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)litreg, 0x150 + J, 0xF, 0xF, false) | w8;
+    } else if constexpr (G == 16) {
+        return scale_offset<W>(
+                   (uint32_t)__builtin_amdgcn_mov_dpp((int)litreg, 0x150 + J, 0xF, 0xF, false)) |
+               w8;
+    } else {
+        return scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + J)) | w8;
+    }
+}
+
+__device__ __forceinline__ uint64_t load_row(const uint64_t *__restrict__ table, uint32_t off) {
     return *(const uint64_t *)((const char *)table + off);
 }
 
 // Clause state of one lane: saturating count of non-false occurrences per slot
-// (c1: >= 1, c2: >= 2) and the slots where some literal is true.
+// (c1: >= 1, c2: >= 2) and the slots where no literal is true (nt).
 struct ClauseState {
-    uint64_t c1, c2, any_t;
+    uint64_t c1, c2, nt;
 };
 
+// Offset of the negated literal's row: stored offsets are literal << 7, so the
+// negation flips bit 7, scaled like the offset.
+template <int W>
+__device__ __forceinline__ uint32_t neg_bit() {
+    return scale_offset<W>(128u);
+}
+
 // One step: U literals per lane.  The chunk holds the group's literals
-// [base, base + G) with the sentinel literal past the clause end, whose rows
-// contribute nothing; this step covers chunk positions [ST, ST + S*U).  The
-// U row reads are issued back to back, folded into the state, and for S > 1
-// the S sub-lanes are merged with a butterfly.
+// [base, base + G), sentinel past the clause end; this step covers chunk
+// positions [ST, ST + S*U).  The U row reads are issued back to back, folded
+// into the state, and for S > 1 the S sub-lanes are merged with a butterfly.
 template <int W, int S, int U, int ST, bool ROWS_F, bool ROWS_T>
-__device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub,
-                                          const uint64_t *__restrict__ table, int w,
-                                          ClauseState &cs) {
+__device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, uint32_t w8,
+                                          const uint64_t *__restrict__ table, ClauseState &cs) {
     constexpr int G = W * S;
     uint64_t f[U], t[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        uint32_t lit;
+        uint32_t off;
         if constexpr (S == 1 && U <= 4) {
-            lit = u == 0 ? chunk_lit<G, ST + 0>(litreg, gbase, sub)
-                : u == 1 ? chunk_lit<G, (ST + 1) % G>(litreg, gbase, sub)
-                : u == 2 ? chunk_lit<G, (ST + 2) % G>(litreg, gbase, sub)
-                         : chunk_lit<G, (ST + 3) % G>(litreg, gbase, sub);
+            off = u == 0 ? chunk_addr<W, G, ST + 0>(litreg, gbase, w8)
+                : u == 1 ? chunk_addr<W, G, (ST + 1) % G>(litreg, gbase, w8)
+                : u == 2 ? chunk_addr<W, G, (ST + 2) % G>(litreg, gbase, w8)
+                         : chunk_addr<W, G, (ST + 3) % G>(litreg, gbase, w8);
         } else if constexpr (S == 1) {
-            lit = (uint32_t)__shfl((int)litreg, gbase + ST + u);
+            off = scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + ST + u)) | w8;
         } else {
-            lit = (uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u);
+            off = scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u)) | w8;
         }
-        if (ROWS_F) f[u] = row_word<W>(table, lit, w);
-        if (ROWS_T) t[u] = row_word<W>(table, lit ^ 1u, w);
+        if (ROWS_F) f[u] = load_row(table, off);
+        if (ROWS_T) t[u] = load_row(table, off ^ neg_bit<W>());
     }
     if constexpr (S == 1) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (ROWS_F) {
-                cs.c2 |= cs.c1 & ~f[u];
-                cs.c1 |= ~f[u];
+                cs.c2 |= cs.c1 & f[u];
+                cs.c1 |= f[u];
             }
-            if (ROWS_T) cs.any_t |= t[u];
+            if (ROWS_T) cs.nt &= t[u];
         }
     } else {
-        uint64_t m1 = 0, m2 = 0, tt = 0;
+        uint64_t m1 = 0, m2 = 0, tt = ~0ull;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (ROWS_F) {
-                m2 |= m1 & ~f[u];
-                m1 |= ~f[u];
+                m2 |= m1 & f[u];
+                m1 |= f[u];
             }
-            if (ROWS_T) tt |= t[u];
+            if (ROWS_T) tt &= t[u];
         }
 #pragma unroll
         for (int d = W; d < G; d <<= 1) {
@@ -151,29 +174,30 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub,
                 m2 = m2 | p2 | (m1 & p1);
                 m1 |= p1;
             }
-            if (ROWS_T) tt |= __shfl_xor(tt, d);
+            if (ROWS_T) tt &= __shfl_xor(tt, d);
         }
         if (ROWS_F) {
             cs.c2 |= m2 | (cs.c1 & m1);
             cs.c1 |= m1;
         }
-        if (ROWS_T) cs.any_t |= tt;
+        if (ROWS_T) cs.nt &= tt;
     }
 }
 
 // Runs the steps of one preloaded chunk while they start before the clause
 // end and `more(cs)` holds.  Returns false if it stopped early.
 template <int W, int S, int U, bool ROWS_F, bool ROWS_T, typename More>
-__device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub, int base,
-                                            int len, const uint64_t *__restrict__ table, int w,
-                                            ClauseState &cs, More &&more) {
+__device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub, uint32_t w8,
+                                            int base, int len,
+                                            const uint64_t *__restrict__ table, ClauseState &cs,
+                                            More &&more) {
     constexpr int G = W * S;
     constexpr int LPS = S * U;
     bool go = true;
     [&]<int... I>(std::integer_sequence<int, I...>) {
         ((go = go && (base + I * LPS < len) &&
                [&] {
-                   step_rows<W, S, U, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, table, w, cs);
+                   step_rows<W, S, U, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, w8, table, cs);
                    return more(cs);
                }()),
          ...);
@@ -188,37 +212,39 @@ __device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub,
 template <int W, int S, int U>
 __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, int len,
                                              uint32_t first_chunk, uint32_t sentinel,
-                                             const uint64_t *__restrict__ table, int q, int g,
-                                             int w, int sub, uint64_t valid, int fuse_len,
-                                             uint64_t &fm, uint64_t &um) {
+                                             const uint64_t *__restrict__ table, int q,
+                                             uint64_t gmask, int gbase, uint32_t w8, int sub,
+                                             uint64_t valid, int fuse_len, uint64_t &fm,
+                                             uint64_t &um) {
     constexpr int G = W * S;
-    const int gbase = g * G;
     // invalid slots start dead
-    ClauseState cs{~valid, ~valid, 0};
+    ClauseState cs{~valid, ~valid, ~0ull};
     const bool fused = len <= fuse_len;
-    auto alive = [&](const ClauseState &c) { return group_any<G>(~(c.c2 | c.any_t) != 0, g); };
+    auto alive = [&](const ClauseState &c) {
+        return group_any<G>((c.nt & ~c.c2) != 0, gmask);
+    };
     uint32_t litreg = first_chunk;
     for (int base = 0; base < len; base += G) {
         if (base) litreg = base + q < len ? cl[base + q] : sentinel;
-        const bool go = fused ? chunk_steps<W, S, U, true, true>(litreg, gbase, sub, base, len,
-                                                                 table, w, cs, alive)
-                              : chunk_steps<W, S, U, true, false>(litreg, gbase, sub, base, len,
-                                                                  table, w, cs, alive);
+        const bool go = fused ? chunk_steps<W, S, U, true, true>(litreg, gbase, sub, w8, base, len,
+                                                                 table, cs, alive)
+                              : chunk_steps<W, S, U, true, false>(litreg, gbase, sub, w8, base,
+                                                                  len, table, cs, alive);
         if (!go) break;  // dead, or the clause ended inside this chunk
     }
     fm = ~cs.c1;
-    const uint64_t cand = cs.c1 & ~cs.c2 & ~cs.any_t;
-    if (!fused && group_any<G>(cand != 0, g)) {
-        auto open = [&](const ClauseState &c) { return group_any<G>((cand & ~c.any_t) != 0, g); };
+    const uint64_t cand = cs.c1 & ~cs.c2 & cs.nt;
+    if (!fused && group_any<G>(cand != 0, gmask)) {
+        auto open = [&](const ClauseState &c) { return group_any<G>((cand & c.nt) != 0, gmask); };
         litreg = first_chunk;
         for (int base = 0; base < len; base += G) {
             if (base) litreg = base + q < len ? cl[base + q] : sentinel;
-            if (!chunk_steps<W, S, U, false, true>(litreg, gbase, sub, base, len, table, w, cs,
+            if (!chunk_steps<W, S, U, false, true>(litreg, gbase, sub, w8, base, len, table, cs,
                                                    open))
                 break;
         }
     }
-    um = cand & ~cs.any_t;
+    um = cand & cs.nt;
 }
 
 __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int nb, int task) {
@@ -240,18 +266,22 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     const int q = lane % G;
     const int sub = q / W;
     const int w = lane % W;
+    const int gbase = g * G;
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    const uint32_t w8 = (uint32_t)w * 8u;
     const uint64_t valid = a.valid[w];
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    // wave id and count are wave-uniform: keep the task walk scalar
+    const int gw = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nw = (int)((gridDim.x * blockDim.x) >> 6);
 
     int chunk = -1;
     int fill = 0;
     for (int task = gw; task < a.ntasks; task += nw) {
-        const int bi = find_bucket(a.buckets, a.nbuckets, task);
+        const int bi = uniform(find_bucket(a.buckets, a.nbuckets, task));
         const BucketDesc bd = a.buckets[bi];
-        const int len = (int)bd.len;
-        const int64_t row0 = (int64_t)(task - (int)bd.task_begin) * a.task_clauses;
-        const int64_t rem = (int64_t)bd.count - row0;
+        const int len = uniform((int)bd.len);
+        const int64_t row0 = (int64_t)(task - uniform((int)bd.task_begin)) * a.task_clauses;
+        const int64_t rem = (int64_t)uniform((int)bd.count) - row0;
         const int nrows = rem < a.task_clauses ? (int)rem : a.task_clauses;
         const uint32_t *rows_base = bd.lits + (size_t)row0 * len;
         // literal q of the group's first clause chunk, prefetched one pass ahead
@@ -266,7 +296,8 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
             uint64_t fm = 0, um = 0;
             if (active) {
                 check_clause<W, S, U>(bd.lits + (size_t)row * len, len, chunk0, a.sentinel,
-                                      a.table, q, g, w, sub, valid, a.fuse_len, fm, um);
+                                      a.table, q, gmask, gbase, w8, sub, valid, a.fuse_len, fm,
+                                      um);
             }
             const bool hit = active && sub == 0 && (fm | um) != 0;
             const unsigned long long bal = __ballot(hit);
@@ -277,7 +308,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                         a.fills[chunk] = (uint32_t)fill;
                     uint32_t c = 0;
                     if (lane == 0) c = atomicAdd(a.alloc, 1u);
-                    chunk = (int)__shfl(c, 0);
+                    chunk = uniform((int)__shfl(c, 0));
                     fill = 0;
                 }
                 if (hit && (uint32_t)chunk < a.chunk_cap) {
@@ -307,24 +338,27 @@ __global__ void k_fill_u64(uint64_t *p, int64_t n, uint64_t v) {
         p[i] = v;
 }
 
+// Copies segments of uint32; with shift > 0 each element is shifted left
+// (literal -> stored row offset on clause upload).
 __global__ void k_scatter(const ScatterSeg *__restrict__ segs, int nseg,
-                          const uint32_t *__restrict__ src) {
+                          const uint32_t *__restrict__ src, int shift) {
     for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
         const ScatterSeg sg = segs[s];
-        for (uint64_t i = threadIdx.x; i < sg.count; i += blockDim.x) sg.dst[i] = src[sg.src + i];
+        for (uint64_t i = threadIdx.x; i < sg.count; i += blockDim.x)
+            sg.dst[i] = src[sg.src + i] << shift;
     }
 }
 
-// Row 2v of the table holds the slots where v is false (literal 2v false),
-// row 2v+1 the slots where v is true (literal 2v+1 false).
+// Row 2v holds the slots where literal 2v (v) is not false, i.e. v is not
+// false; row 2v+1 the slots where v is not true.
 __global__ void k_assig_apply(const gss_var_update *__restrict__ ups, int64_t n,
                               uint64_t *__restrict__ table, int words) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const gss_var_update u = ups[i];
     const size_t base = (size_t)u.var * 2 * words + u.word;
-    table[base] = u.is_def & ~u.is_true;
-    table[base + words] = u.is_def & u.is_true;
+    table[base] = ~(u.is_def & ~u.is_true);
+    table[base + words] = ~(u.is_def & u.is_true);
 }
 
 __global__ void k_assig_roll(const gss_var_update *__restrict__ ups, int64_t n,
@@ -424,10 +458,11 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, hipStream_t s) {
+hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, int shift,
+                          hipStream_t s) {
     if (nseg <= 0) return hipSuccess;
     const int blocks = nseg < 4096 ? nseg : 4096;
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, segs, nseg, src);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, segs, nseg, src, shift);
     return hipGetLastError();
 }
 
