@@ -33,6 +33,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = ("lit×assig checks/s + %HBM roofline @4M cl×1024 asg; p99 report latency")
+# HBM bytes per sweep launch of the default kernel on this workload, from two
+# rocprofv3 PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, KiB, the
+# factors calibrated on the clause-upload copy kernel of known size); the
+# commands and raw figures are in profiles/k_sweep_sweep1024.md.  None when
+# the instance is not the headline one.
+MEASURED_TRAFFIC_KIB = {("k_sweep<16,1,4>", 1.0): 2 * 2_074_122 + 427_740}
 
 
 def log(msg: str) -> None:
@@ -165,8 +171,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_sweep<16,1>",
+            "traffic": (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4>", args.scale)] * 1024
+                        if ("k_sweep<16,1,4>", args.scale) in MEASURED_TRAFFIC_KIB else None),
+            "traffic_unit": "bytes per launch",
+            "kernel": "k_sweep<16,1,4>",
             "kernel_ms": k_avg_s * 1e3,
             "algorithmic_bytes": alg_bytes,
         },

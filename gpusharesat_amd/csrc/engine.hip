@@ -172,6 +172,7 @@ struct gss_engine {
     int tune_max_chunks = 0;
     int tune_unroll = 0;
     int tune_fuse_len = 0;
+    int tune_schedule = 0;
 
     // staging
     Pinned<uint32_t> h_stage;
@@ -689,7 +690,11 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     HIP_TRY(hipStreamSynchronize(e->stream));  // pinned descriptors / valid reused
     int64_t tasks = 0;
     int k = 0;
-    for (size_t len = 1; len < e->buckets.size(); len++) {
+    const size_t nlen = e->buckets.size();
+    for (size_t i = 1; i < nlen; i++) {
+        // schedule 1 numbers the tasks of long clauses first (they are the
+        // most expensive), so the persistent waves finish together
+        const size_t len = e->tune_schedule == 1 ? nlen - i : i;
         const Bucket &b = e->buckets[len];
         if (!b.count) continue;
         if (b.count > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
@@ -858,6 +863,10 @@ int gss_set_tuning(gss_engine *e, int key, int value) {
             e->tune_unroll = value;
             return GSS_OK;
         case GSS_TUNE_FUSE_LEN: e->tune_fuse_len = value; return GSS_OK;
+        case GSS_TUNE_SCHEDULE:
+            if (value > 1) return set_err(GSS_E_INVAL, "schedule must be 0 or 1");
+            e->tune_schedule = value;
+            return GSS_OK;
         case GSS_TUNE_MAX_HIT_CHUNKS:
             e->tune_max_chunks = value;
             if (value > 0) {
@@ -869,6 +878,71 @@ int gss_set_tuning(gss_engine *e, int key, int value) {
             return GSS_OK;
         default: return set_err(GSS_E_INVAL, "unknown tuning key %d", key);
     }
+}
+
+int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, float *best_ms) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!valid || reps < 1) return set_err(GSS_E_INVAL, "bad autotune arguments");
+    if (e->pending) return set_err(GSS_E_STATE, "a sweep is in flight");
+    struct Cfg {
+        int unroll, fuse, task, wpc, sched;
+    };
+    std::vector<Cfg> grid;
+    const int unrolls[] = {2, 4};
+    const int fuses[] = {1, 4, 8};
+    const int tasks[] = {0, 32, 128};
+    const int wpcs[] = {0, 24};
+    for (int u : unrolls)
+        for (int f : fuses)
+            for (int t : tasks)
+                for (int w : wpcs)
+                    for (int sc = 0; sc < 2; sc++) grid.push_back({u, f, t, w, sc});
+    const Cfg saved{e->tune_unroll, e->tune_fuse_len, e->tune_task_clauses, e->tune_waves_per_cu,
+                    e->tune_schedule};
+    auto apply = [&](const Cfg &c) {
+        e->tune_unroll = e->words == 16 ? c.unroll : 0;
+        e->tune_fuse_len = c.fuse;
+        e->tune_task_clauses = c.task;
+        e->tune_waves_per_cu = c.wpc;
+        e->tune_schedule = c.sched;
+    };
+    double total0 = e->total_ms;
+    int64_t count0 = e->nsweeps;
+    Cfg best_cfg = saved;
+    float best_t = 0.f;
+    bool have = false;
+    for (const Cfg &c : grid) {
+        apply(c);
+        std::vector<float> ts;
+        for (int r = 0; r < reps + 1; r++) {
+            if ((rc = gss_sweep(e, valid, nullptr))) {
+                apply(saved);
+                return rc;
+            }
+            if (r) ts.push_back(e->last_ms);  // first run warms up
+        }
+        std::sort(ts.begin(), ts.end());
+        const float med = ts[ts.size() / 2];
+        if (!have || med < best_t) {
+            best_t = med;
+            best_cfg = c;
+            have = true;
+        }
+    }
+    apply(best_cfg);
+    // autotuning sweeps are not part of the caller's kernel statistics
+    e->total_ms = total0;
+    e->nsweeps = count0;
+    if (best) {
+        best[0] = best_cfg.unroll;
+        best[1] = best_cfg.fuse;
+        best[2] = best_cfg.task;
+        best[3] = best_cfg.wpc;
+        best[4] = best_cfg.sched;
+    }
+    if (best_ms) *best_ms = best_t;
+    return GSS_OK;
 }
 
 }  // extern "C"

@@ -23,6 +23,7 @@ TUNE_TASK_CLAUSES = 2
 TUNE_MAX_HIT_CHUNKS = 3
 TUNE_UNROLL = 4
 TUNE_FUSE_LEN = 5
+TUNE_SCHEDULE = 6
 MAX_WORDS = 64
 
 
@@ -201,6 +202,15 @@ class Engine:
 
     def set_tuning(self, key: int, value: int) -> None:
         _lib.call("gss_set_tuning", self._h, key, value)
+
+    def autotune(self, valid, reps: int = 2) -> tuple[dict, float]:
+        """Picks the fastest sweep settings for the current DB and table."""
+        v = self._valid(valid)
+        best = (ctypes.c_int * 5)()
+        ms = ctypes.c_float()
+        _lib.call("gss_autotune", self._h, _ptr(v), reps, best, ctypes.byref(ms))
+        keys = ("unroll", "fuse_len", "task_clauses", "waves_per_cu", "schedule")
+        return dict(zip(keys, list(best))), ms.value
 
 
 class ClauseMap:

@@ -162,6 +162,16 @@ int gss_set_tuning(gss_engine *e, int key, int value);
 #define GSS_TUNE_FUSE_LEN 5       /* clauses of at most this many literals read
                                      both rows of a literal in one pass
                                      (0 = default 4) */
+#define GSS_TUNE_SCHEDULE 6       /* wave-to-bucket schedule: 0 = tasks of
+                                     short clauses first, 1 = long first */
+
+/* Autotuner: times the sweep on the current clause DB and assignment table
+ * under a grid of (rows in flight, fuse length, clauses per task, waves per
+ * CU, schedule) settings, `reps` sweeps each, and keeps the fastest.  The
+ * chosen values are written to best[0..4] in that order (may be NULL) and
+ * the best average kernel time to *best_ms.  Replaces the reference's fixed
+ * launch guidelines (GpuClauseSharerImpl.cu:50-57). */
+int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, float *best_ms);
 
 #ifdef __cplusplus
 }
