@@ -57,6 +57,9 @@ SIGNATURES: dict[str, list] = {
     "gss_sweep_kernel_stats": [c_p, c_p, c_p, c_int],
     "gss_set_tuning": [c_p, c_int, c_int],
     "gss_autotune": [c_p, c_p, c_int, c_p, c_p],
+    "gss_step_enqueue": [c_p, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p],
+    "gss_step_wait": [c_p, c_p, c_p],
+    "gss_step_hits_copy": [c_p, c_p, c_i64, c_p],
     # include/gss_sharer.h
     "gss_sharer_default_options": [c_p],
     "gss_sharer_create": [c_p, c_p, c_p, c_p],

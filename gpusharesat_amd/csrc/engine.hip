@@ -121,6 +121,36 @@ struct DevBuf {
     }
 };
 
+// One of the two ping-pong slots of batched steps: pinned staging the host
+// fills, the device copies the step's kernels read, this slot's hit buffer,
+// and the graph captured over exactly these buffers.
+struct StepSlot {
+    Pinned<StepHeader> h_hdr;
+    Pinned<BucketDesc> h_desc;
+    Pinned<ScatterSeg> h_segs;
+    Pinned<uint32_t> h_lits;
+    Pinned<gss_var_update> h_ups;
+    Pinned<uint32_t> h_out;  // [0] = chunks claimed, [1 + c] = fill of chunk c
+    DevBuf<StepHeader> d_hdr;
+    DevBuf<BucketDesc> d_desc;
+    DevBuf<ScatterSeg> d_segs;
+    DevBuf<uint32_t> d_lits;
+    DevBuf<gss_var_update> d_ups;
+    DevBuf<gss_hit> d_hits;
+    DevBuf<uint32_t> d_fills;
+    DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
+    Pinned<ScatterSeg> h_cseg;   // compaction segments (one per non-empty chunk)
+    DevBuf<ScatterSeg> d_cseg;
+    uint32_t *d_alloc = nullptr;
+    uint32_t chunk_cap = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool busy = false;
+    bool results = false;  // h_out / d_hits hold the results of the last waited step
+    std::vector<const void *> key;  // everything baked into the captured graph
+};
+
 }  // namespace
 
 int gss::record_error(int code, const char *msg) {
@@ -132,6 +162,8 @@ struct gss_engine {
     int device = 0;
     int cu = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // step result copies, overlapping the next step
+    hipStream_t capture_stream = nullptr;  // records step graphs (never executes work)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     std::vector<Bucket> buckets;  // index = clause length
@@ -187,6 +219,15 @@ struct gss_engine {
     DevBuf<uint32_t> d_offsets;
     DevBuf<gss_hit> d_dense;
     Pinned<gss_hit> h_hits;
+
+    // batched steps (gss_step_*)
+    StepSlot slots[2];
+    int next_slot = 0;
+    std::vector<int> inflight;  // slot indices, oldest first
+    int last_step_slot = -1;
+    int64_t last_step_hits = 0;
+    uint32_t last_step_chunks = 0;
+    float last_step_ms = 0.f;
 };
 
 namespace {
@@ -230,7 +271,8 @@ int launch_current_sweep(gss_engine *e) {
     HIP_TRY(launch_reset_u32(e->d_alloc, 1, e->stream));
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
     if (e->ntasks > 0) {
-        SweepArgs a;
+        SweepArgs a{};
+        a.dyn = nullptr;  // counts passed by value
         a.buckets = e->d_desc.p;
         a.nbuckets = e->nbuckets;
         a.ntasks = e->ntasks;
@@ -282,7 +324,8 @@ int gss_engine_create(int device, gss_engine **out) {
     };
     if (hipDeviceGetAttribute(&e->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "cannot query CU count"));
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "stream creation failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "event creation failed"));
@@ -302,6 +345,23 @@ int gss_engine_destroy(gss_engine *e) {
     if (!e) return GSS_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->copy_stream) (void)hipStreamSynchronize(e->copy_stream);
+    for (StepSlot &sl : e->slots) {
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        if (sl.t0) (void)hipEventDestroy(sl.t0);
+        if (sl.t1) (void)hipEventDestroy(sl.t1);
+        if (sl.d_alloc) (void)hipFree(sl.d_alloc);
+        sl.d_hdr.release();
+        sl.d_desc.release();
+        sl.d_segs.release();
+        sl.d_lits.release();
+        sl.d_ups.release();
+        sl.d_hits.release();
+        sl.d_fills.release();
+        sl.d_dense.release();
+        sl.d_cseg.release();
+    }
     for (Bucket &b : e->buckets)
         if (b.d) (void)hipFree(b.d);
     e->table.release();
@@ -319,6 +379,8 @@ int gss_engine_destroy(gss_engine *e) {
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
+    if (e->capture_stream) (void)hipStreamDestroy(e->capture_stream);
     delete e;
     return GSS_OK;
 }
@@ -503,6 +565,10 @@ static int table_fits(int64_t vars, int words) {
     if ((uint64_t)table_rows(vars) * (uint64_t)words * 8ull > 0xFFFFFFFFull)
         return set_err(GSS_E_INVAL, "assignment table of %lld vars x %d words exceeds 4 GiB",
                        (long long)vars, words);
+    // every row, the two sentinels included, must have a 32-bit stored offset
+    if (table_rows(vars) - 1 > (int64_t)kMaxLit)
+        return set_err(GSS_E_INVAL, "%lld variables exceed the %u the table can address",
+                       (long long)vars, (kMaxLit - 1) / 2);
     return GSS_OK;
 }
 
@@ -942,6 +1008,327 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
         best[4] = best_cfg.sched;
     }
     if (best_ms) *best_ms = best_t;
+    return GSS_OK;
+}
+
+}  // extern "C"
+
+/* ---- batched steps -------------------------------------------------------- */
+
+// Persistent grid of a step: enough 4-wave blocks for the tasks, rounded up
+// to a power of two (so a growing DB re-captures the graph rarely), at most
+// the resident maximum.
+static int step_blocks(gss_engine *e, int64_t tasks) {
+    const int li = log2_exact(e->words);
+    const int ui = std::min(e->tune_unroll, 8);
+    if (!e->occ[li][ui]) e->occ[li][ui] = sweep_occupancy(e->words, e->tune_unroll);
+    int per_cu = e->occ[li][ui];
+    if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
+    const int64_t full = (int64_t)e->cu * per_cu;
+    int64_t b = 1;
+    while (b < (tasks + 3) / 4 && b < full) b *= 2;
+    return (int)std::min(b, full);
+}
+
+// Captures the step's fixed launch sequence over this slot's buffers.
+static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse) {
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    sl.exec = nullptr;
+    sl.graph = nullptr;
+    if (!e->capture_stream)
+        HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = e->capture_stream;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    StageArgs st;
+    st.h_hdr = sl.h_hdr.p;
+    st.h_desc = sl.h_desc.p;
+    st.h_segs = sl.h_segs.p;
+    st.h_lits = sl.h_lits.p;
+    st.h_ups = sl.h_ups.p;
+    st.d_hdr = sl.d_hdr.p;
+    st.d_desc = sl.d_desc.p;
+    st.d_segs = sl.d_segs.p;
+    st.d_lits = sl.d_lits.p;
+    st.d_ups = sl.d_ups.p;
+    StepHeader *dh = sl.d_hdr.p;
+    SweepArgs a{};
+    a.buckets = sl.d_desc.p;
+    a.dyn = &dh->nbuckets;  // {nbuckets, ntasks}
+    a.task_clauses = tc;
+    a.table = e->table.p;
+    a.valid = dh->valid;
+    a.hits = sl.d_hits.p;
+    a.fills = sl.d_fills.p;
+    a.alloc = sl.d_alloc;
+    a.chunk_cap = sl.chunk_cap;
+    a.fuse_len = fuse;
+    a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
+    hipError_t he = hipSuccess;
+    if (he == hipSuccess) he = launch_stage_in(st, cs);
+    if (he == hipSuccess) he = launch_scatter_dyn(sl.d_segs.p, &dh->nsegs, sl.d_lits.p, kLitShift, cs);
+    if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
+    if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
+    if (he == hipSuccess) he = launch_sweep(a, e->words, e->tune_unroll, blocks, cs);
+    if (he == hipSuccess)
+        he = launch_assig_roll_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, dh->last_pos,
+                                   cs);
+    if (he == hipSuccess)
+        he = launch_stage_out(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.h_out.p, cs);
+    hipGraph_t g = nullptr;
+    hipError_t he2 = hipStreamEndCapture(cs, &g);
+    if (he != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return set_err(GSS_E_HIP, "step capture failed: %s", hipGetErrorString(he));
+    }
+    if (he2 != hipSuccess) return set_err(GSS_E_HIP, "step capture: %s", hipGetErrorString(he2));
+    sl.graph = g;
+    HIP_TRY(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    return GSS_OK;
+}
+
+extern "C" {
+
+int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                     uint32_t *out_rows, int64_t n_ups, const gss_var_update *ups,
+                     const uint64_t *valid, const int8_t *last_pos) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!lits || !lens)) || n_ups < 0 || (n_ups > 0 && !ups) || !valid)
+        return set_err(GSS_E_INVAL, "bad step arguments");
+    if (e->pending) return set_err(GSS_E_STATE, "a plain sweep is in flight");
+    const int S = e->next_slot;
+    StepSlot &sl = e->slots[S];
+    if (sl.busy) return set_err(GSS_E_STATE, "two steps in flight: wait for the oldest first");
+    const int W = e->words;
+    // ---- validation (nothing changes before it passes)
+    for (int64_t i = 0; i < n_ups; i++)
+        if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
+            return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
+                           (long long)i, ups[i].var, ups[i].word);
+    int active_words = 0;
+    for (int w = 0; w < W; w++) {
+        if (last_pos && last_pos[w] > 63) return set_err(GSS_E_INVAL, "last_pos > 63");
+        if (valid[w]) active_words++;
+    }
+    int maxlen = 0;
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
+            return set_err(GSS_E_INVAL, "clause %lld: length %d outside [1, %d]", (long long)i,
+                           lens[i], GSS_MAX_CLAUSE_LEN);
+        maxlen = std::max(maxlen, (int)lens[i]);
+        total += lens[i];
+    }
+    uint32_t mx = 0;
+    for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
+    if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
+    const int64_t need_lit = std::max<int64_t>(n ? (int64_t)mx : -1, e->max_lit);
+    if (need_lit >= 2 * e->vars)
+        return set_err(GSS_E_STATE, "clause literal %lld needs variable %lld but the table has "
+                                    "%lld variables; grow the table before the step",
+                       (long long)need_lit, (long long)(need_lit >> 1), (long long)e->vars);
+    std::vector<int64_t> per(maxlen + 1, 0);
+    for (int64_t i = 0; i < n; i++) per[lens[i]]++;
+    // ---- capacity: buckets, staging, device copies, hit buffer
+    if ((int)e->buckets.size() <= maxlen) e->buckets.resize(maxlen + 1);  // empty buckets
+    bool grow = false;
+    for (int len = 1; len <= maxlen; len++)
+        if (per[len] && e->buckets[len].count + per[len] > e->buckets[len].cap) grow = true;
+    if (grow) {
+        // bucket arrays move: steps in flight must be done with them
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (int len = 1; len <= maxlen; len++)
+            if (per[len] && (rc = ensure_bucket_cap(e, len, e->buckets[len].count + per[len])))
+                return rc;
+    }
+    const int cpp = sweep_clauses_per_pass(W);
+    int tc = e->tune_task_clauses > 0 ? e->tune_task_clauses : 16 * cpp;
+    tc = std::max(cpp, (tc / cpp) * cpp);
+    const size_t nlen = std::max(e->buckets.size(), per.size());
+    auto rows_after = [&](size_t len) -> int64_t {
+        const int64_t c = len < e->buckets.size() ? e->buckets[len].count : 0;
+        return c + (len < per.size() ? per[len] : 0);
+    };
+    int nb = 0, nseg = 0;
+    int64_t tasks = 0;
+    for (size_t len = 1; len < nlen; len++) {
+        const int64_t c = rows_after(len);
+        if (c > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
+        if (c) nb++;
+        tasks += (c + tc - 1) / tc;
+        if (len < per.size() && per[len]) nseg++;
+    }
+    if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
+    const int blocks = step_blocks(e, tasks);
+    // an abandoned chunk holds more than kHitChunk - 64 records and each wave
+    // leaves at most one partial chunk: records <= clauses * active words
+    const uint64_t ub = (uint64_t)(e->total_clauses + n) * (uint64_t)active_words;
+    uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
+    if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_lits.reserve(std::max<int64_t>(total, 1))) ||
+        (rc = sl.h_segs.reserve(std::max(nseg, 1))) ||
+        (rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1))) ||
+        (rc = sl.h_desc.reserve(std::max(nb, 1))) || (rc = sl.d_hdr.reserve(1)) ||
+        (rc = sl.d_lits.reserve(sl.h_lits.cap)) || (rc = sl.d_segs.reserve(sl.h_segs.cap)) ||
+        (rc = sl.d_ups.reserve(sl.h_ups.cap)) || (rc = sl.d_desc.reserve(sl.h_desc.cap)))
+        return rc;
+    if (chunks > sl.chunk_cap) {
+        chunks += chunks / 4;  // headroom for the DB growing over the next steps
+        if ((rc = sl.d_hits.reserve((size_t)chunks * kHitChunk)) ||
+            (rc = sl.d_fills.reserve((size_t)chunks)) || (rc = sl.h_out.reserve(chunks + 1)))
+            return rc;
+        sl.chunk_cap = (uint32_t)std::min<size_t>(
+            std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / kHitChunk), sl.h_out.cap - 1);
+    }
+    if (!sl.d_alloc) HIP_TRY(hipMalloc((void **)&sl.d_alloc, 64));
+    if (!sl.t0) HIP_TRY(hipEventCreate(&sl.t0));
+    if (!sl.t1) HIP_TRY(hipEventCreate(&sl.t1));
+    // ---- graph over exactly these buffers and settings
+    const int fuse = e->tune_fuse_len > 0 ? e->tune_fuse_len : kDefaultFuseLen;
+    std::vector<const void *> key = {
+        sl.h_hdr.p, sl.h_desc.p, sl.h_segs.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
+        sl.d_hdr.p, sl.d_desc.p, sl.d_segs.p, sl.d_lits.p, sl.d_ups.p, sl.d_hits.p,
+        sl.d_fills.p, e->table.p,
+        (const void *)(intptr_t)W, (const void *)(intptr_t)e->vars,
+        (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
+        (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
+        (const void *)(intptr_t)sl.chunk_cap};
+    if (!sl.exec || key != sl.key) {
+        sl.key.clear();
+        if ((rc = build_step_graph(e, sl, blocks, tc, fuse))) return rc;
+        sl.key = key;
+    }
+    // ---- stage this step (the slot is idle) and commit it to the DB
+    std::vector<int64_t> seg_off(maxlen + 1, 0), seg_fill(maxlen + 1, 0);
+    {
+        int64_t acc = 0;
+        for (int len = 1; len <= maxlen; len++) {
+            seg_off[len] = acc;
+            acc += per[len] * len;
+        }
+        int64_t pos = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const int len = lens[i];
+            const int64_t k = seg_fill[len]++;
+            // raw literals: the scatter shifts them to row offsets on the device
+            std::memcpy(sl.h_lits.p + seg_off[len] + k * len, lits + pos,
+                        len * sizeof(uint32_t));
+            if (out_rows) out_rows[i] = (uint32_t)(e->buckets[len].count + k);
+            pos += len;
+        }
+    }
+    int si = 0;
+    for (int len = 1; len <= maxlen; len++) {
+        if (!per[len]) continue;
+        Bucket &b = e->buckets[len];
+        ScatterSeg &sg = sl.h_segs.p[si++];
+        sg.dst = b.d + (size_t)b.count * len;
+        sg.src = (uint64_t)seg_off[len];
+        sg.count = (uint64_t)per[len] * len;
+        b.count += per[len];
+    }
+    e->total_clauses += n;
+    e->total_lits += total;
+    if (n) e->max_lit = std::max<int64_t>(e->max_lit, mx);
+    if (n_ups) std::memcpy(sl.h_ups.p, ups, (size_t)n_ups * sizeof(gss_var_update));
+    // descriptors of every non-empty bucket, the step's own inserts included
+    int k = 0;
+    int64_t t = 0;
+    const size_t nb_len = e->buckets.size();
+    for (size_t i = 1; i < nb_len; i++) {
+        const size_t len = e->tune_schedule == 1 ? nb_len - i : i;
+        const Bucket &b = e->buckets[len];
+        if (!b.count) continue;
+        BucketDesc &d = sl.h_desc.p[k++];
+        d.lits = b.d;
+        d.len = (uint32_t)len;
+        d.count = (uint32_t)b.count;
+        d.task_begin = (uint32_t)t;
+        d.pad = 0;
+        t += (b.count + tc - 1) / tc;
+    }
+    StepHeader &h = *sl.h_hdr.p;
+    std::memset(&h, 0, sizeof h);
+    h.nbuckets = k;
+    h.ntasks = (int32_t)t;
+    h.nsegs = si;
+    h.nlits = total;
+    h.nups = n_ups;
+    for (int w = 0; w < GSS_MAX_WORDS; w++) {
+        h.valid[w] = w < W ? valid[w] : 0;
+        h.last_pos[w] = (w < W && last_pos) ? last_pos[w] : -1;
+    }
+    HIP_TRY(hipEventRecord(sl.t0, e->stream));
+    HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
+    HIP_TRY(hipEventRecord(sl.t1, e->stream));
+    sl.busy = true;
+    sl.results = false;
+    e->inflight.push_back(S);
+    e->next_slot ^= 1;
+    return GSS_OK;
+}
+
+int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (e->inflight.empty()) return set_err(GSS_E_STATE, "no step in flight");
+    const int S = e->inflight.front();
+    StepSlot &sl = e->slots[S];
+    HIP_TRY(hipEventSynchronize(sl.t1));
+    e->inflight.erase(e->inflight.begin());
+    sl.busy = false;
+    const uint32_t used = sl.h_out.p[0];
+    if (used > sl.chunk_cap)
+        return set_err(GSS_E_NOMEM, "step hit buffer overflow (%u > %u chunks)", used,
+                       sl.chunk_cap);
+    int64_t tot = 0;
+    for (uint32_t c = 0; c < used; c++) tot += sl.h_out.p[1 + c];
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, sl.t0, sl.t1));
+    sl.results = true;
+    e->last_step_slot = S;
+    e->last_step_hits = tot;
+    e->last_step_chunks = used;
+    e->last_step_ms = ms;
+    if (n_hits) *n_hits = tot;
+    if (step_ms) *step_ms = ms;
+    return GSS_OK;
+}
+
+int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (e->last_step_slot < 0 || !e->slots[e->last_step_slot].results)
+        return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
+    StepSlot &sl = e->slots[e->last_step_slot];
+    if (n) *n = e->last_step_hits;
+    const int64_t want = std::min<int64_t>(cap, e->last_step_hits);
+    if (want <= 0) return GSS_OK;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    const uint32_t nc = e->last_step_chunks;
+    if ((rc = sl.d_dense.reserve((size_t)e->last_step_hits)) || (rc = sl.h_cseg.reserve(nc)) ||
+        (rc = sl.d_cseg.reserve(nc)))
+        return rc;
+    int ns = 0;
+    uint64_t acc = 0;
+    for (uint32_t c = 0; c < nc; c++) {
+        const uint32_t f = sl.h_out.p[1 + c];
+        if (!f) continue;
+        ScatterSeg &sg = sl.h_cseg.p[ns++];
+        sg.dst = (uint32_t *)(sl.d_dense.p + acc);
+        sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
+        sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
+        acc += f;
+    }
+    // the step's kernels are done (waited); the next step may be running on
+    // the engine stream, so the copy goes on its own stream
+    hipStream_t cs = e->copy_stream;
+    HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
+                           hipMemcpyHostToDevice, cs));
+    HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
+    HIP_TRY(hipMemcpyAsync(out, sl.d_dense.p, (size_t)want * sizeof(gss_hit),
+                           hipMemcpyDeviceToHost, cs));
+    HIP_TRY(hipStreamSynchronize(cs));
     return GSS_OK;
 }
 

@@ -34,6 +34,33 @@ struct SweepArgs {
     uint32_t chunk_cap;
     int fuse_len;            // clauses of at most this many literals use the fused schedule
     uint32_t sentinel;       // stored row offset of the sentinel literal
+    const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
+};
+
+// Per-step inputs of a batched step (gss_step_*), written by the host into a
+// pinned staging slot and copied to the device by k_stage_in.
+struct StepHeader {
+    int32_t nbuckets;
+    int32_t ntasks;
+    int32_t nsegs;
+    int32_t pad0;
+    int64_t nlits;
+    int64_t nups;
+    uint64_t valid[GSS_MAX_WORDS];
+    int8_t last_pos[GSS_MAX_WORDS];
+};
+
+struct StageArgs {
+    const StepHeader *h_hdr;  // pinned host
+    const BucketDesc *h_desc;
+    const void *h_segs;
+    const uint32_t *h_lits;
+    const gss_var_update *h_ups;
+    StepHeader *d_hdr;        // device
+    BucketDesc *d_desc;
+    void *d_segs;
+    uint32_t *d_lits;
+    gss_var_update *d_ups;
 };
 
 struct ScatterSeg {
@@ -57,5 +84,17 @@ hipError_t launch_assig_apply(const gss_var_update *ups, int64_t n, uint64_t *ta
 hipError_t launch_assig_roll(const gss_var_update *ups, int64_t n, uint64_t *table, int words,
                              const int8_t *last_pos, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+
+// Fixed-shape launches of a batched step (counts read from the device header,
+// so a captured graph can be replayed with new data).
+hipError_t launch_stage_in(const StageArgs &a, hipStream_t s);
+hipError_t launch_scatter_dyn(const ScatterSeg *segs, const int32_t *nseg, const uint32_t *src,
+                              int shift, hipStream_t s);
+hipError_t launch_assig_apply_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
+                                  int words, hipStream_t s);
+hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
+                                 int words, const int8_t *last_pos, hipStream_t s);
+hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                            uint32_t *h_out, hipStream_t s);
 
 }  // namespace gss

@@ -274,10 +274,15 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     const int gw = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nw = (int)((gridDim.x * blockDim.x) >> 6);
 
+    int nbuckets = a.nbuckets, ntasks = a.ntasks;
+    if (a.dyn) {
+        nbuckets = uniform(a.dyn[0]);
+        ntasks = uniform(a.dyn[1]);
+    }
     int chunk = -1;
     int fill = 0;
-    for (int task = gw; task < a.ntasks; task += nw) {
-        const int bi = uniform(find_bucket(a.buckets, a.nbuckets, task));
+    for (int task = gw; task < ntasks; task += nw) {
+        const int bi = uniform(find_bucket(a.buckets, nbuckets, task));
         const BucketDesc bd = a.buckets[bi];
         const int len = uniform((int)bd.len);
         const int64_t row0 = (int64_t)(task - uniform((int)bd.task_begin)) * a.task_clauses;
@@ -373,6 +378,81 @@ __global__ void k_assig_roll(const gss_var_update *__restrict__ ups, int64_t n,
     const uint64_t f = table[base], t = table[base + words];
     table[base] = ((f >> p) & 1ull) ? ~0ull : 0ull;
     table[base + words] = ((t >> p) & 1ull) ? ~0ull : 0ull;
+}
+
+// ---- batched steps: fixed launch shapes, counts from device memory --------
+
+// Copies a step's staged inputs from pinned host memory to the device.  The
+// counts are read once per block from the pinned header.
+__global__ void k_stage_in(StageArgs a) {
+    __shared__ int64_t n[4];  // desc, segs, lits, ups (in uint32 words)
+    if (threadIdx.x == 0) {
+        const StepHeader *h = a.h_hdr;
+        n[0] = (int64_t)h->nbuckets * (sizeof(BucketDesc) / 4);
+        n[1] = (int64_t)h->nsegs * (sizeof(ScatterSeg) / 4);
+        n[2] = h->nlits;
+        n[3] = h->nups * (int64_t)(sizeof(gss_var_update) / 4);
+    }
+    __syncthreads();
+    const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const uint32_t *src[5] = {(const uint32_t *)a.h_hdr, (const uint32_t *)a.h_desc,
+                              (const uint32_t *)a.h_segs, a.h_lits, (const uint32_t *)a.h_ups};
+    uint32_t *dst[5] = {(uint32_t *)a.d_hdr, (uint32_t *)a.d_desc, (uint32_t *)a.d_segs, a.d_lits,
+                        (uint32_t *)a.d_ups};
+    const int64_t cnt[5] = {(int64_t)(sizeof(StepHeader) / 4), n[0], n[1], n[2], n[3]};
+#pragma unroll
+    for (int r = 0; r < 5; r++)
+        for (int64_t i = tid; i < cnt[r]; i += stride) dst[r][i] = src[r][i];
+}
+
+__global__ void k_scatter_dyn(const ScatterSeg *__restrict__ segs, const int32_t *nseg_p,
+                              const uint32_t *__restrict__ src, int shift) {
+    const int nseg = *nseg_p;
+    for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
+        const ScatterSeg sg = segs[s];
+        for (uint64_t i = threadIdx.x; i < sg.count; i += blockDim.x)
+            sg.dst[i] = src[sg.src + i] << shift;
+    }
+}
+
+__global__ void k_assig_apply_dyn(const gss_var_update *__restrict__ ups, const int64_t *n_p,
+                                  uint64_t *__restrict__ table, int words) {
+    const int64_t n = *n_p;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const gss_var_update u = ups[i];
+        const size_t base = (size_t)u.var * 2 * words + u.word;
+        table[base] = ~(u.is_def & ~u.is_true);
+        table[base + words] = ~(u.is_def & u.is_true);
+    }
+}
+
+__global__ void k_assig_roll_dyn(const gss_var_update *__restrict__ ups, const int64_t *n_p,
+                                 uint64_t *__restrict__ table, int words,
+                                 const int8_t *__restrict__ last_pos) {
+    const int64_t n = *n_p;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const gss_var_update u = ups[i];
+        const int p = last_pos[u.word];
+        if (p < 0) continue;
+        const size_t base = (size_t)u.var * 2 * words + u.word;
+        const uint64_t f = table[base], t = table[base + words];
+        table[base] = ((f >> p) & 1ull) ? ~0ull : 0ull;
+        table[base + words] = ((t >> p) & 1ull) ? ~0ull : 0ull;
+    }
+}
+
+// Writes the claim count and the fill of every claimed chunk to pinned host
+// memory: h_out[0] = claimed chunks, h_out[1 + c] = fill of chunk c.
+__global__ void k_stage_out(const uint32_t *alloc, const uint32_t *__restrict__ fills,
+                            uint32_t cap, uint32_t *h_out) {
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) h_out[0] = used;
+    for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) h_out[1 + i] = fills[i];
 }
 
 template <int W, int S, int U>
@@ -471,6 +551,36 @@ hipError_t launch_assig_apply(const gss_var_update *ups, int64_t n, uint64_t *ta
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_assig_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ups, n,
                        table, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_in(const StageArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_in, dim3(512), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_dyn(const ScatterSeg *segs, const int32_t *nseg, const uint32_t *src,
+                              int shift, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_dyn, dim3(1024), dim3(256), 0, s, segs, nseg, src, shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_assig_apply_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
+                                  int words, hipStream_t s) {
+    hipLaunchKernelGGL(k_assig_apply_dyn, dim3(1024), dim3(256), 0, s, ups, n, table, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
+                                 int words, const int8_t *last_pos, hipStream_t s) {
+    hipLaunchKernelGGL(k_assig_roll_dyn, dim3(1024), dim3(256), 0, s, ups, n, table, words,
+                       last_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                            uint32_t *h_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage_out, dim3(256), dim3(256), 0, s, alloc, fills, cap, h_out);
     return hipGetLastError();
 }
 

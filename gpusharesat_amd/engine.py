@@ -212,6 +212,41 @@ class Engine:
         keys = ("unroll", "fuse_len", "task_clauses", "waves_per_cu", "schedule")
         return dict(zip(keys, list(best))), ms.value
 
+    # -- batched steps --------------------------------------------------------
+    def step_enqueue(self, lits=None, lens=None, ups=None, valid=0, last_pos=None) -> np.ndarray:
+        """Enqueues one step (append, apply, sweep, roll forward) as a graph
+        replay; returns the bucket rows of the appended clauses."""
+        lits = np.ascontiguousarray(np.zeros(0) if lits is None else lits, dtype=np.uint32)
+        lens = np.ascontiguousarray(np.zeros(0) if lens is None else lens, dtype=np.int32)
+        if int(lens.sum(dtype=np.int64)) != lits.size:
+            raise ValueError("sum(lens) != len(lits)")
+        ups = np.ascontiguousarray(np.zeros(0, VAR_UPDATE_DTYPE) if ups is None else ups,
+                                   dtype=VAR_UPDATE_DTYPE)
+        v = self._valid(valid)
+        lp = None
+        if last_pos is not None:
+            lp = np.full(self.words, -1, dtype=np.int8)
+            lp[:len(last_pos)] = last_pos
+        rows = np.empty(lens.size, dtype=np.uint32)
+        _lib.call("gss_step_enqueue", self._h, lens.size, _ptr(lits) if lits.size else None,
+                  _ptr(lens) if lens.size else None, _ptr(rows) if rows.size else None,
+                  ups.size, _ptr(ups) if ups.size else None, _ptr(v),
+                  None if lp is None else _ptr(lp))
+        return rows
+
+    def step_wait(self) -> tuple[int, float]:
+        n, ms = ctypes.c_int64(), ctypes.c_float()
+        _lib.call("gss_step_wait", self._h, ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
+    def step_hits(self) -> np.ndarray:
+        n = ctypes.c_int64()
+        _lib.call("gss_step_hits_copy", self._h, None, 0, ctypes.byref(n))
+        out = np.empty(n.value, dtype=HIT_DTYPE)
+        if n.value:
+            _lib.call("gss_step_hits_copy", self._h, _ptr(out), n.value, ctypes.byref(n))
+        return out
+
 
 class ClauseMap:
     """Maps (bucket length, row) of hit records back to caller clause ids."""

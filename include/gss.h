@@ -173,6 +173,28 @@ int gss_set_tuning(gss_engine *e, int key, int value);
  * launch guidelines (GpuClauseSharerImpl.cu:50-57). */
 int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, float *best_ms);
 
+/* ---- batched steps ---------------------------------------------------------
+ * One step = append n clauses (lits/lens as gss_db_append; their rows to
+ * out_rows if non-null) -> apply n_ups updates -> sweep every clause against
+ * the slots in valid -> roll forward (last_pos as gss_assig_roll_forward, may
+ * be null = none) -> stage the hit counts to pinned host memory.  The device
+ * side of a step is one captured HIP graph per ping-pong slot, replayed with
+ * new data (the counts are read from device memory), so two steps can be in
+ * flight and the host fills one slot while the other runs.  Replaces the
+ * per-run sequence of GpuRunner::wholeRun / startGpuRunAsync / launchGpu /
+ * gatherGpuRunResults (GpuRunner.cu:211-345,360) with one graph launch.
+ * Clause literals must have table rows (grow the table before the step). */
+int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                     uint32_t *out_rows, int64_t n_ups, const gss_var_update *ups,
+                     const uint64_t *valid, const int8_t *last_pos);
+/* Wait for the oldest step in flight: its number of hit records and the time
+ * (ms) between events recorded around its graph launch. */
+int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
+/* Copy the hit records of the last waited step to host (up to cap; *n =
+ * total).  Runs on a copy stream of its own, so it overlaps a step that is
+ * still in flight.  Valid until that step's slot is enqueued again. */
+int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
+
 #ifdef __cplusplus
 }
 #endif

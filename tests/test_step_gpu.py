@@ -1,0 +1,199 @@
+"""Batched steps (gss_step_*) through the C ABI.
+
+A step appends clauses, applies assignment updates, sweeps every clause and
+rolls the table forward, as one captured HIP graph per ping-pong slot that is
+replayed with new data.  Each step must give exactly the records of the plain
+sequence append -> apply -> sweep -> roll forward on a second engine, and the
+oracle's records on the table that sequence had at sweep time; the table after
+the steps must be the same on both engines.  Graphs are replayed many times,
+with eager work on the same stream between the replays, with two steps in
+flight, across bucket reallocation and across a table regrowth (which
+re-captures the graphs).
+"""
+import numpy as np
+import pytest
+
+import _oracle as orc
+from gpusharesat_amd import workloads
+from gpusharesat_amd._lib import GssError
+from gpusharesat_amd.engine import ClauseMap, Engine, VAR_UPDATE_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+ALL = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def make_step(rng, V, W, n_new, maxlen, n_ups, p_def=0.6):
+    lens = rng.integers(1, maxlen + 1, size=n_new).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    sel = rng.choice(V * W, size=min(n_ups, V * W), replace=False)
+    ups = np.zeros(sel.size, VAR_UPDATE_DTYPE)
+    ups["var"] = sel // W
+    ups["word"] = sel % W
+    ups["is_def"] = workloads.bernoulli_words(rng, sel.size, 64, p_def)[:, 0]
+    ups["is_true"] = rng.integers(0, 2**63, size=sel.size, dtype=np.uint64) << np.uint64(1)
+    valid = rng.integers(0, 2**63, size=W, dtype=np.uint64) | np.uint64(1)
+    valid[rng.random(W) < 0.3] = ALL
+    if W > 1:
+        valid[rng.integers(0, W)] = 0
+    last_pos = rng.integers(-1, 64, size=W).astype(np.int8)
+    return lits, lens, ups, valid, last_pos
+
+
+def to_records(hits, lens_all, rows_all):
+    ids = ClauseMap(lens_all, rows_all).ids(hits)
+    assert (ids >= 0).all()
+    out = np.empty(hits.size, dtype=orc.HIT_DTYPE)
+    out["clause"] = ids
+    out["word"] = hits["word"]
+    out["false_mask"] = hits["false_mask"]
+    out["unit_mask"] = hits["unit_mask"]
+    return out[np.lexsort((out["word"], out["clause"]))]
+
+
+def assert_same(got, exp, what):
+    assert got.size == exp.size, (what, got.size, exp.size)
+    for f in ("clause", "word", "false_mask", "unit_mask"):
+        assert np.array_equal(got[f], exp[f]), (what, f)
+
+
+class Plain:
+    """The same steps through the plain entry points, checked by the oracle."""
+
+    def __init__(self, e: Engine):
+        self.e = e
+        self.lits = np.zeros(0, np.uint32)
+        self.lens = np.zeros(0, np.int32)
+        self.rows = np.zeros(0, np.uint32)
+
+    def step(self, lits, lens, ups, valid, last_pos):
+        e = self.e
+        if lens.size:
+            r = e.append(lits, lens)
+            self.lits = np.concatenate([self.lits, lits])
+            self.lens = np.concatenate([self.lens, lens])
+            self.rows = np.concatenate([self.rows, r])
+        e.apply(ups)
+        d, t = e.download()
+        off = np.zeros(self.lens.size + 1, np.int64)
+        off[1:] = np.cumsum(self.lens)
+        exp = orc.sweep(off, self.lits, d, t, valid)
+        n = e.sweep(valid)
+        got = to_records(e.hits(), self.lens, self.rows)
+        assert n == got.size
+        assert_same(got, exp, "plain vs oracle")
+        e.roll_forward(ups, last_pos)
+        return exp
+
+
+def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
+              grow_at=None, eager=True):
+    rng = np.random.default_rng(seed)
+    a, b = engine_factory(), engine_factory()
+    a.reset_assignments(V, W)
+    b.reset_assignments(V, W)
+    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    a.upload(d0, t0)
+    b.upload(d0, t0)
+    plain = Plain(b)
+    lens_a = np.zeros(0, np.int32)
+    rows_a = np.zeros(0, np.uint32)
+    expected = []
+    waited = 0
+
+    def check_one():
+        nonlocal waited
+        n, ms = a.step_wait()
+        assert ms >= 0.0
+        hits = a.step_hits()
+        assert hits.size == n
+        got = to_records(hits, lens_a, rows_a)
+        assert_same(got, expected[waited], f"step {waited} vs oracle")
+        waited += 1
+
+    for k in range(nsteps):
+        if grow_at is not None and k == grow_at:
+            while waited < k:
+                check_one()
+            V += 700
+            a.grow_vars(V)
+            b.grow_vars(V)
+        nn = int(rng.integers(0, n_new + 1)) if k % 3 else n_new
+        lits, lens, ups, valid, last_pos = make_step(rng, V, W, nn, maxlen, n_ups)
+        print(f"step {k}: {nn} clauses, {ups.size} updates, {V} vars, {W} words", flush=True)
+        expected.append(plain.step(lits, lens, ups, valid, last_pos))
+        rows = a.step_enqueue(lits, lens, ups, valid, last_pos)
+        lens_a = np.concatenate([lens_a, lens])
+        rows_a = np.concatenate([rows_a, rows])
+        if not pipelined:
+            check_one()
+            if eager:  # eager work on the engine stream between replays
+                da, ta = a.download()
+                db, tb = b.download()
+                assert np.array_equal(da, db) and np.array_equal(ta, tb), f"table after {k}"
+        elif waited < k:
+            check_one()
+    while waited < nsteps:
+        check_one()
+    da, ta = a.download()
+    db, tb = b.download()
+    assert np.array_equal(da, db) and np.array_equal(ta, tb)
+    assert a.totals() == b.totals()
+    for length in range(1, maxlen + 1):
+        assert np.array_equal(a.read_bucket(length), b.read_bucket(length)), length
+
+
+@pytest.mark.parametrize("W", [1, 4, 16, 64])
+def test_steps_match_plain_and_oracle(engine_factory, W):
+    run_steps(engine_factory, seed=300 + W, V=2000, W=W, nsteps=8, n_new=600, maxlen=30,
+              n_ups=900, pipelined=False)
+
+
+@pytest.mark.parametrize("W", [2, 16])
+def test_two_steps_in_flight(engine_factory, W):
+    run_steps(engine_factory, seed=400 + W, V=1500, W=W, nsteps=9, n_new=500, maxlen=20,
+              n_ups=700, pipelined=True)
+
+
+def test_table_regrowth_recaptures(engine_factory):
+    run_steps(engine_factory, seed=500, V=1200, W=8, nsteps=7, n_new=400, maxlen=12,
+              n_ups=500, pipelined=True, grow_at=4)
+
+
+def test_bucket_reallocation_mid_sequence(engine_factory):
+    # steps of many short clauses make every bucket outgrow its capacity
+    run_steps(engine_factory, seed=600, V=800, W=4, nsteps=5, n_new=20000, maxlen=4,
+              n_ups=300, pipelined=True)
+
+
+def test_sweep_only_step_and_errors(engine_factory):
+    rng = np.random.default_rng(700)
+    V, W = 400, 2
+    a, b = engine_factory(), engine_factory()
+    lits, lens, _, valid, _ = make_step(rng, V, W, 300, 9, 0)
+    for e in (a, b):
+        e.reset_assignments(V, W)
+        d = workloads.bernoulli_words(np.random.default_rng(1), V, W * 64, 0.7)
+        e.upload(d, d & workloads.bernoulli_words(np.random.default_rng(2), V, W * 64, 0.5))
+    rows_a = a.append(lits, lens)
+    rows_b = b.append(lits, lens)
+    for _ in range(3):  # replays with nothing new: a sweep, table unchanged
+        a.step_enqueue(valid=valid)
+        n, _ = a.step_wait()
+        got = to_records(a.step_hits(), lens, rows_a)
+        b.sweep(valid)
+        exp = to_records(b.hits(), lens, rows_b)
+        assert n == got.size
+        assert_same(got, exp, "sweep-only step")
+    with pytest.raises(GssError):
+        a.step_wait()  # nothing in flight
+    with pytest.raises(GssError):  # literal without a table row
+        a.step_enqueue(np.array([2 * V], np.uint32), np.array([1], np.int32), valid=valid)
+    a.step_enqueue(valid=valid)
+    a.step_enqueue(valid=valid)
+    with pytest.raises(GssError):  # both slots busy
+        a.step_enqueue(valid=valid)
+    a.step_wait()
+    a.step_wait()
+    assert a.totals() == b.totals()

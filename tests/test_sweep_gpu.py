@@ -289,6 +289,8 @@ def test_invalid_arguments_are_errors(engine_factory):
     from gpusharesat_amd import _lib
     with pytest.raises(GssError):
         _lib.call("gss_assig_reset", e.handle, 10, 3)  # not a power of two
+    with pytest.raises(GssError):  # sentinel rows beyond 32-bit stored offsets
+        _lib.call("gss_assig_reset", e.handle, 1 << 24, 1)
     e.reset_assignments(10, 1)
     ups = np.zeros(1, VAR_UPDATE_DTYPE)
     ups["var"] = 10
