@@ -17,6 +17,16 @@ before timing starts.  Prints one JSON line.
                  events recorded on the engine's stream, against 8 TB/s
   cpu_baseline   the CPU oracle (bit-sliced rule, all host threads) on a
                  bounded sample of the same instance
+
+--workload batched-16 measures the batched form instead: 16 solver streams
+over 1M vars, each step inserting Poisson(16 x 2800) clauses of Pareto(1.5)
+lengths (2..500) and 16 uneven batches of U[16, 256] assignments (64-word
+table, 4096 slots), as one graph-captured step (gss_step_enqueue /
+gss_step_wait / gss_step_hits_copy, two steps in flight).  The DB grows from
+1M to ~10M clauses over 200 steps; the warmup steps are also replayed through
+the plain entry points on a second engine and their records must be
+identical.  value = sum over timed steps of (DB literals x valid assignments)
+/ wall-clock; latency = enqueue -> records on the host, per step.
 """
 from __future__ import annotations
 
@@ -88,17 +98,28 @@ def cpu_baseline(inst, target_s: float = 12.0) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (sweep-1024: 50, batched-16: 196)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps (sweep-1024: 5, batched-16: 4, all checked)")
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--words-tile", type=int, default=0)
+    ap.add_argument("--workload", default="sweep-1024", choices=["sweep-1024", "batched-16"])
+    ap.add_argument("--no-autotune", action="store_true", help="batched-16: default settings")
     args = ap.parse_args()
+    batched = args.workload == "batched-16"
+    if args.steps is None:
+        args.steps = 196 if batched else 50
+    if args.warmup is None:
+        args.warmup = 4 if batched else 5
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sweep is built and measured on a single "
-              "MI355X (see DESIGN.md for how it would shard); run with --gpus 1",
-              file=sys.stderr)
+              "MI355X (a multi-GPU run would shard the clause DB, not the slots); "
+              "run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.workload == "batched-16":
+        return run_batched(args)
 
     from gpusharesat_amd import workloads
     from gpusharesat_amd.engine import Engine
@@ -179,6 +200,119 @@ def main() -> int:
             "algorithmic_bytes": alg_bytes,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    e.close()
+    return 0
+
+
+def _sorted_hits(h: np.ndarray) -> np.ndarray:
+    return np.sort(h, order=["len", "row", "word"])
+
+
+def run_batched(args) -> int:
+    from gpusharesat_amd import workloads
+    from gpusharesat_amd.engine import Engine
+
+    steps_total = args.warmup + args.steps
+    t0 = time.time()
+    log("generating workload batched-16 ...")
+    gen = workloads.SolverStreams(seed=3, var_count=1_000_000)
+    d0, t0p = gen.initial_table()
+    init_lits, init_lens = gen.initial_clauses(1_000_000)
+    plan = [gen.step() for _ in range(steps_total)]
+    log(f"generated {steps_total} steps in {time.time() - t0:.1f}s; "
+        f"{sum(p.lens.size for p in plan)} clauses inserted, "
+        f"{sum(p.ups.size for p in plan)} updates")
+
+    e = Engine(0)
+    e.append(init_lits, init_lens)
+    e.reset_assignments(gen.V, gen.words)
+    e.upload(d0, t0p)
+    tuned = None
+    if not args.no_autotune:
+        tuned, tms = e.autotune(plan[0].valid)
+        log(f"autotuned: {tuned} ({tms:.3f} ms per sweep of the initial DB)")
+    # the warmup steps are checked against the plain entry points
+    ref = Engine(0)
+    ref.append(init_lits, init_lens)
+    ref.reset_assignments(gen.V, gen.words)
+    ref.upload(d0, t0p)
+    del d0, t0p
+    verified = 0
+    for k in range(args.warmup):
+        st = plan[k]
+        e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
+        n, _ = e.step_wait()
+        got = _sorted_hits(e.step_hits())
+        ref.append(st.lits, st.lens)
+        ref.apply(st.ups)
+        ref.sweep(st.valid)
+        exp = _sorted_hits(ref.hits())
+        ref.roll_forward(st.ups, st.last_pos)
+        if got.size != n or not np.array_equal(got, exp):
+            log(f"step {k}: batched records differ from the plain replay")
+            return 1
+        verified += 1
+    ref.close()
+    log(f"{verified} warmup steps identical to the plain replay; timing {args.steps} steps")
+    lits_db = e.totals()[1]
+    checks = 0
+    lat = []
+    step_ms = []
+    hits_total = 0
+    pending = []
+    e.sync()
+    t_start = time.perf_counter()
+    def finish_oldest():
+        nonlocal hits_total
+        _, ms = e.step_wait()
+        hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
+        lat.append(time.perf_counter() - pending.pop(0))
+        step_ms.append(ms)
+
+    for k in range(args.warmup, steps_total):
+        st = plan[k]
+        pending.append(time.perf_counter())
+        e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
+        lits_db += int(st.lits.size)
+        checks += lits_db * st.assignments
+        if len(pending) == 2:  # two steps in flight: collect the older one
+            finish_oldest()
+    while pending:
+        finish_oldest()
+    e.sync()
+    elapsed = time.perf_counter() - t_start
+    clauses_end = e.totals()[0]
+    lat_ms = np.array(lat) * 1e3
+    value = checks / elapsed
+    log(f"{elapsed * 1e3 / args.steps:.3f} ms/step, {value:.3e} checks/s, "
+        f"p50 {np.percentile(lat_ms, 50):.2f} ms, p99 {np.percentile(lat_ms, 99):.2f} ms, "
+        f"DB {clauses_end} clauses")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "checks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched-16: 1M vars, 16 streams x U[16,256] assignments (64 words), "
+                        "Poisson(16x2800) Pareto(1.5) clauses per step, DB 1M -> ~10M",
+            "clauses_end": int(clauses_end), "literals_end": int(lits_db),
+            "hit_records": int(hits_total), "tuned": tuned,
+            "verified_steps": verified,
+        },
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
+                       "p99": float(np.percentile(lat_ms, 99)),
+                       "max": float(lat_ms.max())},
+        "graph_ms": {"mean": float(np.mean(step_ms)), "max": float(np.max(step_ms))},
     }
     print(json.dumps(out), flush=True)
     e.close()

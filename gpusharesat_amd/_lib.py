@@ -60,6 +60,7 @@ SIGNATURES: dict[str, list] = {
     "gss_step_enqueue": [c_p, c_i64, c_p, c_p, c_p, c_i64, c_p, c_p, c_p],
     "gss_step_wait": [c_p, c_p, c_p],
     "gss_step_hits_copy": [c_p, c_p, c_i64, c_p],
+    "gss_step_hits_map": [c_p, c_p, c_p],
     # include/gss_sharer.h
     "gss_sharer_default_options": [c_p],
     "gss_sharer_create": [c_p, c_p, c_p, c_p],

@@ -139,6 +139,8 @@ struct StepSlot {
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
+    Pinned<gss_hit> h_hits;      // the same records on the host (gss_step_hits_map)
+    bool host_ready = false;     // h_hits holds the last waited step's records
     Pinned<ScatterSeg> h_cseg;   // compaction segments (one per non-empty chunk)
     DevBuf<ScatterSeg> d_cseg;
     uint32_t *d_alloc = nullptr;
@@ -1263,6 +1265,7 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
     HIP_TRY(hipEventRecord(sl.t1, e->stream));
     sl.busy = true;
     sl.results = false;
+    sl.host_ready = false;
     e->inflight.push_back(S);
     e->next_slot ^= 1;
     return GSS_OK;
@@ -1286,6 +1289,7 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, sl.t0, sl.t1));
     sl.results = true;
+    sl.host_ready = false;
     e->last_step_slot = S;
     e->last_step_hits = tot;
     e->last_step_chunks = used;
@@ -1296,39 +1300,56 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
 }
 
 int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
+    const gss_hit *p = nullptr;
+    int64_t total = 0;
+    int rc = gss_step_hits_map(e, &p, &total);
+    if (rc) return rc;
+    if (n) *n = total;
+    const int64_t want = std::min<int64_t>(cap, total);
+    if (want <= 0) return GSS_OK;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    std::memcpy(out, p, (size_t)want * sizeof(gss_hit));
+    return GSS_OK;
+}
+
+int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    *out = nullptr;
     if (e->last_step_slot < 0 || !e->slots[e->last_step_slot].results)
         return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
     StepSlot &sl = e->slots[e->last_step_slot];
-    if (n) *n = e->last_step_hits;
-    const int64_t want = std::min<int64_t>(cap, e->last_step_hits);
-    if (want <= 0) return GSS_OK;
-    if (!out) return set_err(GSS_E_INVAL, "null output");
-    const uint32_t nc = e->last_step_chunks;
-    if ((rc = sl.d_dense.reserve((size_t)e->last_step_hits)) || (rc = sl.h_cseg.reserve(nc)) ||
-        (rc = sl.d_cseg.reserve(nc)))
-        return rc;
-    int ns = 0;
-    uint64_t acc = 0;
-    for (uint32_t c = 0; c < nc; c++) {
-        const uint32_t f = sl.h_out.p[1 + c];
-        if (!f) continue;
-        ScatterSeg &sg = sl.h_cseg.p[ns++];
-        sg.dst = (uint32_t *)(sl.d_dense.p + acc);
-        sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
-        sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
-        acc += f;
+    const int64_t total = e->last_step_hits;
+    if (n) *n = total;
+    if (!sl.host_ready && total > 0) {
+        const uint32_t nc = e->last_step_chunks;
+        if ((rc = sl.d_dense.reserve((size_t)total)) || (rc = sl.h_cseg.reserve(nc)) ||
+            (rc = sl.d_cseg.reserve(nc)) || (rc = sl.h_hits.reserve((size_t)total)))
+            return rc;
+        int ns = 0;
+        uint64_t acc = 0;
+        for (uint32_t c = 0; c < nc; c++) {
+            const uint32_t f = sl.h_out.p[1 + c];
+            if (!f) continue;
+            ScatterSeg &sg = sl.h_cseg.p[ns++];
+            sg.dst = (uint32_t *)(sl.d_dense.p + acc);
+            sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
+            sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
+            acc += f;
+        }
+        // the step's kernels are done (waited); the next step may be running
+        // on the engine stream, so the compaction and the copy go on their own
+        hipStream_t cs = e->copy_stream;
+        HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
+                               hipMemcpyHostToDevice, cs));
+        HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
+        HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * sizeof(gss_hit),
+                               hipMemcpyDeviceToHost, cs));
+        HIP_TRY(hipStreamSynchronize(cs));
     }
-    // the step's kernels are done (waited); the next step may be running on
-    // the engine stream, so the copy goes on its own stream
-    hipStream_t cs = e->copy_stream;
-    HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
-                           hipMemcpyHostToDevice, cs));
-    HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
-    HIP_TRY(hipMemcpyAsync(out, sl.d_dense.p, (size_t)want * sizeof(gss_hit),
-                           hipMemcpyDeviceToHost, cs));
-    HIP_TRY(hipStreamSynchronize(cs));
+    sl.host_ready = true;
+    *out = sl.h_hits.p;
     return GSS_OK;
 }
 

@@ -239,6 +239,16 @@ class Engine:
         _lib.call("gss_step_wait", self._h, ctypes.byref(n), ctypes.byref(ms))
         return n.value, ms.value
 
+    def step_hits_view(self) -> np.ndarray:
+        """Records of the last waited step as a view of the engine's pinned
+        memory, valid until that step's slot is enqueued again."""
+        p, n = ctypes.c_void_p(), ctypes.c_int64()
+        _lib.call("gss_step_hits_map", self._h, ctypes.byref(p), ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros(0, dtype=HIT_DTYPE)
+        buf = (ctypes.c_char * (n.value * HIT_DTYPE.itemsize)).from_address(p.value)
+        return np.frombuffer(buf, dtype=HIT_DTYPE)
+
     def step_hits(self) -> np.ndarray:
         n = ctypes.c_int64()
         _lib.call("gss_step_hits_copy", self._h, None, 0, ctypes.byref(n))

@@ -182,3 +182,148 @@ def sweep_1024(seed: int = 2, scale: float = 1.0) -> Instance:
     """1M vars, 4M clauses, 1024 assignments (40% assigned), 0.1% planted."""
     return make_instance(seed, max(1000, int(1_000_000 * scale)), int(4_000_000 * scale), 0.1,
                          200, 1024, 0.4, 0.0005, 0.0005)
+
+
+def pareto_lengths(rng: np.random.Generator, n: int, alpha: float = 1.5, xmin: int = 2,
+                   cap: int = 500) -> np.ndarray:
+    """Heavy-tailed clause lengths: floor of Pareto(alpha) from xmin, capped."""
+    u = 1.0 - rng.random(n)
+    return np.minimum(cap, np.floor(xmin * u ** (-1.0 / alpha))).astype(np.int32)
+
+
+def pareto_clauses(rng: np.random.Generator, n: int, var_count: int,
+                   cap: int = 500) -> tuple[np.ndarray, np.ndarray]:
+    lens = pareto_lengths(rng, n, cap=cap)
+    vars_ = distinct_vars(rng, lens, var_count)
+    signs = rng.integers(0, 2, size=vars_.size, dtype=np.uint32)
+    return (vars_.astype(np.uint32) << np.uint32(1)) | signs, lens
+
+
+@dataclass
+class Step:
+    """One batched step: new clauses, table updates, slot masks, roll-forward."""
+    lits: np.ndarray
+    lens: np.ndarray
+    ups: np.ndarray        # VAR_UPDATE records (var, word, is_def, is_true)
+    valid: np.ndarray      # uint64 [words]
+    last_pos: np.ndarray   # int8 [words], -1 = word unused this step
+
+    @property
+    def assignments(self) -> int:
+        return int(sum(int(v).bit_count() for v in self.valid))
+
+
+class SolverStreams:
+    """batched-16: solver streams sending uneven batches into one table.
+
+    A planted total assignment sigma (uniform) stands for the solution the
+    solvers converge to: every assigned variable takes sigma's value with
+    probability `agree`, and every clause literal is true under sigma with
+    probability `lit_true` (learned clauses are implied by the formula, so
+    most of them are satisfied near a solution).  Random clauses against
+    random assignments would make nearly every short clause unit in nearly
+    every 64-slot word, which no solver population produces.
+
+    Stream s owns words [s * wps, (s + 1) * wps) of the table (64 slots per
+    word).  After every step each word holds one assignment in all its slots
+    (the roll-forward copies the word's last sent slot), its base.  In a step
+    stream s sends n_s ~ U[assig_lo, assig_hi] assignments into its first n_s
+    slots; each one is the word's base with `flips` uniform variables redrawn
+    (assigned with probability p_assigned), so only the touched (var, word)
+    pairs are sent as updates.  It also inserts Poisson(clause_mean) clauses
+    of Pareto(1.5) lengths from 2 capped at 500, distinct uniform variables.
+    """
+
+    def __init__(self, seed: int = 3, var_count: int = 1_000_000, streams: int = 16,
+                 words_per_stream: int = 4, p_assigned: float = 0.95, agree: float = 0.99,
+                 lit_true: float = 0.9, flips: int = 64, clause_mean: float = 2800.0,
+                 assig_lo: int = 16, assig_hi: int = 256):
+        if assig_hi > 64 * words_per_stream:
+            raise ValueError("a stream sends more assignments than it has slots")
+        self.rng = np.random.default_rng(seed)
+        self.V = var_count
+        self.streams = streams
+        self.wps = words_per_stream
+        self.words = streams * words_per_stream
+        self.p_assigned = p_assigned
+        self.agree = agree
+        self.lit_true = lit_true
+        self.flips = flips
+        self.clause_mean = clause_mean
+        self.assig_lo, self.assig_hi = assig_lo, assig_hi
+        self.sigma = self.rng.random(var_count) < 0.5
+        shape = (var_count, self.words)
+        self.base_def = self.rng.random(shape, dtype=np.float32) < p_assigned
+        agrees = self.rng.random(shape, dtype=np.float32) < agree
+        self.base_true = self.base_def & (agrees == self.sigma[:, None])
+
+    def initial_table(self) -> tuple[np.ndarray, np.ndarray]:
+        ones = np.uint64(0xFFFFFFFFFFFFFFFF)
+        d = np.where(self.base_def, ones, np.uint64(0))
+        t = np.where(self.base_true, ones, np.uint64(0))
+        return d, t
+
+    def clauses(self, n: int) -> tuple[np.ndarray, np.ndarray]:
+        """n clauses whose literals are each true under sigma w.p. lit_true."""
+        lens = pareto_lengths(self.rng, n)
+        vars_ = distinct_vars(self.rng, lens, self.V)
+        st = self.rng.random(vars_.size) < self.lit_true
+        # literal 2v + 1 is the negation: true under sigma iff sigma(v) is false
+        sign = (st != self.sigma[vars_]).astype(np.uint32)
+        return (vars_.astype(np.uint32) << np.uint32(1)) | sign, lens
+
+    def initial_clauses(self, n: int) -> tuple[np.ndarray, np.ndarray]:
+        return self.clauses(n)
+
+    def _word_updates(self, w: int, k: int):
+        rng, V, f = self.rng, self.V, self.flips
+        slots = np.repeat(np.arange(k, dtype=np.int64), f)
+        vs = rng.integers(0, V, size=k * f, dtype=np.int64)
+        nd = rng.random(k * f) < self.p_assigned
+        nt = nd & ((rng.random(k * f) < self.agree) == self.sigma[vs])
+        key = vs * 64 + slots
+        # a (var, slot) drawn twice keeps its last draw
+        uk, ridx = np.unique(key[::-1], return_index=True)
+        sel = key.size - 1 - ridx
+        v_u = uk // 64
+        bit = np.uint64(1) << (uk % 64).astype(np.uint64)
+        uv, start = np.unique(v_u, return_index=True)
+        changed = np.bitwise_or.reduceat(bit, start)
+        dbits = np.bitwise_or.reduceat(np.where(nd[sel], bit, np.uint64(0)), start)
+        tbits = np.bitwise_or.reduceat(np.where(nt[sel], bit, np.uint64(0)), start)
+        ones = np.uint64(0xFFFFFFFFFFFFFFFF)
+        bd = np.where(self.base_def[uv, w], ones, np.uint64(0))
+        bt = np.where(self.base_true[uv, w], ones, np.uint64(0))
+        is_def = (bd & ~changed) | dbits
+        is_true = ((bt & ~changed) | tbits) & is_def
+        last = np.uint64(k - 1)
+        self.base_def[uv, w] = ((is_def >> last) & np.uint64(1)).astype(bool)
+        self.base_true[uv, w] = ((is_true >> last) & np.uint64(1)).astype(bool)
+        return uv, is_def, is_true
+
+    def step(self) -> Step:
+        from .engine import VAR_UPDATE_DTYPE
+        rng = self.rng
+        valid = np.zeros(self.words, dtype=np.uint64)
+        last_pos = np.full(self.words, -1, dtype=np.int8)
+        parts = []
+        for s in range(self.streams):
+            n = int(rng.integers(self.assig_lo, self.assig_hi + 1))
+            for j in range(self.wps):
+                k = min(64, n - 64 * j)
+                if k <= 0:
+                    break
+                w = s * self.wps + j
+                valid[w] = np.uint64(0xFFFFFFFFFFFFFFFF) if k == 64 else np.uint64((1 << k) - 1)
+                last_pos[w] = k - 1
+                uv, d, t = self._word_updates(w, k)
+                u = np.empty(uv.size, VAR_UPDATE_DTYPE)
+                u["var"] = uv
+                u["word"] = w
+                u["is_def"] = d
+                u["is_true"] = t
+                parts.append(u)
+        ups = np.concatenate(parts) if parts else np.zeros(0, VAR_UPDATE_DTYPE)
+        n_cl = int(rng.poisson(self.clause_mean * self.streams))
+        lits, lens = self.clauses(n_cl)
+        return Step(lits, lens, ups, valid, last_pos)

@@ -28,8 +28,10 @@
  *   assignment table
  *              `words` uint64 words per literal; bit b of word w is assignment
  *              slot 64*w + b.  Internally the row of literal l holds the mask of
- *              slots where l is false; gss_assig_* take the reference's
- *              per-variable isDef/isTrue view.
+ *              slots where l is NOT false, and the DB stores each literal as
+ *              its row offset; gss_assig_* and gss_db_read take and give the
+ *              reference's per-variable isDef/isTrue view and plain literals.
+ *              At most 2^24 - 1 variables (32-bit row offsets).
  *   hit        one record per (clause, word) whose false or unit mask is
  *              nonzero; nothing is ever dropped.
  */
@@ -194,6 +196,11 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
  * total).  Runs on a copy stream of its own, so it overlaps a step that is
  * still in flight.  Valid until that step's slot is enqueued again. */
 int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
+/* Zero-copy form: *out points at the records of the last waited step in the
+ * engine's pinned host memory (*n records), valid until that step's slot is
+ * enqueued again (two gss_step_enqueue calls later).  Same source as
+ * gss_step_hits_copy; replaces Reporter::getCopiedToHost (Reporter.cuh:103). */
+int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,7 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
         assert ms >= 0.0
         hits = a.step_hits()
         assert hits.size == n
+        assert np.array_equal(a.step_hits_view(), hits)  # pinned view, same records
         got = to_records(hits, lens_a, rows_a)
         assert_same(got, expected[waited], f"step {waited} vs oracle")
         waited += 1
