@@ -104,7 +104,10 @@ def main() -> int:
                     help="untimed steps (sweep-1024: 5, batched-16: 4, all checked)")
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", default="sweep-1024", choices=["sweep-1024", "batched-16"])
+    ap.add_argument("--workload", default="sweep-1024",
+                    choices=["sweep-1024", "batched-16", "serving-32", "residency"])
+    ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
+    ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
     ap.add_argument("--no-autotune", action="store_true", help="batched-16: default settings")
     args = ap.parse_args()
     batched = args.workload == "batched-16"
@@ -120,6 +123,10 @@ def main() -> int:
 
     if args.workload == "batched-16":
         return run_batched(args)
+    if args.workload == "serving-32":
+        return run_serving(args)
+    if args.workload == "residency":
+        return run_residency(args)
 
     from gpusharesat_amd import workloads
     from gpusharesat_amd.engine import Engine
@@ -316,6 +323,70 @@ def run_batched(args) -> int:
     }
     print(json.dumps(out), flush=True)
     e.close()
+    return 0
+
+
+def run_serving(args) -> int:
+    from gpusharesat_amd import serving as sv
+    cfg = sv.ServeConfig(duration=args.duration)
+    log(f"serving-32: {cfg}")
+    srv = sv.Server(cfg, log=log)
+    try:
+        out = srv.run()
+    finally:
+        srv.close()
+    lat = out.latency_ms
+    pct = {q: float(np.percentile(lat, q)) for q in (50, 99, 99.9)}
+    log(f"{out.requests} requests in {out.steps} steps over {out.elapsed:.1f}s, "
+        f"{out.checks_per_s:.3e} checks/s, p50 {pct[50]:.2f} ms p99 {pct[99]:.2f} ms "
+        f"p999 {pct[99.9]:.2f} ms")
+    print(json.dumps({
+        "metric": METRIC, "value": out.checks_per_s, "unit": "checks/s", "n_gpus": 1,
+        "steps": out.steps, "warmup": 0, "ms_per_step": out.elapsed * 1e3 / max(1, out.steps),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "serving-32: 32 solvers over 2M vars, Poisson 2000 requests/s of "
+                               "U[64,1024] assignments (30-70% assigned), 200k clauses/s of "
+                               "len 2+Geom(0.1) cap 200, 120 GB budget",
+                   "duration_s": cfg.duration, "requests": out.requests,
+                   "clauses_end": out.clauses_end, "literals_end": out.literals_end,
+                   "records": out.records, "max_queue": out.max_queue},
+        "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
+                       "max": float(lat.max())},
+        "graph_ms": {"mean": float(np.mean(out.step_ms)), "max": float(np.max(out.step_ms))},
+    }), flush=True)
+    return 0
+
+
+def run_residency(args) -> int:
+    from gpusharesat_amd import residency as rs
+    s = args.res_scale
+    cfg = rs.SimConfig(var_count=int(2_000_000 * s), arrivals=int(60_000_000 * s),
+                       budget=int(20_000_000 * s))
+    log(f"residency: {cfg}")
+    res = {}
+    for pol, name in ((rs.POLICY_ACTIVITY, "activity"), (rs.POLICY_LEARNED, "learned")):
+        t0 = time.time()
+        out = rs.simulate(cfg, pol, log=log)
+        res[name] = out
+        log(f"{name}: {out.hits_per_gb:.4e} hit pairs/GB, {out.reductions} reductions, "
+            f"{time.time() - t0:.1f}s")
+    a, l = res["activity"], res["learned"]
+    print(json.dumps({
+        "metric": "hit pairs per GB of clause literals checked (residency policy)",
+        "value": l.hits_per_gb, "unit": "hit pairs/GB", "n_gpus": 1, "steps": cfg.rounds,
+        "warmup": 0, "higher_is_better": True, "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "residency: 60M arrivals over 2M vars into a 20M-clause budget, "
+                               "16 solvers x 64 slots, hot sets of 5k vars redrawn every 50 "
+                               "rounds", "scale": s},
+        "activity": {"hits_per_gb": a.hits_per_gb, "hit_pairs": a.hit_pairs,
+                     "bytes_checked": a.bytes_checked, "reductions": a.reductions},
+        "learned": {"hits_per_gb": l.hits_per_gb, "hit_pairs": l.hit_pairs,
+                    "bytes_checked": l.bytes_checked, "reductions": l.reductions,
+                    "train_loss": l.train_loss},
+        "gain": l.hits_per_gb / a.hits_per_gb if a.hits_per_gb else None,
+    }), flush=True)
     return 0
 
 

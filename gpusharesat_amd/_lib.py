@@ -41,6 +41,10 @@ SIGNATURES: dict[str, list] = {
     "gss_db_totals": [c_p, c_p, c_p],
     "gss_db_read": [c_p, c_int, c_i64, c_i64, c_p],
     "gss_db_clear": [c_p],
+    "gss_db_enable_tags": [c_p],
+    "gss_db_append_tagged": [c_p, c_i64, c_p, c_p, c_p, c_p],
+    "gss_db_read_tags": [c_p, c_int, c_i64, c_i64, c_p],
+    "gss_db_compact": [c_p, c_p, c_i64],
     "gss_assig_reset": [c_p, c_i64, c_int],
     "gss_assig_grow_vars": [c_p, c_i64],
     "gss_assig_upload": [c_p, c_p, c_p],
@@ -61,6 +65,22 @@ SIGNATURES: dict[str, list] = {
     "gss_step_wait": [c_p, c_p, c_p],
     "gss_step_hits_copy": [c_p, c_p, c_i64, c_p],
     "gss_step_hits_map": [c_p, c_p, c_p],
+    "gss_step_submit": [c_p, c_p],
+    "gss_bases_reset": [c_p, c_int],
+    # include/gss_residency.h
+    "gss_res_create": [c_p, c_i64, ctypes.c_float, ctypes.c_uint64, c_p],
+    "gss_res_destroy": [c_p],
+    "gss_res_add": [c_p, c_i64, c_p, c_p, c_p, ctypes.c_int32],
+    "gss_res_account": [c_p, ctypes.c_int32, c_p],
+    "gss_res_snapshot": [c_p, ctypes.c_int32, c_i64, c_p],
+    "gss_res_train": [c_p, ctypes.c_int32, c_int, c_int, c_int, ctypes.c_float, c_p, c_p],
+    "gss_res_reduce": [c_p, c_int, ctypes.c_int32, c_p, c_p],
+    "gss_res_features": [c_p, ctypes.c_int32, c_i64, c_p, c_p],
+    "gss_res_activity": [c_p, c_i64, c_p, c_p],
+    "gss_res_get_params": [c_p, c_p],
+    "gss_res_set_params": [c_p, c_p],
+    "gss_res_grad": [c_p, c_i64, c_p, c_p, c_p, c_p, c_p],
+    "gss_res_score": [c_p, c_i64, c_p, c_p],
     # include/gss_sharer.h
     "gss_sharer_default_options": [c_p],
     "gss_sharer_create": [c_p, c_p, c_p, c_p],

@@ -25,6 +25,7 @@ ARCH = os.environ.get("GSS_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "kernels.hip",
     "engine.hip",
+    "residency.hip",
     "host_state.cpp",
     "sharer.cpp",
     "sharer_capi.cpp",

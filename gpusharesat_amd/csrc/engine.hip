@@ -28,6 +28,7 @@
 
 #include "errors.h"
 #include "gss.h"
+#include "engine_internal.h"
 #include "kernels.h"
 
 using namespace gss;
@@ -65,6 +66,7 @@ int set_err(int code, const char *fmt, ...) {
 
 struct Bucket {
     uint32_t *d = nullptr;
+    uint32_t *tags = nullptr;  // per-row caller tag (tagged DBs), same capacity
     int64_t count = 0;
     int64_t cap = 0;  // rows
 };
@@ -130,12 +132,18 @@ struct StepSlot {
     Pinned<ScatterSeg> h_segs;
     Pinned<uint32_t> h_lits;
     Pinned<gss_var_update> h_ups;
+    Pinned<gss_var_patch> h_patches;
+    Pinned<gss_base_update> h_bups;
+    Pinned<gss_word_load> h_loads;
     Pinned<uint32_t> h_out;  // [0] = chunks claimed, [1 + c] = fill of chunk c
     DevBuf<StepHeader> d_hdr;
     DevBuf<BucketDesc> d_desc;
     DevBuf<ScatterSeg> d_segs;
     DevBuf<uint32_t> d_lits;
     DevBuf<gss_var_update> d_ups;
+    DevBuf<gss_var_patch> d_patches;
+    DevBuf<gss_base_update> d_bups;
+    DevBuf<gss_word_load> d_loads;
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
@@ -169,6 +177,10 @@ struct gss_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     std::vector<Bucket> buckets;  // index = clause length
+    bool tagged = false;          // every row carries a uint32 tag
+    // bumped whenever existing rows move or vanish (compaction, bucket
+    // replacement, clear); appends leave existing rows in place
+    uint64_t rows_version = 0, sweep_rows_version = 0;
     int64_t total_clauses = 0, total_lits = 0;
     // Largest literal ever stored (running max, never lowered); a sweep is
     // refused unless the table has a row for it, so the kernel's table reads
@@ -179,6 +191,10 @@ struct gss_engine {
     DevBuf<uint64_t> table;
     int64_t vars = 0;
     int words = 0;
+    // solver base planes (gss_bases_reset): [nbases][2][bases_vw] bits
+    DevBuf<uint64_t> bases;
+    int nbases = 0;
+    int64_t bases_vw = 0;
 
     // sweep state
     DevBuf<BucketDesc> d_desc;
@@ -222,6 +238,12 @@ struct gss_engine {
     DevBuf<gss_hit> d_dense;
     Pinned<gss_hit> h_hits;
 
+    // tagged DB compaction
+    DevBuf<CompactRef> d_cref;
+    DevBuf<uint32_t> d_flags, d_pos, d_kept;
+    DevBuf<uint8_t> d_scan_temp, d_keep;
+    Pinned<uint32_t> h_kept;
+
     // batched steps (gss_step_*)
     StepSlot slots[2];
     int next_slot = 0;
@@ -251,19 +273,29 @@ int ensure_bucket_cap(gss_engine *e, int len, int64_t need) {
     Bucket &b = e->buckets[len];
     if (need <= b.cap) return GSS_OK;
     int64_t nc = std::max<int64_t>(std::max<int64_t>(need, b.cap * 2), 256);
-    uint32_t *np = nullptr;
+    uint32_t *np = nullptr, *nt = nullptr;
     hipError_t he = hipMalloc((void **)&np, (size_t)nc * len * sizeof(uint32_t));
+    if (he == hipSuccess && e->tagged) {
+        he = hipMalloc((void **)&nt, (size_t)nc * sizeof(uint32_t));
+        if (he != hipSuccess) (void)hipFree(np);
+    }
     if (he != hipSuccess)
         return set_err(GSS_E_NOMEM, "bucket %d: device allocation of %lld rows failed: %s", len,
                        (long long)nc, hipGetErrorString(he));
     if (b.d) {
-        if (b.count)
+        if (b.count) {
             HIP_TRY(hipMemcpyAsync(np, b.d, (size_t)b.count * len * sizeof(uint32_t),
                                    hipMemcpyDeviceToDevice, e->stream));
+            if (nt && b.tags)
+                HIP_TRY(hipMemcpyAsync(nt, b.tags, (size_t)b.count * sizeof(uint32_t),
+                                       hipMemcpyDeviceToDevice, e->stream));
+        }
         HIP_TRY(hipStreamSynchronize(e->stream));
         (void)hipFree(b.d);
     }
+    if (b.tags) (void)hipFree(b.tags);
     b.d = np;
+    b.tags = nt;
     b.cap = nc;
     return GSS_OK;
 }
@@ -359,14 +391,20 @@ int gss_engine_destroy(gss_engine *e) {
         sl.d_segs.release();
         sl.d_lits.release();
         sl.d_ups.release();
+        sl.d_patches.release();
+        sl.d_bups.release();
+        sl.d_loads.release();
         sl.d_hits.release();
         sl.d_fills.release();
         sl.d_dense.release();
         sl.d_cseg.release();
     }
-    for (Bucket &b : e->buckets)
+    for (Bucket &b : e->buckets) {
         if (b.d) (void)hipFree(b.d);
+        if (b.tags) (void)hipFree(b.tags);
+    }
     e->table.release();
+    e->bases.release();
     e->d_desc.release();
     e->d_hits.release();
     e->d_fills.release();
@@ -414,11 +452,12 @@ int gss_engine_mem_info(gss_engine *e, size_t *free_bytes, size_t *total_bytes) 
 
 /* ---- clause DB ------------------------------------------------------------ */
 
-int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
-                  uint32_t *out_rows) {
+static int db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                     const uint32_t *tags, uint32_t *out_rows) {
     int rc = check_engine(e);
     if (rc) return rc;
     if (n < 0 || (n > 0 && (!lits || !lens))) return set_err(GSS_E_INVAL, "bad clause batch");
+    if (e->tagged && n > 0 && !tags) return set_err(GSS_E_INVAL, "tagged DB: tags required");
     if (n == 0) return GSS_OK;
     // Group by length (stable), so that each bucket's new rows are one
     // contiguous segment of the staging buffer.
@@ -452,32 +491,55 @@ int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t 
         acc += per[len] * len;
         if (per[len]) nseg++;
     }
-    if ((rc = e->h_stage.reserve((size_t)total)) || (rc = e->d_stage.reserve((size_t)total)) ||
-        (rc = e->h_segs.reserve(nseg)) || (rc = e->d_segs.reserve(nseg)))
+    // staging: literals, then (tagged DBs) one tag per clause in the same
+    // bucket order; one segment per bucket for each
+    const int64_t ntag = e->tagged ? n : 0;
+    const int nsegs_all = e->tagged ? 2 * nseg : nseg;
+    if ((rc = e->h_stage.reserve((size_t)(total + ntag))) ||
+        (rc = e->d_stage.reserve((size_t)(total + ntag))) ||
+        (rc = e->h_segs.reserve(nsegs_all)) || (rc = e->d_segs.reserve(nsegs_all)))
         return rc;
     // any previous async user of the staging buffers must be done
     HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<int64_t> tag_off(maxlen + 1, 0);
+    if (e->tagged) {
+        int64_t ta = total;
+        for (int len = 1; len <= maxlen; len++) {
+            tag_off[len] = ta;
+            ta += per[len];
+        }
+    }
     for (int64_t i = 0; i < n; i++) {
         const int len = lens[i];
         const int64_t k = seg_fill[len]++;
         std::memcpy(e->h_stage.p + seg_off[len] + k * len, lits + start[i], len * sizeof(uint32_t));
+        if (e->tagged) e->h_stage.p[tag_off[len] + k] = tags[i];
         if (out_rows) out_rows[i] = (uint32_t)(e->buckets[len].count + k);
     }
     int s = 0;
     for (int len = 1; len <= maxlen; len++) {
         if (!per[len]) continue;
         Bucket &b = e->buckets[len];
-        ScatterSeg &sg = e->h_segs.p[s++];
+        ScatterSeg &sg = e->h_segs.p[s];
         sg.dst = b.d + (size_t)b.count * len;
         sg.src = (uint64_t)seg_off[len];
         sg.count = (uint64_t)per[len] * len;
+        if (e->tagged) {
+            ScatterSeg &tg = e->h_segs.p[nseg + s];
+            tg.dst = b.tags + b.count;
+            tg.src = (uint64_t)tag_off[len];
+            tg.count = (uint64_t)per[len];
+        }
+        s++;
         b.count += per[len];
     }
-    HIP_TRY(hipMemcpyAsync(e->d_stage.p, e->h_stage.p, (size_t)total * sizeof(uint32_t),
+    HIP_TRY(hipMemcpyAsync(e->d_stage.p, e->h_stage.p, (size_t)(total + ntag) * sizeof(uint32_t),
                            hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, (size_t)nseg * sizeof(ScatterSeg),
+    HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, (size_t)nsegs_all * sizeof(ScatterSeg),
                            hipMemcpyHostToDevice, e->stream));
     HIP_TRY(launch_scatter(e->d_segs.p, nseg, e->d_stage.p, kLitShift, e->stream));
+    if (e->tagged)
+        HIP_TRY(launch_scatter(e->d_segs.p + nseg, nseg, e->d_stage.p, 0, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->total_clauses += n;
     e->total_lits += total;
@@ -485,9 +547,160 @@ int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t 
     return GSS_OK;
 }
 
+int gss_db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                  uint32_t *out_rows) {
+    return db_append(e, n, lits, lens, nullptr, out_rows);
+}
+
+int gss_db_append_tagged(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
+                         const uint32_t *tags, uint32_t *out_rows) {
+    if (e && !e->tagged) return set_err(GSS_E_STATE, "the DB is not tagged");
+    return db_append(e, n, lits, lens, tags, out_rows);
+}
+
+int gss_db_enable_tags(gss_engine *e) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (e->tagged) return GSS_OK;
+    if (e->total_clauses) return set_err(GSS_E_STATE, "tags must be enabled on an empty DB");
+    for (Bucket &b : e->buckets)
+        if (b.cap && !b.tags && hipMalloc((void **)&b.tags, (size_t)b.cap * 4) != hipSuccess)
+            return set_err(GSS_E_NOMEM, "tag allocation failed");
+    e->tagged = true;
+    return GSS_OK;
+}
+
+int gss_db_read_tags(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->tagged) return set_err(GSS_E_STATE, "the DB is not tagged");
+    if (len < 1 || len >= (int)e->buckets.size() || row0 < 0 || n < 0 ||
+        row0 + n > e->buckets[len].count || (n && !out))
+        return set_err(GSS_E_INVAL, "bad tag read");
+    if (n)
+        HIP_TRY(hipMemcpy(out, e->buckets[len].tags + row0, (size_t)n * 4,
+                          hipMemcpyDeviceToHost));
+    return GSS_OK;
+}
+
+}  // extern "C"
+
+// Keeps the rows of a tagged DB whose tag t has d_keep[t] != 0 (tags >= ntags
+// are kept), in order, on the device: flags -> exclusive scan -> per-bucket
+// kept counts -> moves into fresh arrays.
+int gss::db_compact_device(gss_engine *e, const uint8_t *d_keep, int64_t ntags) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->tagged) return set_err(GSS_E_STATE, "the DB is not tagged");
+    if (e->pending || !e->inflight.empty())
+        return set_err(GSS_E_STATE, "a sweep or step is in flight");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<CompactRef> refs;
+    std::vector<int> lens;
+    int64_t N = 0;
+    for (size_t len = 1; len < e->buckets.size(); len++) {
+        const Bucket &b = e->buckets[len];
+        if (!b.count) continue;
+        refs.push_back({b.d, b.tags, nullptr, nullptr, N, (uint32_t)len, 0});
+        lens.push_back((int)len);
+        N += b.count;
+    }
+    if (!N) return GSS_OK;
+    if (N > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "DB too large to compact in one pass");
+    const int nb = (int)refs.size();
+    size_t temp_bytes = 0;
+    HIP_TRY(scan_exclusive_u32(nullptr, nullptr, N, nullptr, temp_bytes, e->stream));
+    if ((rc = e->d_cref.reserve(nb)) || (rc = e->d_flags.reserve(N)) ||
+        (rc = e->d_pos.reserve(N)) || (rc = e->d_kept.reserve(nb)) ||
+        (rc = e->h_kept.reserve(nb)) || (rc = e->d_scan_temp.reserve(std::max<size_t>(temp_bytes, 1))))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(e->d_cref.p, refs.data(), nb * sizeof(CompactRef),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(launch_compact_flags(e->d_cref.p, nb, N, d_keep, ntags, e->d_flags.p, e->stream));
+    HIP_TRY(scan_exclusive_u32(e->d_flags.p, e->d_pos.p, N, e->d_scan_temp.p, temp_bytes,
+                               e->stream));
+    HIP_TRY(launch_compact_bounds(e->d_cref.p, nb, N, e->d_flags.p, e->d_pos.p, e->d_kept.p,
+                                  e->stream));
+    HIP_TRY(hipMemcpyAsync(e->h_kept.p, e->d_kept.p, nb * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<int64_t> caps(nb);
+    for (int i = 0; i < nb; i++) {
+        const int64_t k = e->h_kept.p[i];
+        caps[i] = std::max<int64_t>(k + k / 4, 256);
+        uint32_t *dl = nullptr, *dt = nullptr;
+        hipError_t he = hipMalloc((void **)&dl, (size_t)caps[i] * lens[i] * 4);
+        if (he == hipSuccess) he = hipMalloc((void **)&dt, (size_t)caps[i] * 4);
+        if (he != hipSuccess) {
+            if (dl) (void)hipFree(dl);
+            for (int j = 0; j < i; j++) {
+                (void)hipFree(refs[j].dst_lits);
+                (void)hipFree(refs[j].dst_tags);
+            }
+            return set_err(GSS_E_NOMEM, "compaction: allocation failed");
+        }
+        refs[i].dst_lits = dl;
+        refs[i].dst_tags = dt;
+    }
+    HIP_TRY(hipMemcpyAsync(e->d_cref.p, refs.data(), nb * sizeof(CompactRef),
+                           hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(launch_compact_move(e->d_cref.p, nb, N, e->d_flags.p, e->d_pos.p, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->rows_version++;
+    for (int i = 0; i < nb; i++) {
+        Bucket &b = e->buckets[lens[i]];
+        (void)hipFree(b.d);
+        (void)hipFree(b.tags);
+        const int64_t k = e->h_kept.p[i];
+        e->total_clauses -= b.count - k;
+        e->total_lits -= (b.count - k) * lens[i];
+        b.d = refs[i].dst_lits;
+        b.tags = refs[i].dst_tags;
+        b.cap = caps[i];
+        b.count = k;
+    }
+    return GSS_OK;
+}
+
+int gss::db_tag_arrays(gss_engine *e, std::vector<const uint32_t *> &tags) {
+    if (!e->tagged) return set_err(GSS_E_STATE, "the DB is not tagged");
+    tags.assign(e->buckets.size(), nullptr);
+    for (size_t len = 1; len < e->buckets.size(); len++) tags[len] = e->buckets[len].tags;
+    return GSS_OK;
+}
+
+int gss::sweep_records(gss_engine *e, const gss_hit **hits, const uint32_t **fills,
+                       uint32_t *chunks) {
+    if (!e->waited) return set_err(GSS_E_STATE, "no waited sweep");
+    if (e->sweep_rows_version != e->rows_version)
+        return set_err(GSS_E_STATE, "rows moved since the last sweep");
+    *hits = e->d_hits.p;
+    *fills = e->d_fills.p;
+    *chunks = e->last_chunks;
+    return GSS_OK;
+}
+
+hipStream_t gss::engine_stream(gss_engine *e) { return e->stream; }
+
+int gss::engine_activate(gss_engine *e) { return check_engine(e); }
+
+extern "C" {
+
+int gss_db_compact(gss_engine *e, const uint8_t *keep, int64_t ntags) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (ntags < 0 || (ntags && !keep)) return set_err(GSS_E_INVAL, "bad keep flags");
+    if ((rc = e->d_keep.reserve(std::max<int64_t>(ntags, 1)))) return rc;
+    if (ntags)
+        HIP_TRY(hipMemcpyAsync(e->d_keep.p, keep, (size_t)ntags, hipMemcpyHostToDevice,
+                               e->stream));
+    return gss::db_compact_device(e, e->d_keep.p, ntags);
+}
+
 int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if (e->tagged) return set_err(GSS_E_STATE, "tagged DB: compact with gss_db_compact");
     if (len < 1 || len > GSS_MAX_CLAUSE_LEN || n < 0 || (n > 0 && !lits))
         return set_err(GSS_E_INVAL, "bad bucket replacement");
     uint32_t mx = 0;
@@ -496,6 +709,7 @@ int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
     if ((int)e->buckets.size() <= len) e->buckets.resize(len + 1);
     Bucket &b = e->buckets[len];
     const int64_t old = b.count;
+    e->rows_version++;
     b.count = 0;  // nothing to preserve when growing
     if ((rc = ensure_bucket_cap(e, len, n))) {
         b.count = old;
@@ -547,6 +761,7 @@ int gss_db_clear(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
     for (Bucket &b : e->buckets) b.count = 0;
+    e->rows_version++;
     e->total_clauses = e->total_lits = 0;
     e->max_lit = -1;
     return GSS_OK;
@@ -813,6 +1028,7 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     if (e->tune_max_chunks > 0) chunks = std::min<uint64_t>(chunks, e->tune_max_chunks);
     if (chunks > e->chunk_cap && (rc = reserve_chunks(e, chunks))) return rc;
     if ((rc = launch_current_sweep(e))) return rc;
+    e->sweep_rows_version = e->rows_version;
     e->pending = true;
     e->waited = false;
     e->have_sweep = true;
@@ -957,7 +1173,8 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
         int unroll, fuse, task, wpc, sched;
     };
     std::vector<Cfg> grid;
-    const int unrolls[] = {2, 4};
+    // rows in flight per lane: a choice for 16 words and wider, fixed below
+    std::vector<int> unrolls = e->words >= 16 ? std::vector<int>{2, 4} : std::vector<int>{0};
     const int fuses[] = {1, 4, 8};
     const int tasks[] = {0, 32, 128};
     const int wpcs[] = {0, 24};
@@ -969,7 +1186,7 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
     const Cfg saved{e->tune_unroll, e->tune_fuse_len, e->tune_task_clauses, e->tune_waves_per_cu,
                     e->tune_schedule};
     auto apply = [&](const Cfg &c) {
-        e->tune_unroll = e->words == 16 ? c.unroll : 0;
+        e->tune_unroll = c.unroll;
         e->tune_fuse_len = c.fuse;
         e->tune_task_clauses = c.task;
         e->tune_waves_per_cu = c.wpc;
@@ -1042,17 +1259,23 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
         HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
     hipStream_t cs = e->capture_stream;
     HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    StageArgs st;
-    st.h_hdr = sl.h_hdr.p;
-    st.h_desc = sl.h_desc.p;
-    st.h_segs = sl.h_segs.p;
-    st.h_lits = sl.h_lits.p;
-    st.h_ups = sl.h_ups.p;
-    st.d_hdr = sl.d_hdr.p;
-    st.d_desc = sl.d_desc.p;
-    st.d_segs = sl.d_segs.p;
-    st.d_lits = sl.d_lits.p;
-    st.d_ups = sl.d_ups.p;
+    StageArgs st{};
+    st.h[kRegHeader] = (const uint32_t *)sl.h_hdr.p;
+    st.h[kRegDesc] = (const uint32_t *)sl.h_desc.p;
+    st.h[kRegSegs] = (const uint32_t *)sl.h_segs.p;
+    st.h[kRegLits] = sl.h_lits.p;
+    st.h[kRegUps] = (const uint32_t *)sl.h_ups.p;
+    st.h[kRegPatches] = (const uint32_t *)sl.h_patches.p;
+    st.h[kRegBaseUps] = (const uint32_t *)sl.h_bups.p;
+    st.h[kRegLoads] = (const uint32_t *)sl.h_loads.p;
+    st.d[kRegHeader] = (uint32_t *)sl.d_hdr.p;
+    st.d[kRegDesc] = (uint32_t *)sl.d_desc.p;
+    st.d[kRegSegs] = (uint32_t *)sl.d_segs.p;
+    st.d[kRegLits] = sl.d_lits.p;
+    st.d[kRegUps] = (uint32_t *)sl.d_ups.p;
+    st.d[kRegPatches] = (uint32_t *)sl.d_patches.p;
+    st.d[kRegBaseUps] = (uint32_t *)sl.d_bups.p;
+    st.d[kRegLoads] = (uint32_t *)sl.d_loads.p;
     StepHeader *dh = sl.d_hdr.p;
     SweepArgs a{};
     a.buckets = sl.d_desc.p;
@@ -1069,7 +1292,14 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     hipError_t he = hipSuccess;
     if (he == hipSuccess) he = launch_stage_in(st, cs);
     if (he == hipSuccess) he = launch_scatter_dyn(sl.d_segs.p, &dh->nsegs, sl.d_lits.p, kLitShift, cs);
+    if (he == hipSuccess)
+        he = launch_base_apply_dyn(sl.d_bups.p, &dh->nbups, e->bases.p, e->bases_vw, cs);
+    if (he == hipSuccess)
+        he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
+                                   e->table.p, e->words, e->vars, cs);
     if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
+    if (he == hipSuccess)
+        he = launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words, cs);
     if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
     if (he == hipSuccess) he = launch_sweep(a, e->words, e->tune_unroll, blocks, cs);
     if (he == hipSuccess)
@@ -1094,11 +1324,51 @@ extern "C" {
 int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
                      uint32_t *out_rows, int64_t n_ups, const gss_var_update *ups,
                      const uint64_t *valid, const int8_t *last_pos) {
+    gss_step_desc d{};
+    d.n_clauses = n;
+    d.lits = lits;
+    d.lens = lens;
+    d.out_rows = out_rows;
+    d.n_ups = n_ups;
+    d.ups = ups;
+    d.valid = valid;
+    d.last_pos = last_pos;
+    return gss_step_submit(e, &d);
+}
+
+int gss_bases_reset(gss_engine *e, int nbases) {
     int rc = check_engine(e);
     if (rc) return rc;
-    if (n < 0 || (n > 0 && (!lits || !lens)) || n_ups < 0 || (n_ups > 0 && !ups) || !valid)
+    if (nbases < 0 || nbases > 65535) return set_err(GSS_E_INVAL, "base count %d", nbases);
+    if (!e->inflight.empty() || e->pending) return set_err(GSS_E_STATE, "work in flight");
+    const int64_t vw = (e->vars + 63) / 64;
+    const size_t n = (size_t)nbases * 2 * (size_t)std::max<int64_t>(vw, 1);
+    if (nbases && (rc = e->bases.reserve(n))) return rc;
+    if (nbases) HIP_TRY(launch_fill_u64(e->bases.p, (int64_t)n, 0ull, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->nbases = nbases;
+    e->bases_vw = vw;
+    return GSS_OK;
+}
+
+int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!desc) return set_err(GSS_E_INVAL, "null step");
+    const int64_t n = desc->n_clauses, n_ups = desc->n_ups, n_patches = desc->n_patches,
+                  n_bups = desc->n_base_ups, n_loads = desc->n_loads;
+    const uint32_t *lits = desc->lits;
+    const int32_t *lens = desc->lens;
+    uint32_t *out_rows = desc->out_rows;
+    const gss_var_update *ups = desc->ups;
+    const uint64_t *valid = desc->valid;
+    const int8_t *last_pos = desc->last_pos;
+    if (n < 0 || (n > 0 && (!lits || !lens)) || n_ups < 0 || (n_ups > 0 && !ups) || !valid ||
+        n_patches < 0 || (n_patches > 0 && !desc->patches) || n_bups < 0 ||
+        (n_bups > 0 && !desc->base_ups) || n_loads < 0 || (n_loads > 0 && !desc->loads))
         return set_err(GSS_E_INVAL, "bad step arguments");
     if (e->pending) return set_err(GSS_E_STATE, "a plain sweep is in flight");
+    if (e->tagged && n > 0) return set_err(GSS_E_STATE, "tagged DB: append with gss_db_append_tagged");
     const int S = e->next_slot;
     StepSlot &sl = e->slots[S];
     if (sl.busy) return set_err(GSS_E_STATE, "two steps in flight: wait for the oldest first");
@@ -1108,6 +1378,22 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
         if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
             return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
                            (long long)i, ups[i].var, ups[i].word);
+    for (int64_t i = 0; i < n_patches; i++)
+        if (desc->patches[i].var >= (uint64_t)e->vars || desc->patches[i].word >= (uint32_t)W)
+            return set_err(GSS_E_INVAL, "patch %lld out of range", (long long)i);
+    if ((n_bups || n_loads) && e->bases_vw != (e->vars + 63) / 64)
+        return set_err(GSS_E_STATE, "the table changed size: reset the bases");
+    for (int64_t i = 0; i < n_bups; i++)
+        if (desc->base_ups[i].var >= (uint64_t)e->vars || desc->base_ups[i].base >= e->nbases)
+            return set_err(GSS_E_INVAL, "base update %lld out of range", (long long)i);
+    uint64_t loaded = 0;
+    for (int64_t i = 0; i < n_loads; i++) {
+        const gss_word_load &ld = desc->loads[i];
+        if (ld.word >= W || ld.base >= e->nbases)
+            return set_err(GSS_E_INVAL, "word load %lld out of range", (long long)i);
+        if ((loaded >> ld.word) & 1) return set_err(GSS_E_INVAL, "word %u loaded twice", ld.word);
+        loaded |= 1ull << ld.word;
+    }
     int active_words = 0;
     for (int w = 0; w < W; w++) {
         if (last_pos && last_pos[w] > 63) return set_err(GSS_E_INVAL, "last_pos > 63");
@@ -1171,8 +1457,13 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
         (rc = sl.h_segs.reserve(std::max(nseg, 1))) ||
         (rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1))) ||
         (rc = sl.h_desc.reserve(std::max(nb, 1))) || (rc = sl.d_hdr.reserve(1)) ||
+        (rc = sl.h_patches.reserve(std::max<int64_t>(n_patches, 1))) ||
+        (rc = sl.h_bups.reserve(std::max<int64_t>(n_bups, 1))) ||
+        (rc = sl.h_loads.reserve(std::max<int64_t>(n_loads, 1))) ||
         (rc = sl.d_lits.reserve(sl.h_lits.cap)) || (rc = sl.d_segs.reserve(sl.h_segs.cap)) ||
-        (rc = sl.d_ups.reserve(sl.h_ups.cap)) || (rc = sl.d_desc.reserve(sl.h_desc.cap)))
+        (rc = sl.d_ups.reserve(sl.h_ups.cap)) || (rc = sl.d_desc.reserve(sl.h_desc.cap)) ||
+        (rc = sl.d_patches.reserve(sl.h_patches.cap)) || (rc = sl.d_bups.reserve(sl.h_bups.cap)) ||
+        (rc = sl.d_loads.reserve(sl.h_loads.cap)))
         return rc;
     if (chunks > sl.chunk_cap) {
         chunks += chunks / 4;  // headroom for the DB growing over the next steps
@@ -1189,8 +1480,10 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
     const int fuse = e->tune_fuse_len > 0 ? e->tune_fuse_len : kDefaultFuseLen;
     std::vector<const void *> key = {
         sl.h_hdr.p, sl.h_desc.p, sl.h_segs.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
+        sl.h_patches.p, sl.h_bups.p, sl.h_loads.p,
         sl.d_hdr.p, sl.d_desc.p, sl.d_segs.p, sl.d_lits.p, sl.d_ups.p, sl.d_hits.p,
-        sl.d_fills.p, e->table.p,
+        sl.d_patches.p, sl.d_bups.p, sl.d_loads.p,
+        sl.d_fills.p, e->table.p, e->bases.p, (const void *)(intptr_t)e->bases_vw,
         (const void *)(intptr_t)W, (const void *)(intptr_t)e->vars,
         (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
@@ -1233,6 +1526,11 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
     e->total_lits += total;
     if (n) e->max_lit = std::max<int64_t>(e->max_lit, mx);
     if (n_ups) std::memcpy(sl.h_ups.p, ups, (size_t)n_ups * sizeof(gss_var_update));
+    if (n_patches)
+        std::memcpy(sl.h_patches.p, desc->patches, (size_t)n_patches * sizeof(gss_var_patch));
+    if (n_bups)
+        std::memcpy(sl.h_bups.p, desc->base_ups, (size_t)n_bups * sizeof(gss_base_update));
+    if (n_loads) std::memcpy(sl.h_loads.p, desc->loads, (size_t)n_loads * sizeof(gss_word_load));
     // descriptors of every non-empty bucket, the step's own inserts included
     int k = 0;
     int64_t t = 0;
@@ -1254,8 +1552,18 @@ int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32
     h.nbuckets = k;
     h.ntasks = (int32_t)t;
     h.nsegs = si;
-    h.nlits = total;
     h.nups = n_ups;
+    h.npatches = n_patches;
+    h.nbups = n_bups;
+    h.nloads = n_loads;
+    h.region_words[kRegHeader] = sizeof(StepHeader) / 4;
+    h.region_words[kRegDesc] = (int64_t)k * (sizeof(BucketDesc) / 4);
+    h.region_words[kRegSegs] = (int64_t)si * (sizeof(ScatterSeg) / 4);
+    h.region_words[kRegLits] = total;
+    h.region_words[kRegUps] = n_ups * (int64_t)(sizeof(gss_var_update) / 4);
+    h.region_words[kRegPatches] = n_patches * (int64_t)(sizeof(gss_var_patch) / 4);
+    h.region_words[kRegBaseUps] = n_bups * (int64_t)(sizeof(gss_base_update) / 4);
+    h.region_words[kRegLoads] = n_loads * (int64_t)(sizeof(gss_word_load) / 4);
     for (int w = 0; w < GSS_MAX_WORDS; w++) {
         h.valid[w] = w < W ? valid[w] : 0;
         h.last_pos[w] = (w < W && last_pos) ? last_pos[w] : -1;

@@ -1,0 +1,28 @@
+// Library-internal access to an engine's device state, for modules built into
+// the same shared object (the residency scorer).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "gss.h"
+
+namespace gss {
+
+// Compacts a tagged DB on the device keeping rows whose tag t has d_keep[t]
+// != 0 (d_keep is device memory; tags >= ntags are kept).
+int db_compact_device(gss_engine *e, const uint8_t *d_keep, int64_t ntags);
+// Device tag array of every bucket, indexed by clause length (null if none).
+int db_tag_arrays(gss_engine *e, std::vector<const uint32_t *> &tags);
+// The chunked hit records of the last waited plain sweep, on the device:
+// chunk c holds fills[c] records starting at hits + c * kHitChunk.  Refused
+// once rows have moved since that sweep (compaction, bucket replacement,
+// clear), since the records' (len, row) would no longer name the same rows.
+int sweep_records(gss_engine *e, const gss_hit **hits, const uint32_t **fills,
+                  uint32_t *chunks);
+hipStream_t engine_stream(gss_engine *e);
+// Makes the engine's device current (GSS_OK or an error code).
+int engine_activate(gss_engine *e);
+
+}  // namespace gss

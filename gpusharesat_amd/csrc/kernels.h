@@ -37,6 +37,19 @@ struct SweepArgs {
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
 };
 
+// Staged regions of a batched step, copied host -> device by k_stage_in.
+enum StepRegion {
+    kRegHeader = 0,
+    kRegDesc,
+    kRegSegs,
+    kRegLits,
+    kRegUps,
+    kRegPatches,
+    kRegBaseUps,
+    kRegLoads,
+    kStageRegions
+};
+
 // Per-step inputs of a batched step (gss_step_*), written by the host into a
 // pinned staging slot and copied to the device by k_stage_in.
 struct StepHeader {
@@ -44,23 +57,15 @@ struct StepHeader {
     int32_t ntasks;
     int32_t nsegs;
     int32_t pad0;
-    int64_t nlits;
-    int64_t nups;
+    int64_t nups, npatches, nbups, nloads;
+    int64_t region_words[kStageRegions];  // uint32 words of each region to copy
     uint64_t valid[GSS_MAX_WORDS];
     int8_t last_pos[GSS_MAX_WORDS];
 };
 
 struct StageArgs {
-    const StepHeader *h_hdr;  // pinned host
-    const BucketDesc *h_desc;
-    const void *h_segs;
-    const uint32_t *h_lits;
-    const gss_var_update *h_ups;
-    StepHeader *d_hdr;        // device
-    BucketDesc *d_desc;
-    void *d_segs;
-    uint32_t *d_lits;
-    gss_var_update *d_ups;
+    const uint32_t *h[kStageRegions];  // pinned host (h[kRegHeader] = the header)
+    uint32_t *d[kStageRegions];        // device
 };
 
 struct ScatterSeg {
@@ -69,8 +74,19 @@ struct ScatterSeg {
     uint64_t count;  // uint32 elements
 };
 
+// One non-empty bucket of a tagged DB during compaction.
+struct CompactRef {
+    const uint32_t *lits;  // source rows (stored offsets)
+    const uint32_t *tags;
+    uint32_t *dst_lits;    // kept rows, in order
+    uint32_t *dst_tags;
+    int64_t start;         // first row of this bucket in the concatenation
+    uint32_t len;
+    uint32_t pad;
+};
+
 // words is the table row width (power of two, 1..64); unroll selects the
-// number of rows in flight per lane for 16 words (0 = default).
+// number of rows in flight per lane for 16 words and wider (0 = default).
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
 // Max resident 256-thread blocks per CU for the sweep kernel variant.
@@ -96,5 +112,27 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
                                  int words, const int8_t *last_pos, hipStream_t s);
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s);
+// Solver base planes: bases[(2 b) * vw + i] / bases[(2 b + 1) * vw + i] hold
+// bit (v mod 64) of word i = v / 64 of base b's is_def / is_true.
+hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uint64_t *bases,
+                                 int64_t vw, hipStream_t s);
+// Sets word w of every variable's two rows from base b, for each load.
+hipError_t launch_words_load_dyn(const gss_word_load *loads, const int64_t *n,
+                                 const uint64_t *bases, int64_t vw, uint64_t *table, int words,
+                                 int64_t vars, hipStream_t s);
+hipError_t launch_assig_patch_dyn(const gss_var_patch *p, const int64_t *n, uint64_t *table,
+                                  int words, hipStream_t s);
+
+// Compaction of a tagged DB (rows of every bucket concatenated, N rows):
+// flags[g] = keep[tag of row g] (tags >= ntags are kept), an exclusive scan
+// of the flags, the kept count of each bucket, then the moves.
+hipError_t launch_compact_flags(const CompactRef *refs, int nb, int64_t n, const uint8_t *keep,
+                                int64_t ntags, uint32_t *flags, hipStream_t s);
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, void *temp,
+                              size_t &temp_bytes, hipStream_t s);
+hipError_t launch_compact_bounds(const CompactRef *refs, int nb, int64_t n, const uint32_t *flags,
+                                 const uint32_t *pos, uint32_t *kept, hipStream_t s);
+hipError_t launch_compact_move(const CompactRef *refs, int nb, int64_t n, const uint32_t *flags,
+                               const uint32_t *pos, hipStream_t s);
 
 }  // namespace gss

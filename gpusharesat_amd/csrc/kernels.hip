@@ -34,8 +34,11 @@
 // the end of a clause hold the offset of a sentinel row pair (nf = 0, and
 // all ones for its negation) that contributes nothing, so they need no mask.
 //
-// Work mapping.  A group of G = W*S lanes owns one clause: lane (s, w) handles
-// table word w and literal sub-lane s.  The group loads G literals of the
+// Work mapping.  A group of 16 lanes owns one clause.  Up to 16 words, lane
+// (s, w) of the group handles table word w and literal sub-lane s (G = W*S =
+// 16); at 32 and 64 words each lane handles K = W/16 consecutive words, read
+// with 16-byte loads, so every width checks 4 clauses per wave pass and keeps
+// 4*U row reads per wave in flight.  The group loads G literals of the
 // clause at once (lane q holds literal q of the chunk; the first chunk of the
 // next clause is prefetched while the current one is checked) and hands them
 // out with DPP row broadcasts (G = 16: a group is one 16-lane DPP row) or
@@ -55,6 +58,7 @@
 // counter still counts and the host re-runs the sweep.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
 #include <utility>
 
 #include "kernels.h"
@@ -105,14 +109,38 @@ __device__ __forceinline__ uint32_t chunk_addr(uint32_t litreg, int gbase, uint3
     }
 }
 
-__device__ __forceinline__ uint64_t load_row(const uint64_t *__restrict__ table, uint32_t off) {
-    return *(const uint64_t *)((const char *)table + off);
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// K consecutive table words of one row (K = 1, 2 or 4), read with 8- or
+// 16-byte loads.
+template <int K>
+struct Words {
+    uint64_t v[K];
+};
+
+template <int K>
+__device__ __forceinline__ Words<K> load_words(const uint64_t *__restrict__ table, uint32_t off) {
+    Words<K> r;
+    const char *p = (const char *)table + off;
+    if constexpr (K == 1) {
+        r.v[0] = *(const uint64_t *)p;
+    } else {
+#pragma unroll
+        for (int i = 0; i < K; i += 2) {
+            const u64x2 x = *(const u64x2 *)(p + 8 * i);
+            r.v[i] = x.x;
+            r.v[i + 1] = x.y;
+        }
+    }
+    return r;
 }
 
-// Clause state of one lane: saturating count of non-false occurrences per slot
-// (c1: >= 1, c2: >= 2) and the slots where no literal is true (nt).
+// Clause state of one lane for its K words: saturating count of non-false
+// occurrences per slot (c1: >= 1, c2: >= 2) and the slots where no literal is
+// true (nt).
+template <int K>
 struct ClauseState {
-    uint64_t c1, c2, nt;
+    uint64_t c1[K], c2[K], nt[K];
 };
 
 // Offset of the negated literal's row: stored offsets are literal << 7, so the
@@ -126,11 +154,13 @@ __device__ __forceinline__ uint32_t neg_bit() {
 // [base, base + G), sentinel past the clause end; this step covers chunk
 // positions [ST, ST + S*U).  The U row reads are issued back to back, folded
 // into the state, and for S > 1 the S sub-lanes are merged with a butterfly.
-template <int W, int S, int U, int ST, bool ROWS_F, bool ROWS_T>
+template <int W, int S, int U, int K, int ST, bool ROWS_F, bool ROWS_T>
 __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, uint32_t w8,
-                                          const uint64_t *__restrict__ table, ClauseState &cs) {
-    constexpr int G = W * S;
-    uint64_t f[U], t[U];
+                                          const uint64_t *__restrict__ table,
+                                          ClauseState<K> &cs) {
+    constexpr int LW = W / K;  // lanes across one row
+    constexpr int G = LW * S;
+    Words<K> f[U], t[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         uint32_t off;
@@ -144,27 +174,31 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, u
         } else {
             off = scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u)) | w8;
         }
-        if (ROWS_F) f[u] = load_row(table, off);
-        if (ROWS_T) t[u] = load_row(table, off ^ neg_bit<W>());
+        if (ROWS_F) f[u] = load_words<K>(table, off);
+        if (ROWS_T) t[u] = load_words<K>(table, off ^ neg_bit<W>());
     }
     if constexpr (S == 1) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (ROWS_F) {
-                cs.c2 |= cs.c1 & f[u];
-                cs.c1 |= f[u];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (ROWS_F) {
+                    cs.c2[k] |= cs.c1[k] & f[u].v[k];
+                    cs.c1[k] |= f[u].v[k];
+                }
+                if (ROWS_T) cs.nt[k] &= t[u].v[k];
             }
-            if (ROWS_T) cs.nt &= t[u];
         }
     } else {
+        static_assert(K == 1, "sub-lanes split narrow rows only");
         uint64_t m1 = 0, m2 = 0, tt = ~0ull;
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (ROWS_F) {
-                m2 |= m1 & f[u];
-                m1 |= f[u];
+                m2 |= m1 & f[u].v[0];
+                m1 |= f[u].v[0];
             }
-            if (ROWS_T) tt &= t[u];
+            if (ROWS_T) tt &= t[u].v[0];
         }
 #pragma unroll
         for (int d = W; d < G; d <<= 1) {
@@ -177,27 +211,28 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, u
             if (ROWS_T) tt &= __shfl_xor(tt, d);
         }
         if (ROWS_F) {
-            cs.c2 |= m2 | (cs.c1 & m1);
-            cs.c1 |= m1;
+            cs.c2[0] |= m2 | (cs.c1[0] & m1);
+            cs.c1[0] |= m1;
         }
-        if (ROWS_T) cs.nt &= tt;
+        if (ROWS_T) cs.nt[0] &= tt;
     }
 }
 
 // Runs the steps of one preloaded chunk while they start before the clause
 // end and `more(cs)` holds.  Returns false if it stopped early.
-template <int W, int S, int U, bool ROWS_F, bool ROWS_T, typename More>
+template <int W, int S, int U, int K, bool ROWS_F, bool ROWS_T, typename More>
 __device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub, uint32_t w8,
                                             int base, int len,
-                                            const uint64_t *__restrict__ table, ClauseState &cs,
-                                            More &&more) {
-    constexpr int G = W * S;
+                                            const uint64_t *__restrict__ table,
+                                            ClauseState<K> &cs, More &&more) {
+    constexpr int G = (W / K) * S;
     constexpr int LPS = S * U;
     bool go = true;
     [&]<int... I>(std::integer_sequence<int, I...>) {
         ((go = go && (base + I * LPS < len) &&
                [&] {
-                   step_rows<W, S, U, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, w8, table, cs);
+                   step_rows<W, S, U, K, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, w8, table,
+                                                                 cs);
                    return more(cs);
                }()),
          ...);
@@ -205,46 +240,66 @@ __device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub,
     return go;
 }
 
-// Evaluates one clause for this lane's word.  All G lanes of the group call it
-// together (group-uniform control flow).  first_chunk holds literal q of the
-// clause in lane q of the group (the sentinel past the end), already loaded by
-// the caller.
-template <int W, int S, int U>
+// Evaluates one clause for this lane's K words.  All G lanes of the group call
+// it together (group-uniform control flow).  first_chunk holds literal q of
+// the clause in lane q of the group (the sentinel past the end), already
+// loaded by the caller.
+template <int W, int S, int U, int K>
 __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, int len,
                                              uint32_t first_chunk, uint32_t sentinel,
                                              const uint64_t *__restrict__ table, int q,
                                              uint64_t gmask, int gbase, uint32_t w8, int sub,
-                                             uint64_t valid, int fuse_len, uint64_t &fm,
-                                             uint64_t &um) {
-    constexpr int G = W * S;
+                                             const uint64_t (&valid)[K], int fuse_len,
+                                             uint64_t (&fm)[K], uint64_t (&um)[K]) {
+    constexpr int G = (W / K) * S;
     // invalid slots start dead
-    ClauseState cs{~valid, ~valid, ~0ull};
+    ClauseState<K> cs;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        cs.c1[k] = ~valid[k];
+        cs.c2[k] = ~valid[k];
+        cs.nt[k] = ~0ull;
+    }
     const bool fused = len <= fuse_len;
-    auto alive = [&](const ClauseState &c) {
-        return group_any<G>((c.nt & ~c.c2) != 0, gmask);
+    auto alive = [&](const ClauseState<K> &c) {
+        uint64_t x = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) x |= c.nt[k] & ~c.c2[k];
+        return group_any<G>(x != 0, gmask);
     };
     uint32_t litreg = first_chunk;
     for (int base = 0; base < len; base += G) {
         if (base) litreg = base + q < len ? cl[base + q] : sentinel;
-        const bool go = fused ? chunk_steps<W, S, U, true, true>(litreg, gbase, sub, w8, base, len,
-                                                                 table, cs, alive)
-                              : chunk_steps<W, S, U, true, false>(litreg, gbase, sub, w8, base,
-                                                                  len, table, cs, alive);
+        const bool go = fused ? chunk_steps<W, S, U, K, true, true>(litreg, gbase, sub, w8, base,
+                                                                    len, table, cs, alive)
+                              : chunk_steps<W, S, U, K, true, false>(litreg, gbase, sub, w8,
+                                                                     base, len, table, cs, alive);
         if (!go) break;  // dead, or the clause ended inside this chunk
     }
-    fm = ~cs.c1;
-    const uint64_t cand = cs.c1 & ~cs.c2 & cs.nt;
-    if (!fused && group_any<G>(cand != 0, gmask)) {
-        auto open = [&](const ClauseState &c) { return group_any<G>((cand & c.nt) != 0, gmask); };
+    uint64_t cand[K], any = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        fm[k] = ~cs.c1[k];
+        cand[k] = cs.c1[k] & ~cs.c2[k] & cs.nt[k];
+        any |= cand[k];
+    }
+    if (!fused && group_any<G>(any != 0, gmask)) {
+        auto open = [&](const ClauseState<K> &c) {
+            uint64_t x = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) x |= cand[k] & c.nt[k];
+            return group_any<G>(x != 0, gmask);
+        };
         litreg = first_chunk;
         for (int base = 0; base < len; base += G) {
             if (base) litreg = base + q < len ? cl[base + q] : sentinel;
-            if (!chunk_steps<W, S, U, false, true>(litreg, gbase, sub, w8, base, len, table, cs,
-                                                   open))
+            if (!chunk_steps<W, S, U, K, false, true>(litreg, gbase, sub, w8, base, len, table,
+                                                      cs, open))
                 break;
         }
     }
-    um = cand & cs.nt;
+#pragma unroll
+    for (int k = 0; k < K; k++) um[k] = cand[k] & cs.nt[k];
 }
 
 __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int nb, int task) {
@@ -259,17 +314,21 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
 
 template <int W, int S, int U>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
-    constexpr int G = W * S;
+    constexpr int K = W > 16 ? W / 16 : 1;  // table words per lane
+    constexpr int LW = W / K;               // lanes across one row
+    constexpr int G = LW * S;
     constexpr int CPW = 64 / G;  // clauses per wave pass
     const int lane = (int)__lane_id();
     const int g = lane / G;
     const int q = lane % G;
-    const int sub = q / W;
-    const int w = lane % W;
+    const int sub = q / LW;
+    const int lw = lane % LW;
     const int gbase = g * G;
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
-    const uint32_t w8 = (uint32_t)w * 8u;
-    const uint64_t valid = a.valid[w];
+    const uint32_t w8 = (uint32_t)(lw * K) * 8u;
+    uint64_t valid[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) valid[k] = a.valid[lw * K + k];
     // wave id and count are wave-uniform: keep the task walk scalar
     const int gw = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nw = (int)((gridDim.x * blockDim.x) >> 6);
@@ -298,34 +357,39 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
             const uint32_t chunk0 = next_chunk;
             const int nxt = it + CPW + g;
             next_chunk = (nxt < nrows && q < len) ? rows_base[(size_t)nxt * len + q] : a.sentinel;
-            uint64_t fm = 0, um = 0;
+            uint64_t fm[K], um[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) fm[k] = um[k] = 0;
             if (active) {
-                check_clause<W, S, U>(bd.lits + (size_t)row * len, len, chunk0, a.sentinel,
-                                      a.table, q, gmask, gbase, w8, sub, valid, a.fuse_len, fm,
-                                      um);
+                check_clause<W, S, U, K>(bd.lits + (size_t)row * len, len, chunk0, a.sentinel,
+                                         a.table, q, gmask, gbase, w8, sub, valid, a.fuse_len,
+                                         fm, um);
             }
-            const bool hit = active && sub == 0 && (fm | um) != 0;
-            const unsigned long long bal = __ballot(hit);
-            if (bal) {
-                const int n = __popcll(bal);
-                if (chunk < 0 || fill + n > kHitChunk) {
-                    if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
-                        a.fills[chunk] = (uint32_t)fill;
-                    uint32_t c = 0;
-                    if (lane == 0) c = atomicAdd(a.alloc, 1u);
-                    chunk = uniform((int)__shfl(c, 0));
-                    fill = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const bool hit = active && sub == 0 && (fm[k] | um[k]) != 0;
+                const unsigned long long bal = __ballot(hit);
+                if (bal) {
+                    const int n = __popcll(bal);
+                    if (chunk < 0 || fill + n > kHitChunk) {
+                        if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
+                            a.fills[chunk] = (uint32_t)fill;
+                        uint32_t c = 0;
+                        if (lane == 0) c = atomicAdd(a.alloc, 1u);
+                        chunk = uniform((int)__shfl(c, 0));
+                        fill = 0;
+                    }
+                    if (hit && (uint32_t)chunk < a.chunk_cap) {
+                        gss_hit h;
+                        h.row = (uint32_t)row;
+                        h.len = (uint16_t)len;
+                        h.word = (uint16_t)(lw * K + k);
+                        h.false_mask = fm[k];
+                        h.unit_mask = um[k];
+                        a.hits[(size_t)chunk * kHitChunk + fill + prefix_count(bal)] = h;
+                    }
+                    fill += n;
                 }
-                if (hit && (uint32_t)chunk < a.chunk_cap) {
-                    gss_hit h;
-                    h.row = (uint32_t)row;
-                    h.len = (uint16_t)len;
-                    h.word = (uint16_t)w;
-                    h.false_mask = fm;
-                    h.unit_mask = um;
-                    a.hits[(size_t)chunk * kHitChunk + fill + prefix_count(bal)] = h;
-                }
-                fill += n;
             }
         }
     }
@@ -383,27 +447,71 @@ __global__ void k_assig_roll(const gss_var_update *__restrict__ ups, int64_t n,
 // ---- batched steps: fixed launch shapes, counts from device memory --------
 
 // Copies a step's staged inputs from pinned host memory to the device.  The
-// counts are read once per block from the pinned header.
+// region sizes are read once per block from the pinned header.
 __global__ void k_stage_in(StageArgs a) {
-    __shared__ int64_t n[4];  // desc, segs, lits, ups (in uint32 words)
-    if (threadIdx.x == 0) {
-        const StepHeader *h = a.h_hdr;
-        n[0] = (int64_t)h->nbuckets * (sizeof(BucketDesc) / 4);
-        n[1] = (int64_t)h->nsegs * (sizeof(ScatterSeg) / 4);
-        n[2] = h->nlits;
-        n[3] = h->nups * (int64_t)(sizeof(gss_var_update) / 4);
-    }
+    __shared__ int64_t n[kStageRegions];
+    if (threadIdx.x < kStageRegions)
+        n[threadIdx.x] = ((const StepHeader *)a.h[kRegHeader])->region_words[threadIdx.x];
     __syncthreads();
     const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const uint32_t *src[5] = {(const uint32_t *)a.h_hdr, (const uint32_t *)a.h_desc,
-                              (const uint32_t *)a.h_segs, a.h_lits, (const uint32_t *)a.h_ups};
-    uint32_t *dst[5] = {(uint32_t *)a.d_hdr, (uint32_t *)a.d_desc, (uint32_t *)a.d_segs, a.d_lits,
-                        (uint32_t *)a.d_ups};
-    const int64_t cnt[5] = {(int64_t)(sizeof(StepHeader) / 4), n[0], n[1], n[2], n[3]};
 #pragma unroll
-    for (int r = 0; r < 5; r++)
-        for (int64_t i = tid; i < cnt[r]; i += stride) dst[r][i] = src[r][i];
+    for (int r = 0; r < kStageRegions; r++) {
+        const uint32_t *src = a.h[r];
+        uint32_t *dst = a.d[r];
+        for (int64_t i = tid; i < n[r]; i += stride) dst[i] = src[i];
+    }
+}
+
+__global__ void k_base_apply_dyn(const gss_base_update *__restrict__ u, const int64_t *n_p,
+                                 uint64_t *bases, int64_t vw) {
+    const int64_t n = *n_p;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const gss_base_update x = u[i];
+        const uint64_t bit = 1ull << (x.var & 63);
+        unsigned long long *def = (unsigned long long *)bases + (size_t)(2 * x.base) * vw + x.var / 64;
+        unsigned long long *tru = def + vw;
+        if (x.is_def) atomicOr(def, bit);
+        else atomicAnd(def, ~bit);
+        if (x.is_def && x.is_true) atomicOr(tru, bit);
+        else atomicAnd(tru, ~bit);
+    }
+}
+
+// Element i covers variable i / nloads and load i % nloads, so the lanes of
+// a wave write neighbouring words of the same rows.
+__global__ void k_words_load_dyn(const gss_word_load *__restrict__ loads, const int64_t *n_p,
+                                 const uint64_t *__restrict__ bases, int64_t vw,
+                                 uint64_t *__restrict__ table, int words, int64_t vars) {
+    const int64_t nl = *n_p;
+    const int64_t total = nl * vars;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i / nl;
+        const gss_word_load ld = loads[i - v * nl];
+        const uint64_t *def = bases + (size_t)(2 * ld.base) * vw;
+        const uint64_t bit = 1ull << (v & 63);
+        const bool d = (def[v >> 6] & bit) != 0;
+        const bool t = (def[vw + (v >> 6)] & bit) != 0;
+        const size_t row = (size_t)v * 2 * words + ld.word;
+        table[row] = (d && !t) ? 0ull : ~0ull;     // literal v is false where v is false
+        table[row + words] = (d && t) ? 0ull : ~0ull;  // literal ~v is false where v is true
+    }
+}
+
+// Slots in mask take the patch's value; the others keep theirs.
+__global__ void k_assig_patch_dyn(const gss_var_patch *__restrict__ p, const int64_t *n_p,
+                                  uint64_t *__restrict__ table, int words) {
+    const int64_t n = *n_p;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const gss_var_patch x = p[i];
+        const size_t base = (size_t)x.var * 2 * words + x.word;
+        const uint64_t m = x.mask;
+        table[base] = (table[base] & ~m) | (~(x.is_def & ~x.is_true) & m);
+        table[base + words] = (table[base + words] & ~m) | (~(x.is_def & x.is_true) & m);
+    }
 }
 
 __global__ void k_scatter_dyn(const ScatterSeg *__restrict__ segs, const int32_t *nseg_p,
@@ -455,6 +563,58 @@ __global__ void k_stage_out(const uint32_t *alloc, const uint32_t *__restrict__ 
     for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) h_out[1 + i] = fills[i];
 }
 
+// ---- tagged DB compaction --------------------------------------------------
+
+__device__ __forceinline__ int find_ref(const CompactRef *__restrict__ r, int nb, int64_t g) {
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (r[mid].start <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_compact_flags(const CompactRef *__restrict__ refs, int nb, int64_t n,
+                                const uint8_t *__restrict__ keep, int64_t ntags,
+                                uint32_t *__restrict__ flags) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int b = find_ref(refs, nb, g);
+        const uint32_t tag = refs[b].tags[g - refs[b].start];
+        flags[g] = tag < (uint64_t)ntags ? (keep[tag] ? 1u : 0u) : 1u;
+    }
+}
+
+__global__ void k_compact_bounds(const CompactRef *__restrict__ refs, int nb, int64_t n,
+                                 const uint32_t *__restrict__ flags,
+                                 const uint32_t *__restrict__ pos, uint32_t *__restrict__ kept) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t st = refs[b].start;
+    const int64_t en = b + 1 < nb ? refs[b + 1].start : n;
+    const uint32_t pe = en < n ? pos[en] : pos[n - 1] + flags[n - 1];
+    kept[b] = pe - pos[st];
+}
+
+// One thread per source row: kept rows move to their rank within the bucket.
+__global__ void k_compact_move(const CompactRef *__restrict__ refs, int nb, int64_t n,
+                               const uint32_t *__restrict__ flags,
+                               const uint32_t *__restrict__ pos) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        if (!flags[g]) continue;
+        const int b = find_ref(refs, nb, g);
+        const CompactRef r = refs[b];
+        const int64_t row = g - r.start;
+        const int64_t dst = (int64_t)(pos[g] - pos[r.start]);
+        const uint32_t *src = r.lits + row * r.len;
+        uint32_t *d = r.dst_lits + dst * r.len;
+        for (uint32_t i = 0; i < r.len; i++) d[i] = src[i];
+        r.dst_tags[dst] = r.tags[row];
+    }
+}
+
 template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
     hipLaunchKernelGGL((k_sweep<W, S, U>), dim3(blocks), dim3(256), 0, s, a);
@@ -470,7 +630,7 @@ int occupancy_t() {
     return nb > 0 ? nb : 1;
 }
 
-// Kernel variants: (W, S, U).  `unroll` selects U for W = 16 (0 = default).
+// Kernel variants: (W, S, U).  `unroll` selects U for W >= 16 (0 = default).
 #define GSS_SWEEP_VARIANTS(X)                  \
     X(1, 16, 1)                                \
     X(2, 8, 2)                                 \
@@ -480,8 +640,11 @@ int occupancy_t() {
     X(16, 1, 1)                                \
     X(16, 1, 2)                                \
     X(16, 1, 8)                                \
+    X(32, 1, 2)                                \
     X(32, 1, 4)                                \
-    X(64, 1, 2)
+    X(64, 1, 1)                                \
+    X(64, 1, 2)                                \
+    X(64, 1, 4)
 
 struct Variant {
     int w, s, u;
@@ -496,15 +659,17 @@ Variant pick(int words, int unroll) {
         case 16:
             if (unroll == 1 || unroll == 2 || unroll == 8) return {16, 1, unroll};
             return {16, 1, 4};
-        case 32: return {32, 1, 4};
-        case 64: return {64, 1, 2};
+        case 32: return {32, 1, unroll == 2 ? 2 : 4};
+        case 64: return {64, 1, unroll == 1 || unroll == 4 ? unroll : 2};
         default: return {0, 0, 0};
     }
 }
 
 }  // namespace
 
-int sweep_clauses_per_pass(int words) { return words <= 16 ? 4 : 64 / words; }
+// Every variant gives a clause a group of 16 lanes (W*S = 16 for W <= 16, 16
+// lanes of W/16 words each above), so a wave pass checks 4 clauses.
+int sweep_clauses_per_pass(int) { return 4; }
 
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s) {
     const Variant v = pick(words, unroll);
@@ -559,6 +724,26 @@ hipError_t launch_stage_in(const StageArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uint64_t *bases,
+                                 int64_t vw, hipStream_t s) {
+    hipLaunchKernelGGL(k_base_apply_dyn, dim3(256), dim3(256), 0, s, u, n, bases, vw);
+    return hipGetLastError();
+}
+
+hipError_t launch_words_load_dyn(const gss_word_load *loads, const int64_t *n,
+                                 const uint64_t *bases, int64_t vw, uint64_t *table, int words,
+                                 int64_t vars, hipStream_t s) {
+    hipLaunchKernelGGL(k_words_load_dyn, dim3(4096), dim3(256), 0, s, loads, n, bases, vw, table,
+                       words, vars);
+    return hipGetLastError();
+}
+
+hipError_t launch_assig_patch_dyn(const gss_var_patch *p, const int64_t *n, uint64_t *table,
+                                  int words, hipStream_t s) {
+    hipLaunchKernelGGL(k_assig_patch_dyn, dim3(1024), dim3(256), 0, s, p, n, table, words);
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter_dyn(const ScatterSeg *segs, const int32_t *nseg, const uint32_t *src,
                               int shift, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_dyn, dim3(1024), dim3(256), 0, s, segs, nseg, src, shift);
@@ -581,6 +766,40 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_out, dim3(256), dim3(256), 0, s, alloc, fills, cap, h_out);
+    return hipGetLastError();
+}
+
+static unsigned grid_for(int64_t n, unsigned cap = 65536) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : b > cap ? cap : b);
+}
+
+hipError_t launch_compact_flags(const CompactRef *refs, int nb, int64_t n, const uint8_t *keep,
+                                int64_t ntags, uint32_t *flags, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_flags, dim3(grid_for(n)), dim3(256), 0, s, refs, nb, n, keep,
+                       ntags, flags);
+    return hipGetLastError();
+}
+
+hipError_t scan_exclusive_u32(const uint32_t *in, uint32_t *out, int64_t n, void *temp,
+                              size_t &temp_bytes, hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int)n, s);
+}
+
+hipError_t launch_compact_bounds(const CompactRef *refs, int nb, int64_t n, const uint32_t *flags,
+                                 const uint32_t *pos, uint32_t *kept, hipStream_t s) {
+    if (nb <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_bounds, dim3((nb + 255) / 256), dim3(256), 0, s, refs, nb, n,
+                       flags, pos, kept);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_move(const CompactRef *refs, int nb, int64_t n, const uint32_t *flags,
+                               const uint32_t *pos, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_compact_move, dim3(grid_for(n)), dim3(256), 0, s, refs, nb, n, flags,
+                       pos);
     return hipGetLastError();
 }
 

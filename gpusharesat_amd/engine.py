@@ -115,6 +115,32 @@ class Engine:
     def clear(self) -> None:
         _lib.call("gss_db_clear", self._h)
 
+    def enable_tags(self) -> None:
+        _lib.call("gss_db_enable_tags", self._h)
+
+    def append_tagged(self, lits: np.ndarray, lens: np.ndarray, tags: np.ndarray) -> np.ndarray:
+        lits = np.ascontiguousarray(lits, dtype=np.uint32)
+        lens = np.ascontiguousarray(lens, dtype=np.int32)
+        tags = np.ascontiguousarray(tags, dtype=np.uint32)
+        if int(lens.sum(dtype=np.int64)) != lits.size or tags.size != lens.size:
+            raise ValueError("sum(lens) != len(lits) or one tag per clause missing")
+        rows = np.empty(lens.size, dtype=np.uint32)
+        _lib.call("gss_db_append_tagged", self._h, lens.size, _ptr(lits), _ptr(lens), _ptr(tags),
+                  _ptr(rows))
+        return rows
+
+    def read_tags(self, length: int) -> np.ndarray:
+        n = self.bucket_count(length)
+        out = np.empty(n, dtype=np.uint32)
+        if n:
+            _lib.call("gss_db_read_tags", self._h, length, 0, n, _ptr(out))
+        return out
+
+    def compact(self, keep: np.ndarray) -> None:
+        """Keeps the rows whose tag t has keep[t] set (tags >= keep.size kept)."""
+        keep = np.ascontiguousarray(keep, dtype=np.uint8)
+        _lib.call("gss_db_compact", self._h, _ptr(keep) if keep.size else None, keep.size)
+
     # -- assignment table -----------------------------------------------------
     def reset_assignments(self, var_count: int, words: int) -> None:
         w = round_words(words)

@@ -79,6 +79,17 @@ def distinct_vars(rng: np.random.Generator, lens: np.ndarray, var_count: int) ->
         if grp.size == 0:
             continue
         L = int(lens[grp[0]])
+        idx = off[grp][:, None] + np.arange(L)[None, :]
+        if L * L > 2 * var_count:
+            # redraws would rarely succeed: take the first L of a random
+            # permutation of the variables, per row
+            rows = max(1, (1 << 24) // var_count)
+            m = np.empty((grp.size, L), dtype=np.int64)
+            for r0 in range(0, grp.size, rows):
+                keys = rng.random((min(rows, grp.size - r0), var_count))
+                m[r0:r0 + rows] = np.argpartition(keys, L - 1, axis=1)[:, :L]
+            out[idx] = m
+            continue
         m = rng.integers(0, var_count, size=(grp.size, L), dtype=np.int64)
         bad = np.arange(grp.size)
         for _ in range(10_000):
@@ -90,7 +101,6 @@ def distinct_vars(rng: np.random.Generator, lens: np.ndarray, var_count: int) ->
             m[bad] = rng.integers(0, var_count, size=(bad.size, L), dtype=np.int64)
         else:
             raise RuntimeError("could not make clause variables distinct")
-        idx = off[grp][:, None] + np.arange(L)[None, :]
         out[idx] = m
     return out
 

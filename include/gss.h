@@ -71,6 +71,30 @@ typedef struct gss_var_update {
     uint64_t is_true;    /* slots where it is true (ignored where !is_def) */
 } gss_var_update;
 
+/* Slots in mask take (is_def, is_true); the word's other slots are kept. */
+typedef struct gss_var_patch {
+    uint32_t var;
+    uint32_t word;
+    uint64_t mask;
+    uint64_t is_def;
+    uint64_t is_true;
+} gss_var_patch;
+
+/* Variable var of solver base `base` becomes (is_def, is_true). */
+typedef struct gss_base_update {
+    uint32_t var;
+    uint16_t base;
+    uint8_t is_def;
+    uint8_t is_true;
+} gss_base_update;
+
+/* Every slot of table word `word` takes base `base`'s value, for every
+ * variable. */
+typedef struct gss_word_load {
+    uint16_t word;
+    uint16_t base;
+} gss_word_load;
+
 /* Last error message of the calling thread ("" if none). */
 const char *gss_last_error(void);
 
@@ -106,6 +130,21 @@ int gss_db_totals(gss_engine *e, int64_t *clauses, int64_t *literals);
 int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out);
 /* Remove every clause. */
 int gss_db_clear(gss_engine *e);
+/* Tagged DB: every row carries a caller uint32 tag (a stable clause id) that
+ * moves with it.  Enable on an empty DB; then append with
+ * gss_db_append_tagged (gss_db_set_bucket and clause-carrying steps are
+ * refused).  Tags stand in for the reference's GpuClauseId / ClMetadata
+ * association (Clauses.cu:318-356, Clauses.cuh:120-160). */
+int gss_db_enable_tags(gss_engine *e);
+int gss_db_append_tagged(gss_engine *e, int64_t n, const uint32_t *lits,
+                         const int32_t *lens, const uint32_t *tags, uint32_t *out_rows);
+/* Copy the tags of rows [row0, row0+n) of bucket len to host. */
+int gss_db_read_tags(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out);
+/* Keep the rows whose tag t has keep[t] != 0 (host array of ntags flags; tags
+ * >= ntags are kept), in order, compacting every bucket on the device in one
+ * scan.  Replaces the host-side compaction and full re-upload of
+ * HostClauses::tryRemoveClausesAsync (Clauses.cu:249-282). */
+int gss_db_compact(gss_engine *e, const uint8_t *keep, int64_t ntags);
 
 /* ---- assignment table ------------------------------------------------------
  * (Re)shape the table to var_count variables x words words; every slot of
@@ -159,8 +198,10 @@ int gss_set_tuning(gss_engine *e, int key, int value);
 #define GSS_TUNE_MAX_HIT_CHUNKS 3 /* cap on the hit buffer a sweep starts with
                                      (0 = sized to the worst case); a sweep
                                      that needs more re-runs with a larger one */
-#define GSS_TUNE_UNROLL 4         /* table rows in flight per lane at 16 words:
-                                     1, 2, 4 or 8 (0 = default 4) */
+#define GSS_TUNE_UNROLL 4         /* table rows in flight per lane: 1, 2, 4 or 8
+                                     at 16 words (default 4), 2 or 4 at 32
+                                     (default 4), 1, 2 or 4 at 64 (default 2);
+                                     fixed below 16 words */
 #define GSS_TUNE_FUSE_LEN 5       /* clauses of at most this many literals read
                                      both rows of a literal in one pass
                                      (0 = default 4) */
@@ -189,6 +230,37 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
 int gss_step_enqueue(gss_engine *e, int64_t n, const uint32_t *lits, const int32_t *lens,
                      uint32_t *out_rows, int64_t n_ups, const gss_var_update *ups,
                      const uint64_t *valid, const int8_t *last_pos);
+/* The general step.  In order, on the device: append the clauses, apply the
+ * base updates, load the listed words from their bases, apply the full
+ * updates, apply the patches, sweep, roll forward, stage the hit counts.
+ * Serving uses the base planes: a request's words are loaded from its
+ * solver's base and its assignments are patches over it, so no host-side
+ * table state is needed per request.  At most one update per (var, word)
+ * among ups, one patch per (var, word), one base update per (base, var) and
+ * one load per word in a step. */
+typedef struct gss_step_desc {
+    int64_t n_clauses;
+    const uint32_t *lits;
+    const int32_t *lens;
+    uint32_t *out_rows;                /* may be NULL */
+    int64_t n_ups;
+    const gss_var_update *ups;
+    int64_t n_patches;
+    const gss_var_patch *patches;
+    int64_t n_base_ups;
+    const gss_base_update *base_ups;
+    int64_t n_loads;
+    const gss_word_load *loads;
+    const uint64_t *valid;             /* [words] */
+    const int8_t *last_pos;            /* [words] or NULL */
+} gss_step_desc;
+int gss_step_submit(gss_engine *e, const gss_step_desc *d);
+/* (Re)creates nbases solver base planes over the current variable count,
+ * every variable undefined (2 bits per variable per base).  A table that
+ * grows past a multiple of 64 variables needs a new reset before base
+ * updates or loads.  The reference keeps one MultiLBool array per solver
+ * (Assigs.cuh:39-58); here a base is the solver's last assignment, packed. */
+int gss_bases_reset(gss_engine *e, int nbases);
 /* Wait for the oldest step in flight: its number of hit records and the time
  * (ms) between events recorded around its graph launch. */
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);

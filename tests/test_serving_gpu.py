@@ -1,0 +1,180 @@
+"""Serving steps (gss_step_submit with base planes, word loads and patches).
+
+* Base updates, word loads and patches leave the table exactly as the
+  explicit per-slot assignments say (read back with gss_assig_download).
+* A request's records are the same alone and in a mix (other requests
+  around it, at another word offset), and equal the oracle's on the
+  request's explicit assignments (its solver's base plus its patches).
+* A short run of the serving loop completes every request.
+"""
+import numpy as np
+import pytest
+
+import _oracle as orc
+from gpusharesat_amd import serving as sv
+from gpusharesat_amd.engine import ClauseMap
+
+pytestmark = pytest.mark.gpu
+
+ALL = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def random_bases(rng, nb, V, p=0.7):
+    d = rng.random((nb, V)) < p
+    t = d & (rng.random((nb, V)) < 0.5)
+    return d, t
+
+
+def base_updates(d, t):
+    ups = []
+    for b in range(d.shape[0]):
+        u = np.zeros(d.shape[1], sv.BASE_UP_DTYPE)
+        u["var"] = np.arange(d.shape[1])
+        u["base"] = b
+        u["is_def"] = d[b]
+        u["is_true"] = t[b]
+        ups.append(u)
+    return np.concatenate(ups)
+
+
+def explicit(bd, bt, patches, nwords):
+    """[V, nwords] planes of the request: base broadcast, then patches."""
+    V = bd.size
+    d = np.where(bd[:, None], ALL, np.uint64(0)) * np.ones((1, nwords), np.uint64)
+    t = np.where(bt[:, None], ALL, np.uint64(0)) * np.ones((1, nwords), np.uint64)
+    for p in patches:
+        v, w, m = int(p["var"]), int(p["word"]), p["mask"]
+        d[v, w] = (d[v, w] & ~m) | (p["is_def"] & m)
+        t[v, w] = (t[v, w] & ~m) | (p["is_true"] & p["is_def"] & m)
+    return d, t & d
+
+
+def request_valid(k):
+    nw = (k + 63) // 64
+    v = np.full(nw, ALL)
+    last = k - 64 * (nw - 1)
+    if last < 64:
+        v[-1] = np.uint64((1 << last) - 1)
+    return v
+
+
+def test_loads_and_patches_set_the_table(engine_factory):
+    rng = np.random.default_rng(1)
+    V, W = 500, 8
+    e = engine_factory()
+    e.reset_assignments(V, W)
+    sv.bases_reset(e, 3)
+    bd, bt = random_bases(rng, 3, V)
+    sv.submit(e, np.zeros(W, np.uint64), base_ups=base_updates(bd, bt))
+    e.step_wait()
+    p = sv.slot_patches(rng, V, 128, 5, 0.6)  # relative words 0, 1
+    p = p[p["word"] < 2]
+    loads = np.zeros(2, sv.LOAD_DTYPE)
+    loads["word"] = [2, 5]
+    loads["base"] = [1, 0]
+    pw = p.copy()
+    pw["word"] = np.where(p["word"] == 0, 2, 5)
+    sv.submit(e, np.zeros(W, np.uint64), loads=loads, patches=pw)
+    e.step_wait()
+    d, t = e.download()
+    d2, t2 = explicit(bd[1], bt[1], p[p["word"] == 0], 1)
+    d5, t5 = explicit(bd[0], bt[0], p[p["word"] == 1], 2)
+    assert np.array_equal(d[:, 2], d2[:, 0]) and np.array_equal(t[:, 2], t2[:, 0])
+    assert np.array_equal(d[:, 5], d5[:, 1]) and np.array_equal(t[:, 5], t5[:, 1])
+    for w in (0, 1, 3, 4, 6, 7):  # never loaded: still undefined
+        assert not d[:, w].any() and not t[:, w].any()
+    # a later drift of base 1 reaches word 2 only when it is loaded again
+    u = np.zeros(3, sv.BASE_UP_DTYPE)
+    u["var"], u["base"], u["is_def"], u["is_true"] = [0, 1, 2], 1, [1, 0, 1], [1, 0, 0]
+    sv.submit(e, np.zeros(W, np.uint64), base_ups=u)
+    e.step_wait()
+    assert np.array_equal(e.download()[0][:, 2], d2[:, 0])
+    bd[1, :3], bt[1, :3] = [True, False, True], [True, False, False]
+    ld = np.zeros(1, sv.LOAD_DTYPE)
+    ld["word"], ld["base"] = 2, 1
+    sv.submit(e, np.zeros(W, np.uint64), loads=ld)
+    e.step_wait()
+    d, t = e.download()
+    assert np.array_equal(d[:, 2], np.where(bd[1], ALL, np.uint64(0)))
+    assert np.array_equal(t[:, 2], np.where(bt[1], ALL, np.uint64(0)))
+
+
+def test_request_records_alone_equal_in_mix_and_oracle(engine_factory):
+    rng = np.random.default_rng(2)
+    V, W, N = 400, 16, 3000
+    e = engine_factory()
+    lens = rng.integers(1, 13, size=N).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    rows = e.append(lits, lens)
+    e.reset_assignments(V, W)
+    sv.bases_reset(e, 3)
+    bd, bt = random_bases(rng, 3, V)
+    sv.submit(e, np.zeros(W, np.uint64), base_ups=base_updates(bd, bt))
+    e.step_wait()
+
+    def req(solver, k, w0):
+        p = sv.slot_patches(rng, V, k, 6, 0.6)
+        nw = (k + 63) // 64
+        last = k - 64 * (nw - 1)
+        p = p[p["word"] < nw]
+        if last < 64:
+            tail = p["word"] == nw - 1
+            p["mask"][tail] &= np.uint64((1 << last) - 1)
+            p = p[p["mask"] != 0]
+        return dict(solver=solver, k=k, w0=w0, patches=p)
+
+    def step(reqs):
+        valid = np.zeros(W, np.uint64)
+        loads, patches = [], []
+        for r in reqs:
+            nw = (r["k"] + 63) // 64
+            ld = np.zeros(nw, sv.LOAD_DTYPE)
+            ld["word"] = np.arange(r["w0"], r["w0"] + nw)
+            ld["base"] = r["solver"]
+            loads.append(ld)
+            p = r["patches"].copy()
+            p["word"] += r["w0"]
+            patches.append(p)
+            valid[r["w0"]:r["w0"] + nw] = request_valid(r["k"])
+        sv.submit(e, valid, loads=np.concatenate(loads), patches=np.concatenate(patches))
+        e.step_wait()
+        return e.step_hits()
+
+    R = req(1, 200, 0)
+    alone = step([R])
+    X = req(0, 100, 0)
+    Y = req(2, 64, 6)
+    R2 = dict(R, w0=2)
+    mix = step([X, R2, Y])
+    part = mix[(mix["word"] >= 2) & (mix["word"] < 6)].copy()
+    part["word"] -= 2
+    key = ["len", "row", "word"]
+    assert part.size == alone.size > 0
+    a, b = np.sort(alone, order=key), np.sort(part, order=key)
+    assert np.array_equal(a, b)
+    # the oracle on R's explicit assignments
+    d, t = explicit(bd[1], bt[1], R["patches"], 4)
+    off = np.zeros(N + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    exp = orc.sweep(off, lits, d, t, request_valid(200))
+    ids = ClauseMap(lens, rows).ids(alone)
+    got = np.empty(alone.size, orc.HIT_DTYPE)
+    got["clause"], got["word"] = ids, alone["word"]
+    got["false_mask"], got["unit_mask"] = alone["false_mask"], alone["unit_mask"]
+    got = got[np.lexsort((got["word"], got["clause"]))]
+    assert got.size == exp.size
+    for f in ("clause", "word", "false_mask", "unit_mask"):
+        assert np.array_equal(got[f], exp[f])
+
+
+def test_short_serving_run_completes_every_request():
+    cfg = sv.ServeConfig(var_count=20_000, solvers=4, rate=300.0, duration=2.0,
+                         clause_rate=20_000.0, pool=4, clause_pool=50_000, drift=16, flips=8)
+    srv = sv.Server(cfg, log=print)
+    try:
+        out = srv.run()
+    finally:
+        srv.close()
+    print(out.requests, out.steps, np.percentile(out.latency_ms, [50, 99]), flush=True)
+    assert out.requests > 300 and out.steps > 0 and out.checks > 0
+    assert (out.latency_ms > 0).all()

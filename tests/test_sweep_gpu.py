@@ -192,11 +192,12 @@ def test_tuning_never_changes_results(engine_factory):
         assert_same(gpu_records(e, inst.lens, rows), ref)
 
 
-@pytest.mark.parametrize("unroll", [1, 2, 8])
+@pytest.mark.parametrize("words,unroll", [(16, 1), (16, 2), (16, 8), (32, 2), (32, 4),
+                                          (64, 1), (64, 2), (64, 4)])
 @pytest.mark.parametrize("fuse", [1, 3, 1000])
-def test_parity_unroll_and_fuse_at_16_words(engine_factory, unroll, fuse):
-    lits, lens, d, t, valid = random_case(300 + unroll * 7 + fuse, 400, 4000, 30, 16, p_def=0.9,
-                                          p_true=0.25)
+def test_parity_unroll_and_fuse(engine_factory, words, unroll, fuse):
+    lits, lens, d, t, valid = random_case(300 + unroll * 7 + fuse + words, 400, 4000, 30, words,
+                                          p_def=0.9, p_true=0.25)
     e = engine_factory()
     e.set_tuning(TUNE_UNROLL, unroll)
     e.set_tuning(TUNE_FUSE_LEN, fuse)
