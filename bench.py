@@ -107,6 +107,7 @@ def main() -> int:
     ap.add_argument("--workload", default="sweep-1024",
                     choices=["sweep-1024", "batched-16", "serving-32", "residency"])
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
     ap.add_argument("--no-autotune", action="store_true", help="batched-16: default settings")
     args = ap.parse_args()
@@ -328,7 +329,7 @@ def run_batched(args) -> int:
 
 def run_serving(args) -> int:
     from gpusharesat_amd import serving as sv
-    cfg = sv.ServeConfig(duration=args.duration)
+    cfg = sv.ServeConfig(duration=args.duration, rate=args.rate)
     log(f"serving-32: {cfg}")
     srv = sv.Server(cfg, log=log)
     try:
@@ -345,10 +346,11 @@ def run_serving(args) -> int:
         "steps": out.steps, "warmup": 0, "ms_per_step": out.elapsed * 1e3 / max(1, out.steps),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": "serving-32: 32 solvers over 2M vars, Poisson 2000 requests/s of "
-                               "U[64,1024] assignments (30-70% assigned), 200k clauses/s of "
-                               "len 2+Geom(0.1) cap 200, 120 GB budget",
+        "config": {"workload": f"serving-32: 32 solvers over 2M vars, Poisson {cfg.rate:g} "
+                               "requests/s of U[64,1024] assignments (30-70% assigned), 200k "
+                               "clauses/s of len 2+Geom(0.1) cap 200, 120 GB budget",
                    "duration_s": cfg.duration, "requests": out.requests,
+                   "served_per_s": out.requests / out.elapsed,
                    "clauses_end": out.clauses_end, "literals_end": out.literals_end,
                    "records": out.records, "max_queue": out.max_queue},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],

@@ -16,8 +16,13 @@ Workload (serving-32): 32 solvers over 2M variables, each assigning a
 fraction U[0.3, 0.7] of them; requests arrive as a Poisson process at 2000
 per second, each of U[64, 1024] assignments that differ from the solver's
 base in `flips` variables per slot; clauses arrive at 200k per second with
-lengths 2 + Geometric(0.1) capped at 200 and are admitted while the DB fits
-a 120 GB budget.
+lengths 2 + Geometric(0.1) capped at 200, distinct variables, and are
+admitted while the DB fits a 120 GB budget.  Assignments and clauses are
+drawn around a planted total assignment sigma (an assigned variable takes
+sigma's value with probability `agree`; a clause literal is true under sigma
+with probability `lit_true`): uniformly random clauses against uniformly
+random assignments would make about a quarter of all (clause, 64-slot word)
+pairs a record, hundreds of MB per step, which no solver population sends.
 """
 from __future__ import annotations
 
@@ -93,14 +98,19 @@ def bases_reset(e: Engine, n: int) -> None:
     _lib.call("gss_bases_reset", e.handle, n)
 
 
-def slot_patches(rng, var_count: int, k: int, flips: int, p_assigned: float) -> np.ndarray:
+def slot_patches(rng, var_count: int, k: int, flips: int, p_assigned: float,
+                 sigma: np.ndarray | None = None, agree: float = 0.5) -> np.ndarray:
     """Patches (relative words 0 .. ceil(k/64)-1) of k assignments that each
     redraw `flips` variables of the base: assigned with probability
-    p_assigned, polarity Bernoulli(0.5)."""
+    p_assigned, with the planted value sigma[v] with probability `agree`
+    (polarity Bernoulli(0.5) without sigma)."""
     slots = np.repeat(np.arange(k, dtype=np.int64), flips)
     vs = rng.integers(0, var_count, size=slots.size, dtype=np.int64)
     nd = rng.random(slots.size) < p_assigned
-    nt = nd & (rng.random(slots.size) < 0.5)
+    if sigma is None:
+        nt = nd & (rng.random(slots.size) < 0.5)
+    else:
+        nt = nd & ((rng.random(slots.size) < agree) == sigma[vs])
     key = (vs * 16 + slots // 64) * 64 + slots % 64
     uk, ridx = np.unique(key[::-1], return_index=True)  # a repeated draw keeps the last
     sel = slots.size - 1 - ridx
@@ -130,6 +140,8 @@ class ServeConfig:
     hbm_budget: float = 120e9       # bytes of clause literals admitted
     drift: int = 256                # base variables redrawn between two requests of a solver
     flips: int = 32                 # variables redrawn per assignment
+    agree: float = 0.99             # an assigned variable takes the planted value
+    lit_true: float = 0.9           # a clause literal is true under the planted assignment
     pool: int = 48                  # distinct request payloads per size
     clause_pool: int = 2_000_000
     seed: int = 5
@@ -178,18 +190,28 @@ class Server:
         self.e.reset_assignments(cfg.var_count, WORDS)
         bases_reset(self.e, cfg.solvers)
         self.p_assigned = rng.uniform(cfg.p_min, cfg.p_max, size=cfg.solvers)
+        self.sigma = rng.random(cfg.var_count) < 0.5  # the planted assignment
         # payload pools: per solver-independent sets of 1024 slots; a request
         # of k assignments takes the patches of its first ceil(k/64) words and
         # the slots below k
-        self.pool = [slot_patches(rng, cfg.var_count, 1024, cfg.flips, 0.5)
-                     for _ in range(cfg.pool)]
-        self.log(f"payload pool: {sum(p.size for p in self.pool)} patches")
+        # kept as uint64 [n, 4] rows sorted by word (first column = var |
+        # word << 32), with the first row of every word, so a request takes a
+        # prefix and shifts its words with one add
+        self.pool, self.pool_off = [], []
+        for _ in range(cfg.pool):
+            pt = slot_patches(rng, cfg.var_count, 1024, cfg.flips, 0.5, self.sigma, cfg.agree)
+            pt = pt[np.argsort(pt["word"], kind="stable")]
+            self.pool.append(pt.view(np.uint64).reshape(-1, 4).copy())
+            self.pool_off.append(np.searchsorted(pt["word"], np.arange(17)))
+        self.log(f"payload pool: {sum(p.shape[0] for p in self.pool)} patches")
         lens = workloads.clause_lengths(rng, cfg.clause_pool, 0.1, 200)
         self.cl_lens = lens
         self.cl_off = np.zeros(lens.size + 1, np.int64)
         np.cumsum(lens, out=self.cl_off[1:])
-        v = rng.integers(0, cfg.var_count, size=int(self.cl_off[-1]), dtype=np.int64)
-        self.cl_lits = ((v << 1) | rng.integers(0, 2, size=v.size)).astype(np.uint32)
+        v = workloads.distinct_vars(rng, lens, cfg.var_count)
+        st = rng.random(v.size) < cfg.lit_true
+        # literal 2v + 1 is the negation: true under sigma iff sigma(v) is false
+        self.cl_lits = ((v << 1) | (st != self.sigma[v])).astype(np.uint32)
         self.cl_next = 0
 
     def initial_bases(self) -> None:
@@ -202,7 +224,7 @@ class Server:
             u["var"] = vs
             u["base"] = s
             u["is_def"] = 1
-            u["is_true"] = rng.integers(0, 2, size=vs.size)
+            u["is_true"] = (rng.random(vs.size) < c.agree) == self.sigma[vs]
             submit(self.e, np.zeros(WORDS, np.uint64), base_ups=u)
             self.e.step_wait()
 
@@ -225,11 +247,11 @@ class Server:
     def _drift(self, s: int) -> np.ndarray:
         c, rng = self.cfg, self.rng
         u = np.zeros(c.drift, BASE_UP_DTYPE)
-        u["var"] = rng.choice(c.var_count, size=c.drift, replace=False)
+        u["var"] = rng.integers(0, c.var_count, size=c.drift)  # repeats removed in build()
         u["base"] = s
         d = rng.random(c.drift) < self.p_assigned[s]
         u["is_def"] = d
-        u["is_true"] = d & (rng.random(c.drift) < 0.5)
+        u["is_true"] = d & ((rng.random(c.drift) < c.agree) == self.sigma[u["var"]])
         return u
 
     def build(self, reqs: list[Request], n_clauses: int):
@@ -245,23 +267,33 @@ class Server:
             ld["word"] = np.arange(w0, w0 + nw)
             ld["base"] = r.solver
             loads.append(ld)
-            p = self.pool[r.payload]
-            p = p[p["word"] < nw].copy()
+            pool, off = self.pool[r.payload], self.pool_off[r.payload]
             last = r.k - 64 * (nw - 1)  # slots used in the request's last word
-            if last < 64:
-                lm = np.uint64((1 << last) - 1)
-                tail = p["word"] == nw - 1
-                p["mask"][tail] &= lm
-                p = p[p["mask"] != 0]
-            p["word"] += w0
-            patches.append(p)
+            full_end = off[nw - 1] if last < 64 else off[nw]
+            patches.append((pool[:full_end], w0))
+            if last < 64:  # the last word keeps only the request's slots
+                t = pool[off[nw - 1]:off[nw]].copy()
+                t[:, 1] &= np.uint64((1 << last) - 1)
+                patches.append((t[t[:, 1] != 0], w0))
             valid[w0:w0 + nw - 1] = ONES
             valid[w0 + nw - 1] = ONES if last == 64 else np.uint64((1 << last) - 1)
             w0 += nw
         lits, lens = self._clauses(n_clauses) if n_clauses else (None, None)
         cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
-        return dict(valid=valid, lits=lits, lens=lens, patches=cat(patches, PATCH_DTYPE),
-                    base_ups=cat(bups, BASE_UP_DTYPE), loads=cat(loads, LOAD_DTYPE))
+        bu = cat(bups, BASE_UP_DTYPE)
+        if bu.size:  # one update per (base, var) in a step: the last one wins
+            key = (bu["base"].astype(np.int64) << 32) | bu["var"]
+            _, last = np.unique(key[::-1], return_index=True)
+            bu = bu[bu.size - 1 - last]
+        # one pass: copy every request's rows into place and shift its words
+        out = np.empty((sum(a.shape[0] for a, _ in patches), 4), np.uint64)
+        at = 0
+        for a, w0 in patches:
+            np.add(a, np.array([w0 << 32, 0, 0, 0], np.uint64), out=out[at:at + a.shape[0]])
+            at += a.shape[0]
+        pt = out.view(PATCH_DTYPE).ravel()
+        return dict(valid=valid, lits=lits, lens=lens, patches=pt, base_ups=bu,
+                    loads=cat(loads, LOAD_DTYPE))
 
     def run(self) -> ServeResult:
         c, e = self.cfg, self.e
