@@ -1,0 +1,90 @@
+"""The reference's own Glucose GPU solver, built from its sources against this
+library (oracle/solver.mk -> oracle/_ref/glucose-gpu), solves instances with
+the MI355X engine behind GpuClauseSharer: satisfiable answers come with a
+model that satisfies every clause, unsatisfiable ones are pigeonhole
+formulas, and the run reports clauses through the GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SOLVER = os.path.join(ROOT, "oracle", "_ref", "glucose-gpu")
+
+needs_solver = pytest.mark.skipif(
+    not os.path.exists(SOLVER),
+    reason="oracle/_ref/glucose-gpu is built from the reference sources (oracle/solver.mk), "
+           "which are not present here")
+
+
+def write_cnf(path, nvars, clauses):
+    with open(path, "w") as f:
+        f.write(f"p cnf {nvars} {len(clauses)}\n")
+        for c in clauses:
+            f.write(" ".join(str(x) for x in c) + " 0\n")
+
+
+def planted_3sat(rng, n, ratio):
+    sol = rng.integers(0, 2, size=n + 1).astype(bool)
+    out = []
+    while len(out) < int(ratio * n):
+        vs = rng.choice(np.arange(1, n + 1), size=3, replace=False)
+        lits = [int(v) if rng.random() < 0.5 else -int(v) for v in vs]
+        if any((l > 0) == sol[abs(l)] for l in lits):
+            out.append(lits)
+    return out
+
+
+def pigeonhole(p, h):
+    var = lambda i, j: i * h + j + 1
+    cl = [[var(i, j) for j in range(h)] for i in range(p)]
+    for j in range(h):
+        for a in range(p):
+            for b in range(a + 1, p):
+                cl.append([-var(a, j), -var(b, j)])
+    return p * h, cl
+
+
+def solve(path, threads=4):
+    r = subprocess.run([SOLVER, "-model", "-verb=1", f"-thread-count={threads}",
+                        "-write-stats-period-sec=-1", path],
+                       capture_output=True, text=True, timeout=240)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+@needs_solver
+def test_reference_solver_sat_instances_with_model(tmp_path):
+    rng = np.random.default_rng(4)
+    for n, ratio in ((150, 4.0), (250, 4.2), (600, 4.2)):
+        cls = planted_3sat(rng, n, ratio)
+        path = str(tmp_path / f"planted_{n}.cnf")
+        write_cnf(path, n, cls)
+        r = solve(path)
+        assert r.returncode == 10, r.returncode
+        assert "s SATISFIABLE" in r.stdout
+        model = set()
+        for line in r.stdout.splitlines():
+            if line.startswith("v "):
+                model |= {int(x) for x in line[2:].split() if x != "0"}
+        for c in cls:
+            assert any(l in model for l in c), c
+
+
+@needs_solver
+def test_reference_solver_unsat_pigeonhole(tmp_path):
+    n, cls = pigeonhole(8, 7)
+    path = str(tmp_path / "php_8_7.cnf")
+    write_cnf(path, n, cls)
+    r = solve(path)
+    assert r.returncode == 20, r.returncode
+    assert "s UNSATISFIABLE" in r.stdout
+    # the solver threads shared clauses through the GPU and got reports back
+    stat = lambda name: sum(int(x) for x in re.findall(rf'"{name}":(\d+)', r.stdout))
+    assert stat("gpuRuns") > 0 and stat("gpuClausesAdded") > 0, r.stdout[-2000:]
+    assert stat("reportedClauses") > 0
+    assert stat("clauseTestsOnAssigs") > 0
