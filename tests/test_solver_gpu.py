@@ -1,5 +1,6 @@
-"""The reference's own Glucose GPU solver, built from its sources against this
-library (oracle/solver.mk -> oracle/_ref/glucose-gpu), solves instances with
+"""The reference's own GPU solvers (Glucose GPU and rel-newtech GPU), built
+from their sources against this library (oracle/solver.mk -> oracle/_ref/),
+solve instances with
 the MI355X engine behind GpuClauseSharer: satisfiable answers come with a
 model that satisfies every clause, unsatisfiable ones are pigeonhole
 formulas, and the run reports clauses through the GPU."""
@@ -13,12 +14,15 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SOLVER = os.path.join(ROOT, "oracle", "_ref", "glucose-gpu")
+SOLVERS = {name: os.path.join(ROOT, "oracle", "_ref", name)
+           for name in ("glucose-gpu", "rel-newtech-gpu")}
 
-needs_solver = pytest.mark.skipif(
-    not os.path.exists(SOLVER),
-    reason="oracle/_ref/glucose-gpu is built from the reference sources (oracle/solver.mk), "
-           "which are not present here")
+
+def needs(name):
+    return pytest.mark.skipif(
+        not os.path.exists(SOLVERS[name]),
+        reason=f"oracle/_ref/{name} is built from the reference sources (oracle/solver.mk), "
+               "which are not present here")
 
 
 def write_cnf(path, nvars, clauses):
@@ -49,22 +53,22 @@ def pigeonhole(p, h):
     return p * h, cl
 
 
-def solve(path, threads=4):
-    r = subprocess.run([SOLVER, "-model", "-verb=1", f"-thread-count={threads}",
+def solve(path, solver="glucose-gpu", threads=4):
+    r = subprocess.run([SOLVERS[solver], "-model", "-verb=1", f"-thread-count={threads}",
                         "-write-stats-period-sec=-1", path],
                        capture_output=True, text=True, timeout=240)
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     return r
 
 
-@needs_solver
-def test_reference_solver_sat_instances_with_model(tmp_path):
+@pytest.mark.parametrize("solver", [pytest.param(n, marks=needs(n)) for n in SOLVERS])
+def test_reference_solver_sat_instances_with_model(tmp_path, solver):
     rng = np.random.default_rng(4)
     for n, ratio in ((150, 4.0), (250, 4.2), (600, 4.2)):
         cls = planted_3sat(rng, n, ratio)
         path = str(tmp_path / f"planted_{n}.cnf")
         write_cnf(path, n, cls)
-        r = solve(path)
+        r = solve(path, solver)
         assert r.returncode == 10, r.returncode
         assert "s SATISFIABLE" in r.stdout
         model = set()
@@ -75,14 +79,16 @@ def test_reference_solver_sat_instances_with_model(tmp_path):
             assert any(l in model for l in c), c
 
 
-@needs_solver
-def test_reference_solver_unsat_pigeonhole(tmp_path):
+@pytest.mark.parametrize("solver", [pytest.param(n, marks=needs(n)) for n in SOLVERS])
+def test_reference_solver_unsat_pigeonhole(tmp_path, solver):
     n, cls = pigeonhole(8, 7)
     path = str(tmp_path / "php_8_7.cnf")
     write_cnf(path, n, cls)
-    r = solve(path)
+    r = solve(path, solver)
     assert r.returncode == 20, r.returncode
     assert "s UNSATISFIABLE" in r.stdout
+    if solver != "glucose-gpu":
+        return  # only Glucose GPU prints its sharing statistics at exit
     # the solver threads shared clauses through the GPU and got reports back
     stat = lambda name: sum(int(x) for x in re.findall(rf'"{name}":(\d+)', r.stdout))
     assert stat("gpuRuns") > 0 and stat("gpuClausesAdded") > 0, r.stdout[-2000:]
