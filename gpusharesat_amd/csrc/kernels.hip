@@ -157,7 +157,7 @@ __device__ __forceinline__ uint32_t neg_bit() {
 template <int W, int S, int U, int K, int ST, bool ROWS_F, bool ROWS_T>
 __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, uint32_t w8,
                                           const uint64_t *__restrict__ table,
-                                          ClauseState<K> &cs) {
+                                          ClauseState<K> &cs, bool live) {
     constexpr int LW = W / K;  // lanes across one row
     constexpr int G = LW * S;
     Words<K> f[U], t[U];
@@ -174,8 +174,19 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, u
         } else {
             off = scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u)) | w8;
         }
-        if (ROWS_F) f[u] = load_words<K>(table, off);
-        if (ROWS_T) t[u] = load_words<K>(table, off ^ neg_bit<W>());
+        // A lane whose slots are all settled skips its reads (wide rows:
+        // lines of settled lanes are not fetched); all-ones rows leave its
+        // state's report masks unchanged.
+        if (live) {
+            if (ROWS_F) f[u] = load_words<K>(table, off);
+            if (ROWS_T) t[u] = load_words<K>(table, off ^ neg_bit<W>());
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (ROWS_F) f[u].v[k] = ~0ull;
+                if (ROWS_T) t[u].v[k] = ~0ull;
+            }
+        }
     }
     if constexpr (S == 1) {
 #pragma unroll
@@ -220,19 +231,19 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, u
 
 // Runs the steps of one preloaded chunk while they start before the clause
 // end and `more(cs)` holds.  Returns false if it stopped early.
-template <int W, int S, int U, int K, bool ROWS_F, bool ROWS_T, typename More>
+template <int W, int S, int U, int K, bool ROWS_F, bool ROWS_T, typename More, typename Live>
 __device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub, uint32_t w8,
                                             int base, int len,
                                             const uint64_t *__restrict__ table,
-                                            ClauseState<K> &cs, More &&more) {
+                                            ClauseState<K> &cs, More &&more, Live &&live) {
     constexpr int G = (W / K) * S;
     constexpr int LPS = S * U;
     bool go = true;
     [&]<int... I>(std::integer_sequence<int, I...>) {
         ((go = go && (base + I * LPS < len) &&
                [&] {
-                   step_rows<W, S, U, K, I * LPS, ROWS_F, ROWS_T>(litreg, gbase, sub, w8, table,
-                                                                 cs);
+                   step_rows<W, S, U, K, I * LPS, ROWS_F, ROWS_T>(
+                       litreg, gbase, sub, w8, table, cs, K > 1 ? live(cs) : true);
                    return more(cs);
                }()),
          ...);
@@ -261,19 +272,21 @@ __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, in
         cs.nt[k] = ~0ull;
     }
     const bool fused = len <= fuse_len;
-    auto alive = [&](const ClauseState<K> &c) {
+    auto lane_alive = [&](const ClauseState<K> &c) {
         uint64_t x = 0;
 #pragma unroll
         for (int k = 0; k < K; k++) x |= c.nt[k] & ~c.c2[k];
-        return group_any<G>(x != 0, gmask);
+        return x != 0;
     };
+    auto alive = [&](const ClauseState<K> &c) { return group_any<G>(lane_alive(c), gmask); };
     uint32_t litreg = first_chunk;
     for (int base = 0; base < len; base += G) {
         if (base) litreg = base + q < len ? cl[base + q] : sentinel;
-        const bool go = fused ? chunk_steps<W, S, U, K, true, true>(litreg, gbase, sub, w8, base,
-                                                                    len, table, cs, alive)
-                              : chunk_steps<W, S, U, K, true, false>(litreg, gbase, sub, w8,
-                                                                     base, len, table, cs, alive);
+        const bool go =
+            fused ? chunk_steps<W, S, U, K, true, true>(litreg, gbase, sub, w8, base, len, table,
+                                                       cs, alive, lane_alive)
+                  : chunk_steps<W, S, U, K, true, false>(litreg, gbase, sub, w8, base, len, table,
+                                                        cs, alive, lane_alive);
         if (!go) break;  // dead, or the clause ended inside this chunk
     }
     uint64_t cand[K], any = 0;
@@ -284,17 +297,18 @@ __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, in
         any |= cand[k];
     }
     if (!fused && group_any<G>(any != 0, gmask)) {
-        auto open = [&](const ClauseState<K> &c) {
+        auto lane_open = [&](const ClauseState<K> &c) {
             uint64_t x = 0;
 #pragma unroll
             for (int k = 0; k < K; k++) x |= cand[k] & c.nt[k];
-            return group_any<G>(x != 0, gmask);
+            return x != 0;
         };
+        auto open = [&](const ClauseState<K> &c) { return group_any<G>(lane_open(c), gmask); };
         litreg = first_chunk;
         for (int base = 0; base < len; base += G) {
             if (base) litreg = base + q < len ? cl[base + q] : sentinel;
             if (!chunk_steps<W, S, U, K, false, true>(litreg, gbase, sub, w8, base, len, table,
-                                                      cs, open))
+                                                      cs, open, lane_open))
                 break;
         }
     }
