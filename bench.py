@@ -109,7 +109,8 @@ def main() -> int:
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
-    ap.add_argument("--no-autotune", action="store_true", help="batched-16: default settings")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched-16, serving-32: default tile settings")
     args = ap.parse_args()
     batched = args.workload == "batched-16"
     if args.steps is None:
@@ -333,6 +334,7 @@ def run_serving(args) -> int:
     log(f"serving-32: {cfg}")
     srv = sv.Server(cfg, log=log)
     try:
+        tuned = None if args.no_autotune else srv.tune()
         out = srv.run()
     finally:
         srv.close()
@@ -352,7 +354,7 @@ def run_serving(args) -> int:
                    "duration_s": cfg.duration, "requests": out.requests,
                    "served_per_s": out.requests / out.elapsed,
                    "clauses_end": out.clauses_end, "literals_end": out.literals_end,
-                   "records": out.records, "max_queue": out.max_queue},
+                   "records": out.records, "max_queue": out.max_queue, "tuned": tuned},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
                        "max": float(lat.max())},
         "graph_ms": {"mean": float(np.mean(out.step_ms)), "max": float(np.max(out.step_ms))},

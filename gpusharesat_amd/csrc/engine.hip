@@ -116,11 +116,27 @@ struct DevBuf {
         cap = nc;
         return GSS_OK;
     }
+    // Stream-ordered growth (contents not kept): no host synchronisation.
+    int reserve_async(size_t n, hipStream_t s) {
+        if (n <= cap) return GSS_OK;
+        size_t nc = std::max(n, cap * 2);
+        T *np = nullptr;
+        hipError_t e = hipMallocAsync((void **)&np, nc * sizeof(T), s);
+        if (e != hipSuccess)
+            return set_err(GSS_E_NOMEM, "device allocation of %zu bytes failed: %s",
+                           nc * sizeof(T), hipGetErrorString(e));
+        if (p) HIP_TRY(hipFreeAsync(p, s));
+        p = np;
+        cap = nc;
+        async = true;
+        return GSS_OK;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)(async ? hipFreeAsync(p, nullptr) : hipFree(p));
         p = nullptr;
         cap = 0;
     }
+    bool async = false;
 };
 
 // One of the two ping-pong slots of batched steps: pinned staging the host
@@ -273,11 +289,14 @@ int ensure_bucket_cap(gss_engine *e, int len, int64_t need) {
     Bucket &b = e->buckets[len];
     if (need <= b.cap) return GSS_OK;
     int64_t nc = std::max<int64_t>(std::max<int64_t>(need, b.cap * 2), 256);
+    // Stream-ordered: the new array is allocated, filled and the old one
+    // released in the engine stream's order, after every queued sweep or step
+    // that still reads the old one, so growth never stalls the host.
     uint32_t *np = nullptr, *nt = nullptr;
-    hipError_t he = hipMalloc((void **)&np, (size_t)nc * len * sizeof(uint32_t));
+    hipError_t he = hipMallocAsync((void **)&np, (size_t)nc * len * sizeof(uint32_t), e->stream);
     if (he == hipSuccess && e->tagged) {
-        he = hipMalloc((void **)&nt, (size_t)nc * sizeof(uint32_t));
-        if (he != hipSuccess) (void)hipFree(np);
+        he = hipMallocAsync((void **)&nt, (size_t)nc * sizeof(uint32_t), e->stream);
+        if (he != hipSuccess) (void)hipFreeAsync(np, e->stream);
     }
     if (he != hipSuccess)
         return set_err(GSS_E_NOMEM, "bucket %d: device allocation of %lld rows failed: %s", len,
@@ -290,10 +309,9 @@ int ensure_bucket_cap(gss_engine *e, int len, int64_t need) {
                 HIP_TRY(hipMemcpyAsync(nt, b.tags, (size_t)b.count * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, e->stream));
         }
-        HIP_TRY(hipStreamSynchronize(e->stream));
-        (void)hipFree(b.d);
+        HIP_TRY(hipFreeAsync(b.d, e->stream));
     }
-    if (b.tags) (void)hipFree(b.tags);
+    if (b.tags) HIP_TRY(hipFreeAsync(b.tags, e->stream));
     b.d = np;
     b.tags = nt;
     b.cap = nc;
@@ -363,6 +381,14 @@ int gss_engine_create(int device, gss_engine **out) {
         return fail(set_err(GSS_E_HIP, "stream creation failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "event creation failed"));
+    {
+        // memory released in stream order stays in the device pool for the
+        // next growth instead of going back to the driver
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = UINT64_MAX;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool)
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     if (hipMalloc((void **)&e->d_valid, GSS_MAX_WORDS * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc((void **)&e->d_alloc, 64) != hipSuccess ||
         hipMalloc((void **)&e->d_lastpos, GSS_MAX_WORDS) != hipSuccess)
@@ -400,9 +426,10 @@ int gss_engine_destroy(gss_engine *e) {
         sl.d_cseg.release();
     }
     for (Bucket &b : e->buckets) {
-        if (b.d) (void)hipFree(b.d);
-        if (b.tags) (void)hipFree(b.tags);
+        if (b.d) (void)hipFreeAsync(b.d, e->stream);
+        if (b.tags) (void)hipFreeAsync(b.tags, e->stream);
     }
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     e->table.release();
     e->bases.release();
     e->d_desc.release();
@@ -629,13 +656,13 @@ int gss::db_compact_device(gss_engine *e, const uint8_t *d_keep, int64_t ntags) 
         const int64_t k = e->h_kept.p[i];
         caps[i] = std::max<int64_t>(k + k / 4, 256);
         uint32_t *dl = nullptr, *dt = nullptr;
-        hipError_t he = hipMalloc((void **)&dl, (size_t)caps[i] * lens[i] * 4);
-        if (he == hipSuccess) he = hipMalloc((void **)&dt, (size_t)caps[i] * 4);
+        hipError_t he = hipMallocAsync((void **)&dl, (size_t)caps[i] * lens[i] * 4, e->stream);
+        if (he == hipSuccess) he = hipMallocAsync((void **)&dt, (size_t)caps[i] * 4, e->stream);
         if (he != hipSuccess) {
-            if (dl) (void)hipFree(dl);
+            if (dl) (void)hipFreeAsync(dl, e->stream);
             for (int j = 0; j < i; j++) {
-                (void)hipFree(refs[j].dst_lits);
-                (void)hipFree(refs[j].dst_tags);
+                (void)hipFreeAsync(refs[j].dst_lits, e->stream);
+                (void)hipFreeAsync(refs[j].dst_tags, e->stream);
             }
             return set_err(GSS_E_NOMEM, "compaction: allocation failed");
         }
@@ -649,8 +676,8 @@ int gss::db_compact_device(gss_engine *e, const uint8_t *d_keep, int64_t ntags) 
     e->rows_version++;
     for (int i = 0; i < nb; i++) {
         Bucket &b = e->buckets[lens[i]];
-        (void)hipFree(b.d);
-        (void)hipFree(b.tags);
+        (void)hipFreeAsync(b.d, e->stream);
+        (void)hipFreeAsync(b.tags, e->stream);
         const int64_t k = e->h_kept.p[i];
         e->total_clauses -= b.count - k;
         e->total_lits -= (b.count - k) * lens[i];
@@ -1424,8 +1451,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     for (int len = 1; len <= maxlen; len++)
         if (per[len] && e->buckets[len].count + per[len] > e->buckets[len].cap) grow = true;
     if (grow) {
-        // bucket arrays move: steps in flight must be done with them
-        HIP_TRY(hipStreamSynchronize(e->stream));
+        // bucket arrays move in stream order (ensure_bucket_cap): steps in
+        // flight finish with the old arrays before they are released
         for (int len = 1; len <= maxlen; len++)
             if (per[len] && (rc = ensure_bucket_cap(e, len, e->buckets[len].count + per[len])))
                 return rc;
@@ -1466,9 +1493,10 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         (rc = sl.d_loads.reserve(sl.h_loads.cap)))
         return rc;
     if (chunks > sl.chunk_cap) {
-        chunks += chunks / 4;  // headroom for the DB growing over the next steps
-        if ((rc = sl.d_hits.reserve((size_t)chunks * kHitChunk)) ||
-            (rc = sl.d_fills.reserve((size_t)chunks)) || (rc = sl.h_out.reserve(chunks + 1)))
+        chunks *= 2;  // headroom for the DB growing over the next steps
+        if ((rc = sl.d_hits.reserve_async((size_t)chunks * kHitChunk, e->stream)) ||
+            (rc = sl.d_fills.reserve_async((size_t)chunks, e->stream)) ||
+            (rc = sl.h_out.reserve(chunks + 1)))
             return rc;
         sl.chunk_cap = (uint32_t)std::min<size_t>(
             std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / kHitChunk), sl.h_out.cap - 1);

@@ -43,6 +43,7 @@ def round_words(words: int) -> int:
 
 class Engine:
     def __init__(self, device: int = 0):
+        self.device = device
         self._h = ctypes.c_void_p()
         _lib.call("gss_engine_create", device, ctypes.byref(self._h))
         self.words = 1

@@ -295,6 +295,38 @@ class Server:
         return dict(valid=valid, lits=lits, lens=lens, patches=pt, base_ups=bu,
                     loads=cat(loads, LOAD_DTYPE))
 
+    def tune(self, clauses: int = 2_000_000) -> dict:
+        """Chooses the sweep tiles on a scratch engine holding `clauses` pooled
+        clauses and a 64-word table of the solvers' bases, and applies them
+        to the serving engine (whose DB starts empty)."""
+        from .engine import (TUNE_FUSE_LEN, TUNE_SCHEDULE, TUNE_TASK_CLAUSES, TUNE_UNROLL,
+                             TUNE_WAVES_PER_CU)
+        c = self.cfg
+        t = Engine(self.e.device)
+        try:
+            n = min(clauses, c.clause_pool)
+            t.append(self.cl_lits[:self.cl_off[n]], self.cl_lens[:n])
+            t.reset_assignments(c.var_count, WORDS)
+            bases_reset(t, 1)
+            u = np.zeros(c.var_count, BASE_UP_DTYPE)
+            u["var"] = np.arange(c.var_count)
+            d = self.rng.random(c.var_count) < 0.5
+            u["is_def"] = d
+            u["is_true"] = d & ((self.rng.random(c.var_count) < c.agree) == self.sigma)
+            loads = np.zeros(WORDS, LOAD_DTYPE)
+            loads["word"] = np.arange(WORDS)
+            submit(t, np.zeros(WORDS, np.uint64), base_ups=u, loads=loads)
+            t.step_wait()
+            best, ms = t.autotune(np.full(WORDS, ONES))
+        finally:
+            t.close()
+        for key, name in ((TUNE_UNROLL, "unroll"), (TUNE_FUSE_LEN, "fuse_len"),
+                          (TUNE_TASK_CLAUSES, "task_clauses"),
+                          (TUNE_WAVES_PER_CU, "waves_per_cu"), (TUNE_SCHEDULE, "schedule")):
+            self.e.set_tuning(key, best[name])
+        self.log(f"tuned on {n} clauses: {best} ({ms:.3f} ms per sweep)")
+        return best
+
     def run(self) -> ServeResult:
         c, e = self.cfg, self.e
         self.initial_bases()
