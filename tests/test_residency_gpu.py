@@ -160,10 +160,42 @@ def test_activity_policy_and_kept_db_parity(engine_factory):
         assert np.array_equal(got[fld], exp[fld])
 
 
-def test_training_learns_and_learned_reduction_keeps_short_clauses(engine_factory):
+def test_online_training_learns(engine_factory):
     cfg = rs.SimConfig(var_count=20_000, hot_vars=600, arrivals=240_000, rounds=40,
                        budget=60_000, redraw_every=15, seed=21)
     out = rs.simulate(cfg, rs.POLICY_LEARNED, log=print)
     print(out, out.hits_per_gb, flush=True)
     assert out.reductions > 0 and out.hit_pairs > 0
     assert 0.0 < out.train_loss < 0.69  # below the loss of a constant predictor
+
+
+def test_learned_reduction_keeps_short_clauses_and_half_of_the_rest(engine_factory):
+    rng = np.random.default_rng(13)
+    V, W, n = 500, 1, 4000
+    e, r = make_res(engine_factory, max_tags=n)
+    e.reset_assignments(V, W)
+    e.upload(workloads.bernoulli_words(rng, V, 64, 0.5), np.zeros((V, W), np.uint64))
+    lens = rng.integers(1, 9, size=n).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    tags = np.arange(n, dtype=np.uint32)
+    r.add(lits, lens, tags, 0)
+    e.sweep(np.full(W, ALL))
+    r.account(0)
+    ev, kept = r.reduce(rs.POLICY_LEARNED, 0)
+    assert not ev[lens <= 2].any()
+    long_ = lens > 2
+    frac = ev[long_].mean()
+    assert 0.25 < frac < 0.75, frac
+    assert kept == n - int(ev.sum()) == e.totals()[0]
+
+
+def test_learned_policy_gains_over_activity_policy():
+    """The trained network against the reference's activity median on the
+    same seeded drifting-hot-set workload (scale 0.05: 100k variables, 3M
+    arrivals, 1M-clause budget, 300 rounds): more hit pairs per GB checked."""
+    cfg = rs.SimConfig(var_count=100_000, arrivals=3_000_000, budget=1_000_000, seed=11)
+    act = rs.simulate(cfg, rs.POLICY_ACTIVITY)
+    lrn = rs.simulate(cfg, rs.POLICY_LEARNED)
+    print(f"activity {act.hits_per_gb:.4e}, learned {lrn.hits_per_gb:.4e} hit pairs/GB", flush=True)
+    assert lrn.reductions > 0 and act.reductions > 0
+    assert lrn.hits_per_gb >= 1.02 * act.hits_per_gb
