@@ -1483,14 +1483,21 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_lits.reserve(std::max<int64_t>(total, 1))) ||
         (rc = sl.h_segs.reserve(std::max(nseg, 1))) ||
         (rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1))) ||
-        (rc = sl.h_desc.reserve(std::max(nb, 1))) || (rc = sl.d_hdr.reserve(1)) ||
+        (rc = sl.h_desc.reserve(std::max(nb, 1))) ||
         (rc = sl.h_patches.reserve(std::max<int64_t>(n_patches, 1))) ||
         (rc = sl.h_bups.reserve(std::max<int64_t>(n_bups, 1))) ||
-        (rc = sl.h_loads.reserve(std::max<int64_t>(n_loads, 1))) ||
-        (rc = sl.d_lits.reserve(sl.h_lits.cap)) || (rc = sl.d_segs.reserve(sl.h_segs.cap)) ||
-        (rc = sl.d_ups.reserve(sl.h_ups.cap)) || (rc = sl.d_desc.reserve(sl.h_desc.cap)) ||
-        (rc = sl.d_patches.reserve(sl.h_patches.cap)) || (rc = sl.d_bups.reserve(sl.h_bups.cap)) ||
-        (rc = sl.d_loads.reserve(sl.h_loads.cap)))
+        (rc = sl.h_loads.reserve(std::max<int64_t>(n_loads, 1))))
+        return rc;
+    // the slot's device copies are read only by its own graph, on the engine
+    // stream: they grow in stream order, without stalling the other slot
+    hipStream_t es = e->stream;
+    if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
+        (rc = sl.d_segs.reserve_async(sl.h_segs.cap, es)) ||
+        (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
+        (rc = sl.d_desc.reserve_async(sl.h_desc.cap, es)) ||
+        (rc = sl.d_patches.reserve_async(sl.h_patches.cap, es)) ||
+        (rc = sl.d_bups.reserve_async(sl.h_bups.cap, es)) ||
+        (rc = sl.d_loads.reserve_async(sl.h_loads.cap, es)))
         return rc;
     if (chunks > sl.chunk_cap) {
         chunks *= 2;  // headroom for the DB growing over the next steps
