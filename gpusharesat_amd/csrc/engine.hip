@@ -230,14 +230,14 @@ struct gss_engine {
     float last_ms = 0.f;
     double total_ms = 0.0;
     int64_t nsweeps = 0;
-    int occ[7][9] = {};  // [log2 words][unroll] resident blocks per CU
+    int occ[7][9][2] = {};  // [log2 words][unroll][fused] resident blocks per CU
 
     // tuning
     int tune_waves_per_cu = 0;
     int tune_task_clauses = 0;
     int tune_max_chunks = 0;
     int tune_unroll = 0;
-    int tune_fuse_len = 0;
+    int tune_fuse_len = 0;  // 0 = default, -1 = no fused schedule
     int tune_schedule = 0;
 
     // staging
@@ -276,6 +276,22 @@ int check_engine(gss_engine *e) {
     if (!e) return set_err(GSS_E_INVAL, "null engine");
     return hipSetDevice(e->device) == hipSuccess ? GSS_OK
                                                  : set_err(GSS_E_HIP, "hipSetDevice failed");
+}
+
+// Fuse length the kernel runs with: <= 0 selects the variant without the
+// fused schedule.
+int fuse_of(const gss_engine *e) {
+    return e->tune_fuse_len > 0 ? e->tune_fuse_len : e->tune_fuse_len < 0 ? 0 : kDefaultFuseLen;
+}
+
+// Resident 256-thread blocks per CU of the sweep variant the engine runs.
+int resident_blocks(gss_engine *e) {
+    int l = 0;
+    while ((1 << l) < e->words) l++;
+    const int ui = std::min(e->tune_unroll, 8);
+    const int fi = fuse_of(e) > 0;
+    if (!e->occ[l][ui][fi]) e->occ[l][ui][fi] = sweep_occupancy(e->words, e->tune_unroll, fi);
+    return e->occ[l][ui][fi];
 }
 
 int log2_exact(int w) {
@@ -336,7 +352,7 @@ int launch_current_sweep(gss_engine *e) {
         a.fills = e->d_fills.p;
         a.alloc = e->d_alloc;
         a.chunk_cap = e->chunk_cap;
-        a.fuse_len = e->tune_fuse_len > 0 ? e->tune_fuse_len : kDefaultFuseLen;
+        a.fuse_len = fuse_of(e);
         HIP_TRY(launch_sweep(a, e->words, e->tune_unroll, e->blocks, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
@@ -1030,10 +1046,7 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
         HIP_TRY(hipMemcpyAsync(e->d_desc.p, e->h_desc.p, nb * sizeof(BucketDesc),
                                hipMemcpyHostToDevice, e->stream));
     // persistent grid
-    const int li = log2_exact(W);
-    const int ui = std::min(e->tune_unroll, 8);
-    if (!e->occ[li][ui]) e->occ[li][ui] = sweep_occupancy(W, e->tune_unroll);
-    int per_cu = e->occ[li][ui];
+    int per_cu = resident_blocks(e);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
     int64_t blocks = (int64_t)e->cu * per_cu;
     const int64_t need_blocks = (tasks + 3) / 4;
@@ -1164,7 +1177,8 @@ int gss_sweep_kernel_stats(gss_engine *e, double *total_ms, int64_t *count, int 
 
 int gss_set_tuning(gss_engine *e, int key, int value) {
     if (!e) return set_err(GSS_E_INVAL, "null engine");
-    if (value < 0) return set_err(GSS_E_INVAL, "negative tuning value");
+    if (value < 0 && !(key == GSS_TUNE_FUSE_LEN && value == -1))
+        return set_err(GSS_E_INVAL, "negative tuning value");
     switch (key) {
         case GSS_TUNE_WAVES_PER_CU: e->tune_waves_per_cu = value; return GSS_OK;
         case GSS_TUNE_TASK_CLAUSES: e->tune_task_clauses = value; return GSS_OK;
@@ -1202,7 +1216,7 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
     std::vector<Cfg> grid;
     // rows in flight per lane: a choice for 16 words and wider, fixed below
     std::vector<int> unrolls = e->words >= 16 ? std::vector<int>{2, 4} : std::vector<int>{0};
-    const int fuses[] = {1, 4, 8};
+    const int fuses[] = {-1, 1, 4, 8};
     const int tasks[] = {0, 32, 128};
     const int wpcs[] = {0, 24};
     for (int u : unrolls)
@@ -1265,10 +1279,7 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
 // to a power of two (so a growing DB re-captures the graph rarely), at most
 // the resident maximum.
 static int step_blocks(gss_engine *e, int64_t tasks) {
-    const int li = log2_exact(e->words);
-    const int ui = std::min(e->tune_unroll, 8);
-    if (!e->occ[li][ui]) e->occ[li][ui] = sweep_occupancy(e->words, e->tune_unroll);
-    int per_cu = e->occ[li][ui];
+    int per_cu = resident_blocks(e);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
     const int64_t full = (int64_t)e->cu * per_cu;
     int64_t b = 1;
@@ -1512,7 +1523,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     if (!sl.t0) HIP_TRY(hipEventCreate(&sl.t0));
     if (!sl.t1) HIP_TRY(hipEventCreate(&sl.t1));
     // ---- graph over exactly these buffers and settings
-    const int fuse = e->tune_fuse_len > 0 ? e->tune_fuse_len : kDefaultFuseLen;
+    const int fuse = fuse_of(e);
     std::vector<const void *> key = {
         sl.h_hdr.p, sl.h_desc.p, sl.h_segs.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
         sl.h_patches.p, sl.h_bups.p, sl.h_loads.p,

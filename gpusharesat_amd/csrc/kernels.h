@@ -33,6 +33,7 @@ struct SweepArgs {
     uint32_t *alloc;         // number of chunks claimed (may exceed chunk_cap)
     uint32_t chunk_cap;
     int fuse_len;            // clauses of at most this many literals use the fused schedule
+                             // (<= 0: none, and the kernel without that path runs)
     uint32_t sentinel;       // stored row offset of the sentinel literal
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
 };
@@ -89,8 +90,9 @@ struct CompactRef {
 // number of rows in flight per lane for 16 words and wider (0 = default).
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
-// Max resident 256-thread blocks per CU for the sweep kernel variant.
-int sweep_occupancy(int words, int unroll);
+// Max resident 256-thread blocks per CU for the sweep kernel variant (with or
+// without the fused schedule).
+int sweep_occupancy(int words, int unroll, bool fused);
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, int shift,

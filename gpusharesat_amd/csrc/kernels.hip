@@ -255,7 +255,7 @@ __device__ __forceinline__ bool chunk_steps(uint32_t litreg, int gbase, int sub,
 // it together (group-uniform control flow).  first_chunk holds literal q of
 // the clause in lane q of the group (the sentinel past the end), already
 // loaded by the caller.
-template <int W, int S, int U, int K>
+template <int W, int S, int U, int K, bool FUSE>
 __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, int len,
                                              uint32_t first_chunk, uint32_t sentinel,
                                              const uint64_t *__restrict__ table, int q,
@@ -271,7 +271,9 @@ __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, in
         cs.c2[k] = ~valid[k];
         cs.nt[k] = ~0ull;
     }
-    const bool fused = len <= fuse_len;
+    // FUSE = false compiles the two-pass schedule only (its registers are
+    // then not sized for the fused path's two rows per literal)
+    const bool fused = FUSE && len <= fuse_len;
     auto lane_alive = [&](const ClauseState<K> &c) {
         uint64_t x = 0;
 #pragma unroll
@@ -326,7 +328,7 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
     return lo;
 }
 
-template <int W, int S, int U>
+template <int W, int S, int U, bool FUSE>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     constexpr int K = W > 16 ? W / 16 : 1;  // table words per lane
     constexpr int LW = W / K;               // lanes across one row
@@ -375,7 +377,8 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
 #pragma unroll
             for (int k = 0; k < K; k++) fm[k] = um[k] = 0;
             if (active) {
-                check_clause<W, S, U, K>(bd.lits + (size_t)row * len, len, chunk0, a.sentinel,
+                check_clause<W, S, U, K, FUSE>(bd.lits + (size_t)row * len, len, chunk0,
+                                               a.sentinel,
                                          a.table, q, gmask, gbase, w8, sub, valid, a.fuse_len,
                                          fm, um);
             }
@@ -631,20 +634,27 @@ __global__ void k_compact_move(const CompactRef *__restrict__ refs, int nb, int6
 
 template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_sweep<W, S, U>), dim3(blocks), dim3(256), 0, s, a);
+    if (a.fuse_len > 0)
+        hipLaunchKernelGGL((k_sweep<W, S, U, true>), dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sweep<W, S, U, false>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <int W, int S, int U>
-int occupancy_t() {
+int occupancy_t(bool fused) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U>, 256, 0) !=
-        hipSuccess)
-        return 1;
+    const hipError_t e =
+        fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, true>, 256, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, false>, 256,
+                                                             0);
+    if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 
-// Kernel variants: (W, S, U).  `unroll` selects U for W >= 16 (0 = default).
+// Kernel variants: (W, S, U), each with and without the fused schedule
+// (fuse_len <= 0 selects the one without).  `unroll` selects U for W >= 16
+// (0 = default).
 #define GSS_SWEEP_VARIANTS(X)                  \
     X(1, 16, 1)                                \
     X(2, 8, 2)                                 \
@@ -694,10 +704,10 @@ hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, h
     return hipErrorInvalidValue;
 }
 
-int sweep_occupancy(int words, int unroll) {
+int sweep_occupancy(int words, int unroll, bool fused) {
     const Variant v = pick(words, unroll);
 #define X(W_, S_, U_) \
-    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>();
+    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>(fused);
     GSS_SWEEP_VARIANTS(X)
 #undef X
     return 1;

@@ -204,7 +204,9 @@ int gss_set_tuning(gss_engine *e, int key, int value);
                                      fixed below 16 words */
 #define GSS_TUNE_FUSE_LEN 5       /* clauses of at most this many literals read
                                      both rows of a literal in one pass
-                                     (0 = default 4) */
+                                     (0 = default 4; -1 = none, which runs the
+                                     kernel built without that pass and its
+                                     registers) */
 #define GSS_TUNE_SCHEDULE 6       /* wave-to-bucket schedule: 0 = tasks of
                                      short clauses first, 1 = long first */
 

@@ -16,7 +16,7 @@ import pytest
 import _oracle as orc
 from gpusharesat_amd import workloads
 from gpusharesat_amd._lib import GssError
-from gpusharesat_amd.engine import ClauseMap, Engine, VAR_UPDATE_DTYPE
+from gpusharesat_amd.engine import ClauseMap, Engine, TUNE_FUSE_LEN, VAR_UPDATE_DTYPE
 
 pytestmark = pytest.mark.gpu
 
@@ -87,9 +87,11 @@ class Plain:
 
 
 def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
-              grow_at=None, eager=True):
+              grow_at=None, eager=True, tuning=()):
     rng = np.random.default_rng(seed)
     a, b = engine_factory(), engine_factory()
+    for key, val in tuning:
+        a.set_tuning(key, val)
     a.reset_assignments(V, W)
     b.reset_assignments(V, W)
     d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
@@ -149,6 +151,13 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
 def test_steps_match_plain_and_oracle(engine_factory, W):
     run_steps(engine_factory, seed=300 + W, V=2000, W=W, nsteps=8, n_new=600, maxlen=30,
               n_ups=900, pipelined=False)
+
+
+@pytest.mark.parametrize("W", [8, 64])
+def test_steps_without_fused_schedule(engine_factory, W):
+    # the captured graph runs the sweep variant built without the fused schedule
+    run_steps(engine_factory, seed=350 + W, V=2000, W=W, nsteps=5, n_new=600, maxlen=30,
+              n_ups=900, pipelined=True, tuning=((TUNE_FUSE_LEN, -1),))
 
 
 @pytest.mark.parametrize("W", [2, 16])

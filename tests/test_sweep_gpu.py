@@ -186,7 +186,7 @@ def test_tuning_never_changes_results(engine_factory):
     ref = gpu_records(e, inst.lens, rows)
     for key, val in ((TUNE_WAVES_PER_CU, 4), (TUNE_TASK_CLAUSES, 4), (TUNE_TASK_CLAUSES, 1000),
                      (TUNE_WAVES_PER_CU, 0), (TUNE_TASK_CLAUSES, 0), (TUNE_FUSE_LEN, 1),
-                     (TUNE_FUSE_LEN, 1000), (TUNE_FUSE_LEN, 0)):
+                     (TUNE_FUSE_LEN, 1000), (TUNE_FUSE_LEN, -1), (TUNE_FUSE_LEN, 0)):
         e.set_tuning(key, val)
         e.sweep(inst.valid)
         assert_same(gpu_records(e, inst.lens, rows), ref)
@@ -194,13 +194,24 @@ def test_tuning_never_changes_results(engine_factory):
 
 @pytest.mark.parametrize("words,unroll", [(16, 1), (16, 2), (16, 8), (32, 2), (32, 4),
                                           (64, 1), (64, 2), (64, 4)])
-@pytest.mark.parametrize("fuse", [1, 3, 1000])
+@pytest.mark.parametrize("fuse", [-1, 1, 3, 1000])
 def test_parity_unroll_and_fuse(engine_factory, words, unroll, fuse):
     lits, lens, d, t, valid = random_case(300 + unroll * 7 + fuse + words, 400, 4000, 30, words,
                                           p_def=0.9, p_true=0.25)
     e = engine_factory()
     e.set_tuning(TUNE_UNROLL, unroll)
     e.set_tuning(TUNE_FUSE_LEN, fuse)
+    got, exp = run_both(e, lits, lens, 400, d, t, valid)
+    assert_same(got, exp)
+
+
+@pytest.mark.parametrize("words", [1, 2, 4, 8])
+def test_parity_without_fused_schedule(engine_factory, words):
+    # fuse length -1 runs the kernel variant built without the fused schedule
+    lits, lens, d, t, valid = random_case(900 + words, 400, 4000, 30, words, p_def=0.9,
+                                          p_true=0.25)
+    e = engine_factory()
+    e.set_tuning(TUNE_FUSE_LEN, -1)
     got, exp = run_both(e, lits, lens, 400, d, t, valid)
     assert_same(got, exp)
 
