@@ -48,7 +48,7 @@ METRIC = ("lit×assig checks/s + %HBM roofline @4M cl×1024 asg; p99 report late
 # factors calibrated on the clause-upload copy kernel of known size); the
 # commands and raw figures are in profiles/k_sweep_sweep1024.md.  None when
 # the instance is not the headline one.
-MEASURED_TRAFFIC_KIB = {("k_sweep<16,1,4>", 1.0): 2 * 2_074_029 + 427_894}
+MEASURED_TRAFFIC_KIB = {("k_sweep<16,1,4,true>", 1.0): 2 * 2_073_866 + 430_496}
 
 
 def log(msg: str) -> None:
@@ -111,7 +111,13 @@ def main() -> int:
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched-16, serving-32: default tile settings")
+    ap.add_argument("--tuning", default=None,
+                    help="batched-16, serving-32: fixed settings instead of the autotuner, "
+                         "e.g. unroll=2,fuse_len=-1,task_clauses=32,waves_per_cu=0,schedule=1")
     args = ap.parse_args()
+    if args.tuning:
+        args.tuning = {k: int(v) for k, v in
+                       (item.split("=", 1) for item in args.tuning.split(","))}
     batched = args.workload == "batched-16"
     if args.steps is None:
         args.steps = 196 if batched else 50
@@ -201,10 +207,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4>", args.scale)] * 1024
-                        if ("k_sweep<16,1,4>", args.scale) in MEASURED_TRAFFIC_KIB else None),
+            "traffic": (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4,true>", args.scale)] * 1024
+                        if ("k_sweep<16,1,4,true>", args.scale) in MEASURED_TRAFFIC_KIB else None),
             "traffic_unit": "bytes per launch",
-            "kernel": "k_sweep<16,1,4>",
+            "kernel": "k_sweep<16,1,4,true>",
             "kernel_ms": k_avg_s * 1e3,
             "algorithmic_bytes": alg_bytes,
         },
@@ -239,7 +245,10 @@ def run_batched(args) -> int:
     e.reset_assignments(gen.V, gen.words)
     e.upload(d0, t0p)
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        e.apply_tuning(args.tuning)
+        tuned = args.tuning
+    elif not args.no_autotune:
         tuned, tms = e.autotune(plan[0].valid)
         log(f"autotuned: {tuned} ({tms:.3f} ms per sweep of the initial DB)")
     # the warmup steps are checked against the plain entry points
@@ -334,7 +343,11 @@ def run_serving(args) -> int:
     log(f"serving-32: {cfg}")
     srv = sv.Server(cfg, log=log)
     try:
-        tuned = None if args.no_autotune else srv.tune()
+        if args.tuning:
+            srv.e.apply_tuning(args.tuning)
+            tuned = args.tuning
+        else:
+            tuned = None if args.no_autotune else srv.tune()
         out = srv.run()
     finally:
         srv.close()

@@ -238,7 +238,7 @@ struct gss_engine {
     int tune_max_chunks = 0;
     int tune_unroll = 0;
     int tune_fuse_len = 0;  // 0 = default, -1 = no fused schedule
-    int tune_schedule = 0;
+    int tune_schedule = 0;  // 0 = default (long clauses first), 1 = long first, 2 = short first
 
     // staging
     Pinned<uint32_t> h_stage;
@@ -1018,9 +1018,9 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     int k = 0;
     const size_t nlen = e->buckets.size();
     for (size_t i = 1; i < nlen; i++) {
-        // schedule 1 numbers the tasks of long clauses first (they are the
-        // most expensive), so the persistent waves finish together
-        const size_t len = e->tune_schedule == 1 ? nlen - i : i;
+        // tasks of long clauses come first (they are the most expensive), so
+        // the persistent waves finish together; schedule 2 reverses that
+        const size_t len = e->tune_schedule == 2 ? i : nlen - i;
         const Bucket &b = e->buckets[len];
         if (!b.count) continue;
         if (b.count > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
@@ -1189,7 +1189,7 @@ int gss_set_tuning(gss_engine *e, int key, int value) {
             return GSS_OK;
         case GSS_TUNE_FUSE_LEN: e->tune_fuse_len = value; return GSS_OK;
         case GSS_TUNE_SCHEDULE:
-            if (value > 1) return set_err(GSS_E_INVAL, "schedule must be 0 or 1");
+            if (value > 2) return set_err(GSS_E_INVAL, "schedule must be 0, 1 or 2");
             e->tune_schedule = value;
             return GSS_OK;
         case GSS_TUNE_MAX_HIT_CHUNKS:
@@ -1223,7 +1223,7 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
         for (int f : fuses)
             for (int t : tasks)
                 for (int w : wpcs)
-                    for (int sc = 0; sc < 2; sc++) grid.push_back({u, f, t, w, sc});
+                    for (int sc = 1; sc <= 2; sc++) grid.push_back({u, f, t, w, sc});
     const Cfg saved{e->tune_unroll, e->tune_fuse_len, e->tune_task_clauses, e->tune_waves_per_cu,
                     e->tune_schedule};
     auto apply = [&](const Cfg &c) {
@@ -1582,7 +1582,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     int64_t t = 0;
     const size_t nb_len = e->buckets.size();
     for (size_t i = 1; i < nb_len; i++) {
-        const size_t len = e->tune_schedule == 1 ? nb_len - i : i;
+        const size_t len = e->tune_schedule == 2 ? i : nb_len - i;
         const Bucket &b = e->buckets[len];
         if (!b.count) continue;
         BucketDesc &d = sl.h_desc.p[k++];

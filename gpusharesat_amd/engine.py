@@ -24,6 +24,10 @@ TUNE_MAX_HIT_CHUNKS = 3
 TUNE_UNROLL = 4
 TUNE_FUSE_LEN = 5
 TUNE_SCHEDULE = 6
+# names of the settings the autotuner chooses, as it reports them
+TUNE_KEYS = {"unroll": TUNE_UNROLL, "fuse_len": TUNE_FUSE_LEN,
+             "task_clauses": TUNE_TASK_CLAUSES, "waves_per_cu": TUNE_WAVES_PER_CU,
+             "schedule": TUNE_SCHEDULE}
 MAX_WORDS = 64
 
 
@@ -230,14 +234,20 @@ class Engine:
     def set_tuning(self, key: int, value: int) -> None:
         _lib.call("gss_set_tuning", self._h, key, value)
 
+    def apply_tuning(self, settings: dict) -> None:
+        """Sets named settings, as autotune() returns them."""
+        for name, value in settings.items():
+            if name not in TUNE_KEYS:
+                raise ValueError(f"unknown tuning setting {name!r}")
+            self.set_tuning(TUNE_KEYS[name], int(value))
+
     def autotune(self, valid, reps: int = 2) -> tuple[dict, float]:
         """Picks the fastest sweep settings for the current DB and table."""
         v = self._valid(valid)
         best = (ctypes.c_int * 5)()
         ms = ctypes.c_float()
         _lib.call("gss_autotune", self._h, _ptr(v), reps, best, ctypes.byref(ms))
-        keys = ("unroll", "fuse_len", "task_clauses", "waves_per_cu", "schedule")
-        return dict(zip(keys, list(best))), ms.value
+        return dict(zip(TUNE_KEYS, list(best))), ms.value
 
     # -- batched steps --------------------------------------------------------
     def step_enqueue(self, lits=None, lens=None, ups=None, valid=0, last_pos=None) -> np.ndarray:

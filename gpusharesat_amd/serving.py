@@ -299,8 +299,6 @@ class Server:
         """Chooses the sweep tiles on a scratch engine holding `clauses` pooled
         clauses and a 64-word table of the solvers' bases, and applies them
         to the serving engine (whose DB starts empty)."""
-        from .engine import (TUNE_FUSE_LEN, TUNE_SCHEDULE, TUNE_TASK_CLAUSES, TUNE_UNROLL,
-                             TUNE_WAVES_PER_CU)
         c = self.cfg
         t = Engine(self.e.device)
         try:
@@ -320,10 +318,7 @@ class Server:
             best, ms = t.autotune(np.full(WORDS, ONES))
         finally:
             t.close()
-        for key, name in ((TUNE_UNROLL, "unroll"), (TUNE_FUSE_LEN, "fuse_len"),
-                          (TUNE_TASK_CLAUSES, "task_clauses"),
-                          (TUNE_WAVES_PER_CU, "waves_per_cu"), (TUNE_SCHEDULE, "schedule")):
-            self.e.set_tuning(key, best[name])
+        self.e.apply_tuning(best)
         self.log(f"tuned on {n} clauses: {best} ({ms:.3f} ms per sweep)")
         return best
 

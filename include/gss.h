@@ -207,8 +207,9 @@ int gss_set_tuning(gss_engine *e, int key, int value);
                                      (0 = default 4; -1 = none, which runs the
                                      kernel built without that pass and its
                                      registers) */
-#define GSS_TUNE_SCHEDULE 6       /* wave-to-bucket schedule: 0 = tasks of
-                                     short clauses first, 1 = long first */
+#define GSS_TUNE_SCHEDULE 6       /* wave-to-bucket schedule: tasks of long
+                                     clauses first (0 = default, 1), or short
+                                     clauses first (2) */
 
 /* Autotuner: times the sweep on the current clause DB and assignment table
  * under a grid of (rows in flight, fuse length, clauses per task, waves per
