@@ -51,19 +51,28 @@
 //
 // Hits.  A clause/word with a report becomes one 24-byte record.  Each wave
 // compacts its records with a ballot + mbcnt prefix and appends them to a
-// private chunk of kHitChunk records claimed with one atomic; a chunk is
-// abandoned (fill recorded) when the next pass does not fit.  Capacity is
+// private chunk of kHitChunk records claimed with one atomic (non-temporal
+// stores: records are never re-read by the sweep); a chunk is abandoned (fill
+// recorded) when the next pass does not fit.  Capacity is
 // sized on the host so that nothing is dropped (the reference drops on
 // overflow, gpuShareLib/Reporter.cuh:46-48); if it ever is exceeded the claim
 // counter still counts and the host re-runs the sweep.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <hipcub/hipcub.hpp>
 #include <utility>
 
 #include "kernels.h"
 
 namespace gss {
+
+// k_sweep writes a record as three 8-byte words: {row, len, word}, false_mask,
+// unit_mask.
+static_assert(sizeof(gss_hit) == 24 && offsetof(gss_hit, len) == 4 &&
+                  offsetof(gss_hit, word) == 6 && offsetof(gss_hit, false_mask) == 8 &&
+                  offsetof(gss_hit, unit_mask) == 16,
+              "gss_hit layout");
 
 namespace {
 
@@ -175,8 +184,9 @@ __device__ __forceinline__ void step_rows(uint32_t litreg, int gbase, int sub, u
             off = scale_offset<W>((uint32_t)__shfl((int)litreg, gbase + ST + sub * U + u)) | w8;
         }
         // A lane whose slots are all settled skips its reads (wide rows:
-        // lines of settled lanes are not fetched); all-ones rows leave its
-        // state's report masks unchanged.
+        // lines of settled lanes are not fetched; at one word per lane a
+        // partial line is fetched whole, so 16-word rows do not skip);
+        // all-ones rows leave its state's report masks unchanged.
         if (live) {
             if (ROWS_F) f[u] = load_words<K>(table, off);
             if (ROWS_T) t[u] = load_words<K>(table, off ^ neg_bit<W>());
@@ -403,7 +413,16 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                         h.word = (uint16_t)(lw * K + k);
                         h.false_mask = fm[k];
                         h.unit_mask = um[k];
-                        a.hits[(size_t)chunk * kHitChunk + fill + prefix_count(bal)] = h;
+                        // Records are streamed out and never read by the kernel:
+                        // non-temporal stores keep them from displacing table
+                        // rows in L2 and the memory-side cache.
+                        uint64_t *dst = (uint64_t *)(a.hits + (size_t)chunk * kHitChunk + fill +
+                                                     prefix_count(bal));
+                        __builtin_nontemporal_store((uint64_t)h.row | ((uint64_t)h.len << 32) |
+                                                        ((uint64_t)h.word << 48),
+                                                    dst);
+                        __builtin_nontemporal_store(h.false_mask, dst + 1);
+                        __builtin_nontemporal_store(h.unit_mask, dst + 2);
                     }
                     fill += n;
                 }
