@@ -14,7 +14,10 @@ before timing starts.  Prints one JSON line.
                  early-exit credit) over the timed wall-clock region
   roofline       algorithmic bytes of the sweep kernel per launch
                  (4 L + 16 W V_used + 24 R) / its average duration from HIP
-                 events recorded on the engine's stream, against 8 TB/s
+                 events recorded on the engine's stream, against 8 TB/s;
+                 beside it the kernel's traffic past the L2 (two PMC passes)
+                 as a rate, and the device's random 128-byte line gather rate
+                 on a table of the same size, measured in the same run
   cpu_baseline   the CPU oracle (bit-sliced rule, all host threads) on a
                  bounded sample of the same instance
   latency_ms     after the timed region: submit -> every hit record of the
@@ -110,7 +113,7 @@ def main() -> int:
     ap.add_argument("--lat-steps", type=int, default=30,
                     help="sweep-1024: sweeps timed for report latency after the timed steps")
     ap.add_argument("--workload", default="sweep-1024",
-                    choices=["sweep-1024", "batched-16", "serving-32", "residency"])
+                    choices=["sweep-1024", "batched-16", "serving-32", "residency", "ceilings"])
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
@@ -140,6 +143,8 @@ def main() -> int:
         return run_serving(args)
     if args.workload == "residency":
         return run_residency(args)
+    if args.workload == "ceilings":
+        return run_ceilings(args)
 
     from gpusharesat_amd import workloads
     from gpusharesat_amd.engine import Engine
@@ -202,6 +207,15 @@ def main() -> int:
                    "what": "submit -> all hit records in pinned host memory"}
         log(f"report latency p50 {latency['p50']:.2f} ms, p99 {latency['p99']:.2f} ms "
             f"({hits * 24 / 1e6:.0f} MB of records per sweep)")
+    # The device's own ceiling for this access pattern: random 128-byte line
+    # gathers (independent, 4 in flight per lane) from a table of the same size.
+    traffic = (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4,true>", args.scale)] * 1024
+               if ("k_sweep<16,1,4,true>", args.scale) in MEASURED_TRAFFIC_KIB else None)
+    table_bytes = (2 * inst.var_count + 2) * W * 8
+    gather = _gather_ceiling((table_bytes + 127) // 128 * 128)
+    log(f"random-line gather ceiling on a {table_bytes / 1e6:.0f} MB table: {gather:.0f} GB/s"
+        + (f"; the sweep's measured traffic runs at {traffic / k_avg_s / 1e9:.0f} GB/s"
+           if traffic else ""))
     cpu = None
     if not args.no_cpu:
         log("timing the CPU baseline (oracle) ...")
@@ -233,9 +247,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4,true>", args.scale)] * 1024
-                        if ("k_sweep<16,1,4,true>", args.scale) in MEASURED_TRAFFIC_KIB else None),
+            "traffic": traffic,
             "traffic_unit": "bytes per launch",
+            "traffic_gbps": traffic / k_avg_s / 1e9 if traffic else None,
+            "gather_ceiling_gbps": gather,
             "kernel": "k_sweep<16,1,4,true>",
             "kernel_ms": k_avg_s * 1e3,
             "algorithmic_bytes": alg_bytes,
@@ -246,6 +261,17 @@ def main() -> int:
     print(json.dumps(out), flush=True)
     e.close()
     return 0
+
+
+def _gather_ceiling(table_bytes: int) -> float:
+    """GB/s of random 128-byte line reads (4 in flight per lane) from a
+    device table of table_bytes (gss_ceiling_gather)."""
+    import ctypes
+    from gpusharesat_amd import _lib
+    gbps, ms = ctypes.c_double(), ctypes.c_double()
+    _lib.call("gss_ceiling_gather", 0, table_bytes, 32 << 20, 4, 5, ctypes.byref(gbps),
+              ctypes.byref(ms))
+    return gbps.value
 
 
 def _sorted_hits(h: np.ndarray) -> np.ndarray:
@@ -399,6 +425,52 @@ def run_serving(args) -> int:
                        "max": float(lat.max())},
         "graph_ms": {"mean": float(np.mean(out.step_ms)), "max": float(np.max(out.step_ms))},
     }), flush=True)
+    return 0
+
+
+def run_ceilings(args) -> int:
+    """Device properties, a plain copy, and random 128-byte line gathers from
+    tables of several sizes (the 16-word sweep's access pattern): what the
+    device reaches, to set the sweep's measured traffic against."""
+    import ctypes
+    from gpusharesat_amd import _lib
+
+    class DeviceInfo(ctypes.Structure):
+        _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 64),
+                    ("cus", ctypes.c_int32), ("clock_khz", ctypes.c_int32),
+                    ("l2_bytes", ctypes.c_int64), ("lds_per_cu", ctypes.c_int64),
+                    ("hbm_bytes", ctypes.c_int64), ("mem_clock_khz", ctypes.c_int32),
+                    ("mem_bus_bits", ctypes.c_int32)]
+
+    info = DeviceInfo()
+    _lib.call("gss_device_query", 0, ctypes.byref(info))
+    dev = {"name": info.name.decode(), "arch": info.arch.decode(), "cus": info.cus,
+           "clock_mhz": info.clock_khz / 1e3, "l2_bytes": info.l2_bytes,
+           "lds_per_cu": info.lds_per_cu, "hbm_bytes": info.hbm_bytes,
+           "mem_clock_mhz": info.mem_clock_khz / 1e3, "mem_bus_bits": info.mem_bus_bits}
+    log(f"device: {dev}")
+    gbps, ms = ctypes.c_double(), ctypes.c_double()
+    copy_bytes = 4 << 30
+    _lib.call("gss_ceiling_copy", 0, copy_bytes, 10, ctypes.byref(gbps), ctypes.byref(ms))
+    copy = {"bytes": copy_bytes, "gbps": gbps.value, "frac": gbps.value / HBM_PEAK_GBS,
+            "ms": ms.value}
+    log(f"copy {copy_bytes >> 20} MiB: {gbps.value:.0f} GB/s read+write "
+        f"({gbps.value / HBM_PEAK_GBS:.2f} of peak)")
+    gathers = []
+    reads = 32 << 20  # 4 GiB of lines per run
+    for table in (2 << 20, 32 << 20, 256 << 20, 1 << 30, 8 << 30):
+        for unroll in (4, 8):
+            _lib.call("gss_ceiling_gather", 0, table, reads, unroll, 5, ctypes.byref(gbps),
+                      ctypes.byref(ms))
+            gathers.append({"table_bytes": table, "unroll": unroll, "gbps": gbps.value,
+                            "frac": gbps.value / HBM_PEAK_GBS, "ms": ms.value})
+            log(f"gather {table >> 20:5d} MiB table, {unroll} lines in flight per lane: "
+                f"{gbps.value:.0f} GB/s of lines")
+    out = {"metric": "device memory ceilings (copy; random 128-byte line gathers)",
+           "value": copy["gbps"], "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic", "config": {"workload": "ceilings", "device": dev},
+           "copy": copy, "gather": gathers}
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -27,8 +27,8 @@ class GssError(RuntimeError):
         self.code = code
 
 
-# Every symbol exported by include/gss.h and include/gss_sharer.h, with its
-# argument types (all return int status unless listed in _RESTYPES).
+# Every symbol exported by the headers under include/ (gss.h, gss_sharer.h,
+# gss_residency.h, gss_ceilings.h), with its argument types (all return int status unless listed in _RESTYPES).
 SIGNATURES: dict[str, list] = {
     "gss_last_error": [],
     "gss_engine_create": [c_int, c_p],
@@ -81,6 +81,10 @@ SIGNATURES: dict[str, list] = {
     "gss_res_set_params": [c_p, c_p],
     "gss_res_grad": [c_p, c_i64, c_p, c_p, c_p, c_p, c_p],
     "gss_res_score": [c_p, c_i64, c_p, c_p],
+    # include/gss_ceilings.h
+    "gss_device_query": [c_int, c_p],
+    "gss_ceiling_copy": [c_int, c_i64, c_int, c_p, c_p],
+    "gss_ceiling_gather": [c_int, c_i64, c_i64, c_int, c_int, c_p, c_p],
     # include/gss_sharer.h
     "gss_sharer_default_options": [c_p],
     "gss_sharer_create": [c_p, c_p, c_p, c_p],

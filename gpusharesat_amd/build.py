@@ -26,6 +26,7 @@ SOURCES = [
     "kernels.hip",
     "engine.hip",
     "residency.hip",
+    "ceilings.hip",
     "host_state.cpp",
     "sharer.cpp",
     "sharer_capi.cpp",
