@@ -1,4 +1,4 @@
-// Device ceilings for the sweep's access patterns: a plain 16-byte copy and
+// Device ceilings for the sweep's access patterns: a streaming 16-byte copy and
 // random 128-byte line gathers (one line per 16-lane group, as the 16-word
 // sweep reads a table row), timed with HIP events on a private stream.
 #include <hip/hip_runtime.h>
@@ -23,11 +23,24 @@ int set_err(int code, const char *fmt, ...) {
     return record_error(code, buf);
 }
 
-__global__ __launch_bounds__(256) void k_ceiling_copy(const uint4 *__restrict__ src,
-                                                      uint4 *__restrict__ dst, uint64_t n) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride)
-        dst[i] = src[i];
+// Four 16-byte loads in flight per thread, then the four stores; streaming
+// (non-temporal) on both sides.  Elements past the last full round of four
+// are copied one by one.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_ceiling_copy(const u32x4 *__restrict__ src,
+                                                      u32x4 *__restrict__ dst, uint64_t n) {
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t full = n / (4 * nthreads) * (4 * nthreads);
+    for (uint64_t base = tid; base < full; base += 4 * nthreads) {
+        u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = __builtin_nontemporal_load(src + base + u * nthreads);
+#pragma unroll
+        for (int u = 0; u < 4; u++) __builtin_nontemporal_store(v[u], dst + base + u * nthreads);
+    }
+    for (uint64_t i = full + tid; i < n; i += nthreads) dst[i] = src[i];
 }
 
 // Uniform line index in [0, nlines) for read r: murmur3's 32-bit finalizer
@@ -163,7 +176,7 @@ int gss_ceiling_copy(int device, int64_t bytes, int reps, double *gbps, double *
     const int blocks = persistent_blocks(device);
     auto launch = [&]() {
         hipLaunchKernelGGL(k_ceiling_copy, dim3(blocks), dim3(256), 0, sc.s,
-                           (const uint4 *)sc.a, (uint4 *)sc.b, n);
+                           (const u32x4 *)sc.a, (u32x4 *)sc.b, n);
         return hipGetLastError();
     };
     if ((rc = time_reps(sc, reps, launch, ms))) return rc;
