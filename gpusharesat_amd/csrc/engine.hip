@@ -217,10 +217,16 @@ struct gss_engine {
     Pinned<BucketDesc> h_desc;
     uint64_t *d_valid = nullptr;
     Pinned<uint64_t> h_valid;
+    // plain sweep: what d_desc / d_valid hold (uploads are skipped when a
+    // sweep's descriptors and slot mask are unchanged), and the pinned copy of
+    // the claim counter and chunk fills that k_stage_out writes after the sweep
+    std::vector<BucketDesc> desc_new, desc_up;
+    const BucketDesc *desc_up_at = nullptr;
+    std::vector<uint64_t> valid_up;
+    Pinned<uint32_t> h_out;  // [0] = chunks claimed, [1 + c] = fill of chunk c
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     uint32_t *d_alloc = nullptr;
-    Pinned<uint32_t> h_small;
     std::vector<uint32_t> fills;
     uint32_t chunk_cap = 0;
     int nbuckets = 0, ntasks = 0, task_clauses = 0, blocks = 0;
@@ -356,6 +362,8 @@ int launch_current_sweep(gss_engine *e) {
         HIP_TRY(launch_sweep(a, e->words, e->tune_unroll, e->blocks, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    // claim counter and fills to pinned memory: the wait reads them there
+    HIP_TRY(launch_stage_out(e->d_alloc, e->d_fills.p, e->chunk_cap, e->h_out.p, e->stream));
     return GSS_OK;
 }
 
@@ -364,7 +372,9 @@ int reserve_chunks(gss_engine *e, uint64_t chunks) {
     int rc;
     if ((rc = e->d_hits.reserve((size_t)chunks * kHitChunk))) return rc;
     if ((rc = e->d_fills.reserve((size_t)chunks))) return rc;
-    e->chunk_cap = (uint32_t)std::min<size_t>(e->d_fills.cap, e->d_hits.cap / kHitChunk);
+    if ((rc = e->h_out.reserve((size_t)chunks + 1))) return rc;
+    e->chunk_cap = (uint32_t)std::min<size_t>(
+        std::min<size_t>(e->d_fills.cap, e->d_hits.cap / kHitChunk), e->h_out.cap - 1);
     return GSS_OK;
 }
 
@@ -409,8 +419,7 @@ int gss_engine_create(int device, gss_engine **out) {
         hipMalloc((void **)&e->d_alloc, 64) != hipSuccess ||
         hipMalloc((void **)&e->d_lastpos, GSS_MAX_WORDS) != hipSuccess)
         return fail(set_err(GSS_E_NOMEM, "device allocation failed"));
-    if ((rc = e->h_valid.reserve(GSS_MAX_WORDS)) || (rc = e->h_small.reserve(16)) ||
-        (rc = e->h_lastpos.reserve(GSS_MAX_WORDS)))
+    if ((rc = e->h_valid.reserve(GSS_MAX_WORDS)) || (rc = e->h_lastpos.reserve(GSS_MAX_WORDS)))
         return fail(rc);
     if ((rc = gss_assig_reset(e, 0, 1))) return fail(rc);
     *out = e;
@@ -1013,9 +1022,8 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
         if (e->buckets[len].count) nb++;
     if ((rc = e->h_desc.reserve(std::max(nb, 1))) || (rc = e->d_desc.reserve(std::max(nb, 1))))
         return rc;
-    HIP_TRY(hipStreamSynchronize(e->stream));  // pinned descriptors / valid reused
     int64_t tasks = 0;
-    int k = 0;
+    e->desc_new.clear();
     const size_t nlen = e->buckets.size();
     for (size_t i = 1; i < nlen; i++) {
         // tasks of long clauses come first (they are the most expensive), so
@@ -1024,27 +1032,44 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
         const Bucket &b = e->buckets[len];
         if (!b.count) continue;
         if (b.count > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
-        BucketDesc &d = e->h_desc.p[k++];
+        BucketDesc d;
         d.lits = b.d;
         d.len = (uint32_t)len;
         d.count = (uint32_t)b.count;
         d.task_begin = (uint32_t)tasks;
         d.pad = 0;
+        e->desc_new.push_back(d);
         tasks += (b.count + tc - 1) / tc;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
     e->nbuckets = nb;
     e->ntasks = (int)tasks;
     int active_words = 0;
-    for (int w = 0; w < W; w++) {
-        e->h_valid.p[w] = valid[w];
+    for (int w = 0; w < W; w++)
         if (valid[w]) active_words++;
-    }
-    HIP_TRY(hipMemcpyAsync(e->d_valid, e->h_valid.p, W * sizeof(uint64_t), hipMemcpyHostToDevice,
-                           e->stream));
-    if (nb)
-        HIP_TRY(hipMemcpyAsync(e->d_desc.p, e->h_desc.p, nb * sizeof(BucketDesc),
+    // Uploads only when what the device holds differs (the previous sweep has
+    // been waited for, so its copies out of the pinned staging are done).
+    const bool valid_same = (int)e->valid_up.size() == W &&
+                            std::memcmp(e->valid_up.data(), valid, W * sizeof(uint64_t)) == 0;
+    const bool desc_same =
+        e->desc_up_at == e->d_desc.p && e->desc_up.size() == e->desc_new.size() &&
+        (nb == 0 ||
+         std::memcmp(e->desc_up.data(), e->desc_new.data(), nb * sizeof(BucketDesc)) == 0);
+    if (!valid_same) {
+        std::memcpy(e->h_valid.p, valid, W * sizeof(uint64_t));
+        HIP_TRY(hipMemcpyAsync(e->d_valid, e->h_valid.p, W * sizeof(uint64_t),
                                hipMemcpyHostToDevice, e->stream));
+        e->valid_up.assign(valid, valid + W);
+    }
+    if (!desc_same) {
+        if (nb) {
+            std::memcpy(e->h_desc.p, e->desc_new.data(), nb * sizeof(BucketDesc));
+            HIP_TRY(hipMemcpyAsync(e->d_desc.p, e->h_desc.p, nb * sizeof(BucketDesc),
+                                   hipMemcpyHostToDevice, e->stream));
+        }
+        e->desc_up = e->desc_new;
+        e->desc_up_at = e->d_desc.p;
+    }
     // persistent grid
     int per_cu = resident_blocks(e);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
@@ -1058,7 +1083,7 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     const uint64_t ub = (uint64_t)e->total_clauses * (uint64_t)active_words;
     uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    if (chunks > e->chunk_cap && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
         // never take more than a quarter of the free memory; beyond that the
         // wait re-runs with exactly what the claims asked for
         const uint64_t budget =
@@ -1080,10 +1105,8 @@ int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
     if (rc) return rc;
     if (!e->pending) return set_err(GSS_E_STATE, "no sweep in flight");
     for (;;) {
-        HIP_TRY(hipMemcpyAsync(e->h_small.p, e->d_alloc, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
-        const uint32_t used = e->h_small.p[0];
+        const uint32_t used = e->h_out.p[0];
         if (used <= e->chunk_cap) {
             e->last_chunks = used;
             break;
@@ -1105,10 +1128,7 @@ int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
     e->last_ms = ms;
     e->total_ms += ms;
     e->nsweeps++;
-    e->fills.resize(e->last_chunks);
-    if (e->last_chunks)
-        HIP_TRY(hipMemcpy(e->fills.data(), e->d_fills.p, e->last_chunks * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost));
+    e->fills.assign(e->h_out.p + 1, e->h_out.p + 1 + e->last_chunks);
     int64_t tot = 0;
     for (uint32_t f : e->fills) tot += f;
     e->last_hits = tot;
