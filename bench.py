@@ -335,23 +335,27 @@ def run_batched(args) -> int:
     pending = []
     e.sync()
     t_start = time.perf_counter()
-    def finish_oldest():
-        nonlocal hits_total
-        _, ms = e.step_wait()
-        hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
-        lat.append(time.perf_counter() - pending.pop(0))
-        step_ms.append(ms)
+    todo = list(range(args.warmup, steps_total))
 
-    for k in range(args.warmup, steps_total):
-        st = plan[k]
+    def enqueue_next():
+        nonlocal lits_db, checks
+        st = plan[todo.pop(0)]
         pending.append(time.perf_counter())
         e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
         lits_db += int(st.lits.size)
         checks += lits_db * st.assignments
-        if len(pending) == 2:  # two steps in flight: collect the older one
-            finish_oldest()
+
+    # two steps in flight; once the older one ends its records travel to the
+    # host while the next step is staged, then they are read
+    while todo and len(pending) < 2:
+        enqueue_next()
     while pending:
-        finish_oldest()
+        _, ms = e.step_wait()
+        if todo:
+            enqueue_next()
+        hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
+        lat.append(time.perf_counter() - pending.pop(0))
+        step_ms.append(ms)
     e.sync()
     elapsed = time.perf_counter() - t_start
     clauses_end = e.totals()[0]

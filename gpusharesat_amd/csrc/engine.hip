@@ -163,7 +163,20 @@ struct StepSlot {
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
+    DevBuf<uint64_t> d_offs;     // record offset of each chunk (the graph's compaction)
+    uint64_t dense_want = 0;     // records the graph's compaction should hold
+    uint64_t graph_dense = 0;    // records the captured graph compacts into d_dense (0 = none)
+    uint64_t step_dense = 0;     // the same, for the last submitted step
     Pinned<gss_hit> h_hits;      // the same records on the host (gss_step_hits_map)
+    // The records of a waited step go to h_hits on the slot's own stream as
+    // soon as gss_step_wait has seen the step end (one DMA copy when the graph
+    // compacted them); the slot's next graph waits for that copy on the
+    // engine stream, so the records stay valid until the slot's next step is
+    // waited and the host can stage that step while they travel.
+    hipStream_t copy = nullptr;
+    hipEvent_t ev_copied = nullptr;
+    bool copy_queued = false;
+    int64_t rec_total = 0;  // records of the slot's last waited step
     bool host_ready = false;     // h_hits holds the last waited step's records
     Pinned<ScatterSeg> h_cseg;   // compaction segments (one per non-empty chunk)
     DevBuf<ScatterSeg> d_cseg;
@@ -188,7 +201,6 @@ struct gss_engine {
     int device = 0;
     int cu = 0;
     hipStream_t stream = nullptr;
-    hipStream_t copy_stream = nullptr;  // step result copies, overlapping the next step
     hipStream_t capture_stream = nullptr;  // records step graphs (never executes work)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -402,8 +414,7 @@ int gss_engine_create(int device, gss_engine **out) {
     };
     if (hipDeviceGetAttribute(&e->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "cannot query CU count"));
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "stream creation failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "event creation failed"));
@@ -430,12 +441,16 @@ int gss_engine_destroy(gss_engine *e) {
     if (!e) return GSS_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->copy_stream) (void)hipStreamSynchronize(e->copy_stream);
     for (StepSlot &sl : e->slots) {
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
         if (sl.t0) (void)hipEventDestroy(sl.t0);
         if (sl.t1) (void)hipEventDestroy(sl.t1);
+        if (sl.ev_copied) (void)hipEventDestroy(sl.ev_copied);
+        if (sl.copy) {
+            (void)hipStreamSynchronize(sl.copy);
+            (void)hipStreamDestroy(sl.copy);
+        }
         if (sl.d_alloc) (void)hipFree(sl.d_alloc);
         sl.d_hdr.release();
         sl.d_desc.release();
@@ -448,6 +463,7 @@ int gss_engine_destroy(gss_engine *e) {
         sl.d_hits.release();
         sl.d_fills.release();
         sl.d_dense.release();
+        sl.d_offs.release();
         sl.d_cseg.release();
     }
     for (Bucket &b : e->buckets) {
@@ -471,7 +487,6 @@ int gss_engine_destroy(gss_engine *e) {
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
-    if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
     if (e->capture_stream) (void)hipStreamDestroy(e->capture_stream);
     delete e;
     return GSS_OK;
@@ -1307,6 +1322,49 @@ static int step_blocks(gss_engine *e, int64_t tasks) {
     return (int)std::min(b, full);
 }
 
+// Queues the copy of a waited step's `total` records (in `used` chunks) to the
+// slot's pinned h_hits on the slot's stream: one DMA copy when the graph made
+// them dense, else a compaction first (and the next capture's dense buffer is
+// sized for this volume).
+static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32_t used) {
+    int rc;
+    if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
+    if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
+    hipStream_t cs = sl.copy;
+    if ((size_t)total > sl.h_hits.cap) {
+        // the previous copy into this slot's pinned buffer must end before
+        // the buffer is replaced
+        if (sl.copy_queued) HIP_TRY(hipEventSynchronize(sl.ev_copied));
+        sl.copy_queued = false;
+        if ((rc = sl.h_hits.reserve((size_t)total))) return rc;
+    }
+    if ((uint64_t)total > sl.step_dense) {
+        if ((rc = sl.d_dense.reserve_async((size_t)total, cs)) ||
+            (rc = sl.h_cseg.reserve(used)) || (rc = sl.d_cseg.reserve(used)))
+            return rc;
+        int ns = 0;
+        uint64_t acc = 0;
+        for (uint32_t c = 0; c < used; c++) {
+            const uint32_t f = sl.h_out.p[1 + c];
+            if (!f) continue;
+            ScatterSeg &sg = sl.h_cseg.p[ns++];
+            sg.dst = (uint32_t *)(sl.d_dense.p + acc);
+            sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
+            sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
+            acc += f;
+        }
+        HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
+                               hipMemcpyHostToDevice, cs));
+        HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
+        sl.dense_want = std::max<uint64_t>(sl.dense_want, 2 * (uint64_t)total);
+    }
+    HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * sizeof(gss_hit),
+                           hipMemcpyDeviceToHost, cs));
+    HIP_TRY(hipEventRecord(sl.ev_copied, cs));
+    sl.copy_queued = true;
+    return GSS_OK;
+}
+
 // Captures the step's fixed launch sequence over this slot's buffers.
 static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
@@ -1365,6 +1423,12 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
                                    cs);
     if (he == hipSuccess)
         he = launch_stage_out(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.h_out.p, cs);
+    // records made dense on the device right after the sweep, so that reading
+    // them on the host is one DMA copy (no compute units taken from the next
+    // step's sweep)
+    if (he == hipSuccess && sl.d_dense.cap > 0)
+        he = launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_offs.p,
+                                    sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap, cs);
     hipGraph_t g = nullptr;
     hipError_t he2 = hipStreamEndCapture(cs, &g);
     if (he != hipSuccess) {
@@ -1520,8 +1584,12 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         (rc = sl.h_loads.reserve(std::max<int64_t>(n_loads, 1))))
         return rc;
     // the slot's device copies are read only by its own graph, on the engine
-    // stream: they grow in stream order, without stalling the other slot
+    // stream: they grow in stream order, without stalling the other slot.
+    // The copy of this slot's previous records (its own stream) still reads
+    // d_hits and d_dense: the engine stream waits for it before anything of
+    // this step, frees of replaced buffers included, runs there.
     hipStream_t es = e->stream;
+    if (sl.copy_queued) HIP_TRY(hipStreamWaitEvent(es, sl.ev_copied, 0));
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
         (rc = sl.d_segs.reserve_async(sl.h_segs.cap, es)) ||
         (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
@@ -1539,6 +1607,9 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         sl.chunk_cap = (uint32_t)std::min<size_t>(
             std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / kHitChunk), sl.h_out.cap - 1);
     }
+    if ((rc = sl.d_offs.reserve_async((size_t)sl.chunk_cap + 1, es))) return rc;
+    if (sl.dense_want > sl.d_dense.cap && (rc = sl.d_dense.reserve_async(sl.dense_want, es)))
+        return rc;
     if (!sl.d_alloc) HIP_TRY(hipMalloc((void **)&sl.d_alloc, 64));
     if (!sl.t0) HIP_TRY(hipEventCreate(&sl.t0));
     if (!sl.t1) HIP_TRY(hipEventCreate(&sl.t1));
@@ -1549,7 +1620,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         sl.h_patches.p, sl.h_bups.p, sl.h_loads.p,
         sl.d_hdr.p, sl.d_desc.p, sl.d_segs.p, sl.d_lits.p, sl.d_ups.p, sl.d_hits.p,
         sl.d_patches.p, sl.d_bups.p, sl.d_loads.p,
-        sl.d_fills.p, e->table.p, e->bases.p, (const void *)(intptr_t)e->bases_vw,
+        sl.d_fills.p, sl.d_offs.p, sl.d_dense.p, (const void *)(intptr_t)sl.d_dense.cap,
+        e->table.p, e->bases.p, (const void *)(intptr_t)e->bases_vw,
         (const void *)(intptr_t)W, (const void *)(intptr_t)e->vars,
         (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
@@ -1558,7 +1630,9 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         sl.key.clear();
         if ((rc = build_step_graph(e, sl, blocks, tc, fuse))) return rc;
         sl.key = key;
+        sl.graph_dense = sl.d_dense.cap;
     }
+    sl.step_dense = sl.graph_dense;
     // ---- stage this step (the slot is idle) and commit it to the DB
     std::vector<int64_t> seg_off(maxlen + 1, 0), seg_fill(maxlen + 1, 0);
     {
@@ -1638,8 +1712,9 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));
     sl.busy = true;
-    sl.results = false;
-    sl.host_ready = false;
+    // the previous records of this slot stay readable only if their copy
+    // was queued (or there were none)
+    if (sl.rec_total > 0 && !sl.copy_queued) sl.results = false;
     e->inflight.push_back(S);
     e->next_slot ^= 1;
     return GSS_OK;
@@ -1662,12 +1737,15 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     for (uint32_t c = 0; c < used; c++) tot += sl.h_out.p[1 + c];
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, sl.t0, sl.t1));
-    sl.results = true;
+    sl.results = false;
     sl.host_ready = false;
+    sl.rec_total = tot;
     e->last_step_slot = S;
     e->last_step_hits = tot;
     e->last_step_chunks = used;
     e->last_step_ms = ms;
+    if (tot > 0 && (rc = queue_records_copy(e, sl, tot, used))) return rc;
+    sl.results = true;
     if (n_hits) *n_hits = tot;
     if (step_ms) *step_ms = ms;
     return GSS_OK;
@@ -1696,32 +1774,7 @@ int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n) {
     StepSlot &sl = e->slots[e->last_step_slot];
     const int64_t total = e->last_step_hits;
     if (n) *n = total;
-    if (!sl.host_ready && total > 0) {
-        const uint32_t nc = e->last_step_chunks;
-        if ((rc = sl.d_dense.reserve((size_t)total)) || (rc = sl.h_cseg.reserve(nc)) ||
-            (rc = sl.d_cseg.reserve(nc)) || (rc = sl.h_hits.reserve((size_t)total)))
-            return rc;
-        int ns = 0;
-        uint64_t acc = 0;
-        for (uint32_t c = 0; c < nc; c++) {
-            const uint32_t f = sl.h_out.p[1 + c];
-            if (!f) continue;
-            ScatterSeg &sg = sl.h_cseg.p[ns++];
-            sg.dst = (uint32_t *)(sl.d_dense.p + acc);
-            sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
-            sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
-            acc += f;
-        }
-        // the step's kernels are done (waited); the next step may be running
-        // on the engine stream, so the compaction and the copy go on their own
-        hipStream_t cs = e->copy_stream;
-        HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
-                               hipMemcpyHostToDevice, cs));
-        HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
-        HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * sizeof(gss_hit),
-                               hipMemcpyDeviceToHost, cs));
-        HIP_TRY(hipStreamSynchronize(cs));
-    }
+    if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
     *out = sl.h_hits.p;
     return GSS_OK;

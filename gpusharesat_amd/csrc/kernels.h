@@ -114,6 +114,13 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
                                  int words, const int8_t *last_pos, hipStream_t s);
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s);
+// Dense copy of a sweep's records (chunk order, the records of a chunk in
+// place): offs = exclusive prefix of the fills of the claimed chunks (cap + 1
+// entries), dense = the records, when they all fit in dense_cap; counts are
+// read on the device.
+hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                  uint64_t *offs, const gss_hit *hits, gss_hit *dense,
+                                  uint64_t dense_cap, hipStream_t s);
 // Solver base planes: bases[(2 b) * vw + i] / bases[(2 b + 1) * vw + i] hold
 // bit (v mod 64) of word i = v / 64 of base b's is_def / is_true.
 hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uint64_t *bases,

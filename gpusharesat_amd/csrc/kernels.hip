@@ -599,6 +599,60 @@ __global__ void k_stage_out(const uint32_t *alloc, const uint32_t *__restrict__ 
     for (uint32_t i = tid; i < n; i += gridDim.x * blockDim.x) h_out[1 + i] = fills[i];
 }
 
+// Exclusive prefix of the chunk fills: offs[c] = records before chunk c for
+// the chunks [0, n) a sweep claimed (n = min(*alloc, cap), read on the device
+// so a graph replay needs no host count), offs[n] = all records.  One block:
+// each thread sums a contiguous run of chunks, the 1024 sums are scanned in
+// LDS, then each thread writes its run's prefixes.
+__global__ __launch_bounds__(1024) void k_chunk_offsets(const uint32_t *alloc,
+                                                        const uint32_t *__restrict__ fills,
+                                                        uint32_t cap, uint64_t *__restrict__ offs) {
+    __shared__ uint64_t part[1024];
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = (uint64_t)t * per < n ? t * per : n;
+    const uint32_t e = b + per < n ? b + per : n;
+    uint64_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += fills[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint64_t acc = t ? part[t - 1] : 0;
+    for (uint32_t i = b; i < e; i++) {
+        offs[i] = acc;
+        acc += fills[i];
+    }
+    if (t == 1023u) offs[n] = part[1023];
+}
+
+// Copies the records of every claimed chunk to its place in a dense array
+// (three 8-byte words per record), or nothing when they would not all fit in
+// dense_cap records (the host then compacts them itself).
+__global__ __launch_bounds__(256) void k_gather_chunks(const uint32_t *alloc,
+                                                       const uint32_t *__restrict__ fills,
+                                                       uint32_t cap,
+                                                       const uint64_t *__restrict__ offs,
+                                                       const uint64_t *__restrict__ hits,
+                                                       uint64_t *__restrict__ dense,
+                                                       uint64_t dense_cap) {
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    if (offs[n] > dense_cap) return;
+    for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const uint32_t words = fills[c] * 3u;
+        const uint64_t *src = hits + (uint64_t)c * kHitChunk * 3u;
+        uint64_t *dst = dense + offs[c] * 3u;
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    }
+}
+
 // ---- tagged DB compaction --------------------------------------------------
 
 __device__ __forceinline__ int find_ref(const CompactRef *__restrict__ r, int nb, int64_t g) {
@@ -809,6 +863,18 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_out, dim3(256), dim3(256), 0, s, alloc, fills, cap, h_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                  uint64_t *offs, const gss_hit *hits, gss_hit *dense,
+                                  uint64_t dense_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, s, alloc, fills, cap, offs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gather_chunks, dim3(2048), dim3(256), 0, s, alloc, fills, cap,
+                       (const uint64_t *)offs, (const uint64_t *)hits, (uint64_t *)dense,
+                       dense_cap);
     return hipGetLastError();
 }
 

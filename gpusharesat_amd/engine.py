@@ -278,7 +278,8 @@ class Engine:
 
     def step_hits_view(self) -> np.ndarray:
         """Records of the last waited step as a view of the engine's pinned
-        memory, valid until that step's slot is enqueued again."""
+        memory, valid until the next step_wait of that step's slot (two
+        waits later; enqueueing that slot again meanwhile is fine)."""
         p, n = ctypes.c_void_p(), ctypes.c_int64()
         _lib.call("gss_step_hits_map", self._h, ctypes.byref(p), ctypes.byref(n))
         if n.value == 0:

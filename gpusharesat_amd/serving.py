@@ -341,35 +341,16 @@ class Server:
         last_cl = 0.0
         last_log = 0.0
 
-        def finish():
-            nonlocal records
-            ids, t_sub = inflight.popleft()
-            _, ms = e.step_wait()
-            h = e.step_hits_view()
-            t = time.perf_counter() - start
-            records += h.size
-            step_ms.append(ms)
-            for i in ids:
-                r = reqs[i]
-                r.t_done = t
-                lat.append(t - r.t_arrive)
-
-        while True:
-            now = time.perf_counter() - start
+        def arrivals(now):
+            nonlocal ai, max_queue
             while ai < len(reqs) and reqs[ai].t_arrive <= now:
                 pending.append(ai)
                 ai += 1
             max_queue = max(max_queue, len(pending))
-            if ai >= len(reqs) and not pending and not inflight:
-                break
-            if len(inflight) == 2 or (inflight and not pending):
-                finish()
-                continue
-            if not pending:
-                nxt = reqs[ai].t_arrive if ai < len(reqs) else now
-                time.sleep(max(0.0, min(nxt - now, 0.0005)))
-                continue
-            # admit requests in arrival order while their words fit
+
+        def admit(now):
+            """Submits one step with the pending requests whose words fit."""
+            nonlocal last_cl, lits_db, checks, steps, last_log
             batch, used = [], 0
             while pending:
                 r = reqs[pending[0]]
@@ -396,6 +377,39 @@ class Server:
                 last_log = now
                 self.log(f"t={now:.1f}s steps {steps} served {len(lat)} queue {len(pending)} "
                          f"DB {e.totals()[0]} clauses")
+
+        def finish():
+            """Waits for the oldest step; while its records travel to the host
+            the next step is admitted, then the records are read."""
+            nonlocal records
+            ids, t_sub = inflight.popleft()
+            _, ms = e.step_wait()
+            now = time.perf_counter() - start
+            arrivals(now)
+            if pending:
+                admit(now)
+            h = e.step_hits_view()
+            t = time.perf_counter() - start
+            records += h.size
+            step_ms.append(ms)
+            for i in ids:
+                r = reqs[i]
+                r.t_done = t
+                lat.append(t - r.t_arrive)
+
+        while True:
+            now = time.perf_counter() - start
+            arrivals(now)
+            if ai >= len(reqs) and not pending and not inflight:
+                break
+            if len(inflight) == 2 or (inflight and not pending):
+                finish()
+                continue
+            if not pending:
+                nxt = reqs[ai].t_arrive if ai < len(reqs) else now
+                time.sleep(max(0.0, min(nxt - now, 0.0005)))
+                continue
+            admit(now)
         elapsed = time.perf_counter() - start
         cl, li = e.totals()
         return ServeResult(len(lat), steps, elapsed, checks, np.array(lat) * 1e3, cl, li,

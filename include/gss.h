@@ -265,16 +265,21 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *d);
  * (Assigs.cuh:39-58); here a base is the solver's last assignment, packed. */
 int gss_bases_reset(gss_engine *e, int nbases);
 /* Wait for the oldest step in flight: its number of hit records and the time
- * (ms) between events recorded around its graph launch. */
+ * (ms) between events recorded around its graph launch.  The step's records
+ * start travelling to pinned host memory at once, on a stream of the slot's
+ * own, while the caller goes on (typically enqueueing the next step); the
+ * slot's next step does not overwrite them on the device before that copy is
+ * done. */
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
 /* Copy the hit records of the last waited step to host (up to cap; *n =
- * total).  Runs on a copy stream of its own, so it overlaps a step that is
- * still in flight.  Valid until that step's slot is enqueued again. */
+ * total), waiting for their transfer if it is still running.  Valid until
+ * the next gss_step_wait of that step's slot (two waits later), also when
+ * the slot has been enqueued again meanwhile. */
 int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
 /* Zero-copy form: *out points at the records of the last waited step in the
- * engine's pinned host memory (*n records), valid until that step's slot is
- * enqueued again (two gss_step_enqueue calls later).  Same source as
- * gss_step_hits_copy; replaces Reporter::getCopiedToHost (Reporter.cuh:103). */
+ * engine's pinned host memory (*n records), valid until the next
+ * gss_step_wait of that step's slot.  Same source as gss_step_hits_copy;
+ * replaces Reporter::getCopiedToHost (Reporter.cuh:103). */
 int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n);
 
 #ifdef __cplusplus

@@ -166,6 +166,52 @@ def test_two_steps_in_flight(engine_factory, W):
               n_ups=700, pipelined=True)
 
 
+@pytest.mark.parametrize("W", [4, 64])
+def test_records_read_after_the_slot_is_enqueued_again(engine_factory, W):
+    # wait(k) -> enqueue(k + 2) into the same slot -> read step k's records:
+    # they travel while the next step is staged and must not be overwritten
+    rng = np.random.default_rng(450 + W)
+    V = 1500
+    a, b = engine_factory(), engine_factory()
+    for e in (a, b):
+        e.reset_assignments(V, W)
+    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    a.upload(d0, t0)
+    b.upload(d0, t0)
+    plain = Plain(b)
+    steps = [make_step(rng, V, W, int(rng.integers(100, 800)), 25, 700) for _ in range(9)]
+    expected, lens_a, rows_a = [], np.zeros(0, np.int32), np.zeros(0, np.uint32)
+    todo = list(range(len(steps)))
+    inflight = []
+
+    def enqueue():
+        nonlocal lens_a, rows_a
+        k = todo.pop(0)
+        lits, lens, ups, valid, last_pos = steps[k]
+        expected.append(plain.step(lits, lens, ups, valid, last_pos))
+        rows = a.step_enqueue(lits, lens, ups, valid, last_pos)
+        lens_a = np.concatenate([lens_a, lens])
+        rows_a = np.concatenate([rows_a, rows])
+        inflight.append(k)
+
+    enqueue()
+    enqueue()
+    while inflight:
+        k = inflight.pop(0)
+        n, _ = a.step_wait()
+        if todo:
+            enqueue()  # same slot as step k
+        print(f"step {k}: {n} records, read after the next enqueue", flush=True)
+        view = a.step_hits_view()
+        assert view.size == n
+        assert_same(to_records(view, lens_a, rows_a), expected[k], f"step {k}")
+        assert_same(to_records(a.step_hits(), lens_a, rows_a), expected[k], f"step {k} copy")
+    da, ta = a.download()
+    db, tb = b.download()
+    assert np.array_equal(da, db) and np.array_equal(ta, tb)
+
+
 def test_table_regrowth_recaptures(engine_factory):
     run_steps(engine_factory, seed=500, V=1200, W=8, nsteps=7, n_new=400, maxlen=12,
               n_ups=500, pipelined=True, grow_at=4)
