@@ -145,7 +145,7 @@ struct DevBuf {
 struct StepSlot {
     Pinned<StepHeader> h_hdr;
     Pinned<BucketDesc> h_desc;
-    Pinned<ScatterSeg> h_segs;
+    Pinned<ClauseSeg> h_cls;
     Pinned<uint32_t> h_lits;
     Pinned<gss_var_update> h_ups;
     Pinned<gss_var_patch> h_patches;
@@ -154,7 +154,7 @@ struct StepSlot {
     Pinned<uint32_t> h_out;  // [0] = chunks claimed, [1 + c] = fill of chunk c
     DevBuf<StepHeader> d_hdr;
     DevBuf<BucketDesc> d_desc;
-    DevBuf<ScatterSeg> d_segs;
+    DevBuf<ClauseSeg> d_cls;
     DevBuf<uint32_t> d_lits;
     DevBuf<gss_var_update> d_ups;
     DevBuf<gss_var_patch> d_patches;
@@ -454,7 +454,7 @@ int gss_engine_destroy(gss_engine *e) {
         if (sl.d_alloc) (void)hipFree(sl.d_alloc);
         sl.d_hdr.release();
         sl.d_desc.release();
-        sl.d_segs.release();
+        sl.d_cls.release();
         sl.d_lits.release();
         sl.d_ups.release();
         sl.d_patches.release();
@@ -1378,7 +1378,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     StageArgs st{};
     st.h[kRegHeader] = (const uint32_t *)sl.h_hdr.p;
     st.h[kRegDesc] = (const uint32_t *)sl.h_desc.p;
-    st.h[kRegSegs] = (const uint32_t *)sl.h_segs.p;
+    st.h[kRegClauses] = (const uint32_t *)sl.h_cls.p;
     st.h[kRegLits] = sl.h_lits.p;
     st.h[kRegUps] = (const uint32_t *)sl.h_ups.p;
     st.h[kRegPatches] = (const uint32_t *)sl.h_patches.p;
@@ -1386,7 +1386,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     st.h[kRegLoads] = (const uint32_t *)sl.h_loads.p;
     st.d[kRegHeader] = (uint32_t *)sl.d_hdr.p;
     st.d[kRegDesc] = (uint32_t *)sl.d_desc.p;
-    st.d[kRegSegs] = (uint32_t *)sl.d_segs.p;
+    st.d[kRegClauses] = (uint32_t *)sl.d_cls.p;
     st.d[kRegLits] = sl.d_lits.p;
     st.d[kRegUps] = (uint32_t *)sl.d_ups.p;
     st.d[kRegPatches] = (uint32_t *)sl.d_patches.p;
@@ -1407,7 +1407,8 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
     hipError_t he = hipSuccess;
     if (he == hipSuccess) he = launch_stage_in(st, cs);
-    if (he == hipSuccess) he = launch_scatter_dyn(sl.d_segs.p, &dh->nsegs, sl.d_lits.p, kLitShift, cs);
+    if (he == hipSuccess)
+        he = launch_scatter_clauses(sl.d_cls.p, &dh->nclauses, sl.d_lits.p, kLitShift, cs);
     if (he == hipSuccess)
         he = launch_base_apply_dyn(sl.d_bups.p, &dh->nbups, e->bases.p, e->bases_vw, cs);
     if (he == hipSuccess)
@@ -1495,11 +1496,17 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     StepSlot &sl = e->slots[S];
     if (sl.busy) return set_err(GSS_E_STATE, "two steps in flight: wait for the oldest first");
     const int W = e->words;
-    // ---- validation (nothing changes before it passes)
-    for (int64_t i = 0; i < n_ups; i++)
-        if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
+    if (n > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many clauses in one step");
+    // ---- validation (nothing changes before it passes; the updates are
+    // checked while they are copied into this idle slot's pinned staging)
+    if ((rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1)))) return rc;
+    for (int64_t i = 0; i < n_ups; i++) {
+        const gss_var_update u = ups[i];
+        if (u.var >= (uint64_t)e->vars || u.word >= (uint32_t)W)
             return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
-                           (long long)i, ups[i].var, ups[i].word);
+                           (long long)i, u.var, u.word);
+        sl.h_ups.p[i] = u;
+    }
     for (int64_t i = 0; i < n_patches; i++)
         if (desc->patches[i].var >= (uint64_t)e->vars || desc->patches[i].word >= (uint32_t)W)
             return set_err(GSS_E_INVAL, "patch %lld out of range", (long long)i);
@@ -1530,8 +1537,15 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         maxlen = std::max(maxlen, (int)lens[i]);
         total += lens[i];
     }
+    if (total > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "too many literals in one step");
+    // literals are staged in the caller's order while their maximum is taken
+    if ((rc = sl.h_lits.reserve(std::max<int64_t>(total, 1)))) return rc;
     uint32_t mx = 0;
-    for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
+    for (int64_t i = 0; i < total; i++) {
+        const uint32_t x = lits[i];
+        mx = std::max(mx, x);
+        sl.h_lits.p[i] = x;
+    }
     if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
     const int64_t need_lit = std::max<int64_t>(n ? (int64_t)mx : -1, e->max_lit);
     if (need_lit >= 2 * e->vars)
@@ -1560,14 +1574,13 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         const int64_t c = len < e->buckets.size() ? e->buckets[len].count : 0;
         return c + (len < per.size() ? per[len] : 0);
     };
-    int nb = 0, nseg = 0;
+    int nb = 0;
     int64_t tasks = 0;
     for (size_t len = 1; len < nlen; len++) {
         const int64_t c = rows_after(len);
         if (c > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
         if (c) nb++;
         tasks += (c + tc - 1) / tc;
-        if (len < per.size() && per[len]) nseg++;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
     const int blocks = step_blocks(e, tasks);
@@ -1575,9 +1588,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     // leaves at most one partial chunk: records <= clauses * active words
     const uint64_t ub = (uint64_t)(e->total_clauses + n) * (uint64_t)active_words;
     uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
-    if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_lits.reserve(std::max<int64_t>(total, 1))) ||
-        (rc = sl.h_segs.reserve(std::max(nseg, 1))) ||
-        (rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1))) ||
+    if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_cls.reserve(std::max<int64_t>(n, 1))) ||
         (rc = sl.h_desc.reserve(std::max(nb, 1))) ||
         (rc = sl.h_patches.reserve(std::max<int64_t>(n_patches, 1))) ||
         (rc = sl.h_bups.reserve(std::max<int64_t>(n_bups, 1))) ||
@@ -1591,7 +1602,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     hipStream_t es = e->stream;
     if (sl.copy_queued) HIP_TRY(hipStreamWaitEvent(es, sl.ev_copied, 0));
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
-        (rc = sl.d_segs.reserve_async(sl.h_segs.cap, es)) ||
+        (rc = sl.d_cls.reserve_async(sl.h_cls.cap, es)) ||
         (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
         (rc = sl.d_desc.reserve_async(sl.h_desc.cap, es)) ||
         (rc = sl.d_patches.reserve_async(sl.h_patches.cap, es)) ||
@@ -1616,9 +1627,9 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     // ---- graph over exactly these buffers and settings
     const int fuse = fuse_of(e);
     std::vector<const void *> key = {
-        sl.h_hdr.p, sl.h_desc.p, sl.h_segs.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
+        sl.h_hdr.p, sl.h_desc.p, sl.h_cls.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
         sl.h_patches.p, sl.h_bups.p, sl.h_loads.p,
-        sl.d_hdr.p, sl.d_desc.p, sl.d_segs.p, sl.d_lits.p, sl.d_ups.p, sl.d_hits.p,
+        sl.d_hdr.p, sl.d_desc.p, sl.d_cls.p, sl.d_lits.p, sl.d_ups.p, sl.d_hits.p,
         sl.d_patches.p, sl.d_bups.p, sl.d_loads.p,
         sl.d_fills.p, sl.d_offs.p, sl.d_dense.p, (const void *)(intptr_t)sl.d_dense.cap,
         e->table.p, e->bases.p, (const void *)(intptr_t)e->bases_vw,
@@ -1633,39 +1644,27 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         sl.graph_dense = sl.d_dense.cap;
     }
     sl.step_dense = sl.graph_dense;
-    // ---- stage this step (the slot is idle) and commit it to the DB
-    std::vector<int64_t> seg_off(maxlen + 1, 0), seg_fill(maxlen + 1, 0);
+    // ---- stage this step (the slot is idle) and commit it to the DB: one
+    // record per clause (the device copies it to its bucket row)
     {
-        int64_t acc = 0;
-        for (int len = 1; len <= maxlen; len++) {
-            seg_off[len] = acc;
-            acc += per[len] * len;
-        }
-        int64_t pos = 0;
+        std::vector<int64_t> fill(maxlen + 1, 0);
+        uint32_t pos = 0;
         for (int64_t i = 0; i < n; i++) {
             const int len = lens[i];
-            const int64_t k = seg_fill[len]++;
-            // raw literals: the scatter shifts them to row offsets on the device
-            std::memcpy(sl.h_lits.p + seg_off[len] + k * len, lits + pos,
-                        len * sizeof(uint32_t));
-            if (out_rows) out_rows[i] = (uint32_t)(e->buckets[len].count + k);
-            pos += len;
+            const Bucket &b = e->buckets[len];
+            const int64_t row = b.count + fill[len]++;
+            ClauseSeg &c = sl.h_cls.p[i];
+            c.dst = b.d + (size_t)row * len;
+            c.src = pos;
+            c.len = (uint32_t)len;
+            if (out_rows) out_rows[i] = (uint32_t)row;
+            pos += (uint32_t)len;
         }
-    }
-    int si = 0;
-    for (int len = 1; len <= maxlen; len++) {
-        if (!per[len]) continue;
-        Bucket &b = e->buckets[len];
-        ScatterSeg &sg = sl.h_segs.p[si++];
-        sg.dst = b.d + (size_t)b.count * len;
-        sg.src = (uint64_t)seg_off[len];
-        sg.count = (uint64_t)per[len] * len;
-        b.count += per[len];
+        for (int len = 1; len <= maxlen; len++) e->buckets[len].count += per[len];
     }
     e->total_clauses += n;
     e->total_lits += total;
     if (n) e->max_lit = std::max<int64_t>(e->max_lit, mx);
-    if (n_ups) std::memcpy(sl.h_ups.p, ups, (size_t)n_ups * sizeof(gss_var_update));
     if (n_patches)
         std::memcpy(sl.h_patches.p, desc->patches, (size_t)n_patches * sizeof(gss_var_patch));
     if (n_bups)
@@ -1691,14 +1690,14 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     std::memset(&h, 0, sizeof h);
     h.nbuckets = k;
     h.ntasks = (int32_t)t;
-    h.nsegs = si;
+    h.nclauses = (int32_t)n;
     h.nups = n_ups;
     h.npatches = n_patches;
     h.nbups = n_bups;
     h.nloads = n_loads;
     h.region_words[kRegHeader] = sizeof(StepHeader) / 4;
     h.region_words[kRegDesc] = (int64_t)k * (sizeof(BucketDesc) / 4);
-    h.region_words[kRegSegs] = (int64_t)si * (sizeof(ScatterSeg) / 4);
+    h.region_words[kRegClauses] = n * (int64_t)(sizeof(ClauseSeg) / 4);
     h.region_words[kRegLits] = total;
     h.region_words[kRegUps] = n_ups * (int64_t)(sizeof(gss_var_update) / 4);
     h.region_words[kRegPatches] = n_patches * (int64_t)(sizeof(gss_var_patch) / 4);

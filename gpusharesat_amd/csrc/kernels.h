@@ -42,7 +42,7 @@ struct SweepArgs {
 enum StepRegion {
     kRegHeader = 0,
     kRegDesc,
-    kRegSegs,
+    kRegClauses,
     kRegLits,
     kRegUps,
     kRegPatches,
@@ -56,7 +56,7 @@ enum StepRegion {
 struct StepHeader {
     int32_t nbuckets;
     int32_t ntasks;
-    int32_t nsegs;
+    int32_t nclauses;
     int32_t pad0;
     int64_t nups, npatches, nbups, nloads;
     int64_t region_words[kStageRegions];  // uint32 words of each region to copy
@@ -73,6 +73,14 @@ struct ScatterSeg {
     uint32_t *dst;
     uint64_t src;    // offset in the staging buffer (uint32 units)
     uint64_t count;  // uint32 elements
+};
+
+// One clause staged by a batched step: its literals at src (uint32 units) of
+// the staged literals, copied to its bucket row at dst.
+struct ClauseSeg {
+    uint32_t *dst;
+    uint32_t src;
+    uint32_t len;
 };
 
 // One non-empty bucket of a tagged DB during compaction.
@@ -106,8 +114,10 @@ hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
 // Fixed-shape launches of a batched step (counts read from the device header,
 // so a captured graph can be replayed with new data).
 hipError_t launch_stage_in(const StageArgs &a, hipStream_t s);
-hipError_t launch_scatter_dyn(const ScatterSeg *segs, const int32_t *nseg, const uint32_t *src,
-                              int shift, hipStream_t s);
+// Copies each staged clause (n read on the device) to its bucket row,
+// shifting literals left by `shift`.
+hipError_t launch_scatter_clauses(const ClauseSeg *cls, const int32_t *n, const uint32_t *src,
+                                  int shift, hipStream_t s);
 hipError_t launch_assig_apply_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
                                   int words, hipStream_t s);
 hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,

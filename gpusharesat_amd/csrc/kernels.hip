@@ -550,13 +550,15 @@ __global__ void k_assig_patch_dyn(const gss_var_patch *__restrict__ p, const int
     }
 }
 
-__global__ void k_scatter_dyn(const ScatterSeg *__restrict__ segs, const int32_t *nseg_p,
-                              const uint32_t *__restrict__ src, int shift) {
-    const int nseg = *nseg_p;
-    for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
-        const ScatterSeg sg = segs[s];
-        for (uint64_t i = threadIdx.x; i < sg.count; i += blockDim.x)
-            sg.dst[i] = src[sg.src + i] << shift;
+// Eight lanes per staged clause: lane j copies literals j, j + 8, ...
+__global__ void k_scatter_clauses(const ClauseSeg *__restrict__ cls, const int32_t *n_p,
+                                  const uint32_t *__restrict__ src, int shift) {
+    const int64_t n = *n_p;
+    const int lane = threadIdx.x & 7;
+    const int64_t ng = ((int64_t)gridDim.x * blockDim.x) >> 3;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 3; i < n; i += ng) {
+        const ClauseSeg c = cls[i];
+        for (uint32_t j = lane; j < c.len; j += 8) c.dst[j] = src[(uint64_t)c.src + j] << shift;
     }
 }
 
@@ -841,9 +843,9 @@ hipError_t launch_assig_patch_dyn(const gss_var_patch *p, const int64_t *n, uint
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_dyn(const ScatterSeg *segs, const int32_t *nseg, const uint32_t *src,
-                              int shift, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter_dyn, dim3(1024), dim3(256), 0, s, segs, nseg, src, shift);
+hipError_t launch_scatter_clauses(const ClauseSeg *cls, const int32_t *n, const uint32_t *src,
+                                  int shift, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_clauses, dim3(1024), dim3(256), 0, s, cls, n, src, shift);
     return hipGetLastError();
 }
 
