@@ -442,15 +442,13 @@ int gss_engine_destroy(gss_engine *e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (StepSlot &sl : e->slots) {
+        if (sl.copy) (void)hipStreamSynchronize(sl.copy);
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
         if (sl.t0) (void)hipEventDestroy(sl.t0);
         if (sl.t1) (void)hipEventDestroy(sl.t1);
         if (sl.ev_copied) (void)hipEventDestroy(sl.ev_copied);
-        if (sl.copy) {
-            (void)hipStreamSynchronize(sl.copy);
-            (void)hipStreamDestroy(sl.copy);
-        }
+        if (sl.copy) (void)hipStreamDestroy(sl.copy);
         if (sl.d_alloc) (void)hipFree(sl.d_alloc);
         sl.d_hdr.release();
         sl.d_desc.release();

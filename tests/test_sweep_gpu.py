@@ -118,6 +118,46 @@ def test_parity_long_clauses(engine_factory):
     assert_same(got, exp)
 
 
+def test_parity_maximum_variable_count(engine_factory):
+    # the largest table: its sentinel rows end at the last 32-bit stored row
+    # offset; clauses use the first and the last variables of the range
+    V = (1 << 24) - 1
+    rng = np.random.default_rng(31)
+    n = 4000
+    lens = rng.integers(1, 9, size=n).astype(np.int32)
+    vars_ = rng.integers(0, V, size=int(lens.sum()))
+    vars_[:8] = [V - 1, V - 2, 0, 1, V - 1, 0, V - 3, 2]
+    lits = (2 * vars_ + rng.integers(0, 2, size=vars_.size)).astype(np.uint32)
+    d = rng.integers(0, np.iinfo(np.uint64).max, size=(V, 1), dtype=np.uint64, endpoint=True)
+    t = rng.integers(0, np.iinfo(np.uint64).max, size=(V, 1), dtype=np.uint64,
+                     endpoint=True) & d
+    got, exp = run_both(engine_factory(), lits, lens, V, d, t, np.array([ALL]))
+    assert exp.size > 0
+    assert_same(got, exp)
+
+
+def test_parity_maximum_clause_length(engine_factory):
+    # clauses of the longest accepted length stay alive through both passes in
+    # the low 32 slots of each word (every literal false there)
+    rng = np.random.default_rng(32)
+    V, words = 70000, 2
+    base = rng.integers(0, 2, size=V).astype(np.uint64)
+    lens = np.array([65535, 1, 65535, 3], np.int32)
+    vars_ = rng.integers(0, V, size=int(lens.sum()))
+    lits = (2 * vars_ + base[vars_]).astype(np.uint32)
+    low = np.uint64(0xFFFFFFFF)
+    d = np.full((V, words), ALL, dtype=np.uint64)
+    t = rng.integers(0, 2**63, size=(V, words), dtype=np.uint64) & ~low
+    t |= np.where(base[:, None] == 1, low, np.uint64(0))
+    # one undefined variable in slot 5 of word 1 makes the clauses holding it
+    # unit there instead of falsified
+    v0 = int(vars_[0])
+    d[v0, 1] &= ~np.uint64(1 << 5)
+    got, exp = run_both(engine_factory(), lits, lens, V, d, t & d, np.full(words, ALL))
+    assert exp.size > 0
+    assert_same(got, exp)
+
+
 def test_empty_db_and_empty_masks(engine_factory):
     e = engine_factory()
     e.reset_assignments(10, 1)
