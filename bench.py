@@ -413,6 +413,16 @@ def run_serving(args) -> int:
     log(f"{out.requests} requests in {out.steps} steps over {out.elapsed:.1f}s, "
         f"{out.checks_per_s:.3e} checks/s, p50 {pct[50]:.2f} ms p99 {pct[99]:.2f} ms "
         f"p999 {pct[99.9]:.2f} ms")
+    # latency by arrival time (the DB grows through the run)
+    windows = []
+    for w0 in np.arange(0.0, cfg.duration, 10.0):
+        sel = lat[(out.arrive_s >= w0) & (out.arrive_s < w0 + 10.0)]
+        if sel.size:
+            windows.append({"from_s": float(w0), "requests": int(sel.size),
+                            "p50": float(np.percentile(sel, 50)),
+                            "p99": float(np.percentile(sel, 99))})
+    log("by arrival window: " + ", ".join(
+        f"{w['from_s']:.0f}s p50 {w['p50']:.2f} p99 {w['p99']:.2f}" for w in windows))
     print(json.dumps({
         "metric": METRIC, "value": out.checks_per_s, "unit": "checks/s", "n_gpus": 1,
         "steps": out.steps, "warmup": 0, "ms_per_step": out.elapsed * 1e3 / max(1, out.steps),
@@ -427,6 +437,7 @@ def run_serving(args) -> int:
                    "records": out.records, "max_queue": out.max_queue, "tuned": tuned},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
                        "max": float(lat.max())},
+        "latency_by_arrival_10s": windows,
         "graph_ms": {"mean": float(np.mean(out.step_ms)), "max": float(np.max(out.step_ms))},
     }), flush=True)
     return 0

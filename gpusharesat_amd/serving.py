@@ -171,6 +171,7 @@ class ServeResult:
     records: int
     max_queue: int
     step_ms: list = field(default_factory=list)
+    arrive_s: np.ndarray = None  # arrival time of the request of each latency
 
     @property
     def checks_per_s(self) -> float:
@@ -330,6 +331,7 @@ class Server:
         pending: deque[int] = deque()
         inflight: deque = deque()
         lat = []
+        arrive = []
         ai = 0
         steps = 0
         checks = 0
@@ -396,6 +398,7 @@ class Server:
                 r = reqs[i]
                 r.t_done = t
                 lat.append(t - r.t_arrive)
+                arrive.append(r.t_arrive)
 
         while True:
             now = time.perf_counter() - start
@@ -413,7 +416,7 @@ class Server:
         elapsed = time.perf_counter() - start
         cl, li = e.totals()
         return ServeResult(len(lat), steps, elapsed, checks, np.array(lat) * 1e3, cl, li,
-                           records, max_queue, step_ms)
+                           records, max_queue, step_ms, np.array(arrive))
 
     def close(self):
         self.e.close()
