@@ -268,7 +268,7 @@ struct gss_engine {
     int8_t *d_lastpos = nullptr;
     Pinned<int8_t> h_lastpos;
     int64_t applied = 0;  // updates of the last gss_assig_apply, still in d_ups
-    DevBuf<uint32_t> d_offsets;
+    DevBuf<uint64_t> d_offsets;  // record offset of each chunk (dense copies of a sweep)
     DevBuf<gss_hit> d_dense;
     Pinned<gss_hit> h_hits;
 
@@ -1163,28 +1163,13 @@ int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
     const int64_t want = std::min<int64_t>(cap, e->last_hits);
     if (want <= 0) return GSS_OK;
     if (!out) return set_err(GSS_E_INVAL, "null output");
-    // Compact the claimed chunks into a dense device buffer, then one D2H.
-    const uint32_t nc = e->last_chunks;
-    std::vector<uint64_t> offs(nc);
-    int64_t acc = 0;
-    for (uint32_t c = 0; c < nc; c++) {
-        offs[c] = (uint64_t)acc;
-        acc += e->fills[c];
-    }
-    if ((rc = e->d_dense.reserve((size_t)e->last_hits))) return rc;
-    // segment copies as uint32 words through the scatter kernel
-    if ((rc = e->h_segs.reserve(nc)) || (rc = e->d_segs.reserve(nc))) return rc;
-    int ns = 0;
-    for (uint32_t c = 0; c < nc; c++) {
-        if (!e->fills[c]) continue;
-        ScatterSeg &sg = e->h_segs.p[ns++];
-        sg.dst = (uint32_t *)(e->d_dense.p + offs[c]);
-        sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
-        sg.count = (uint64_t)e->fills[c] * (sizeof(gss_hit) / 4);
-    }
-    HIP_TRY(hipMemcpyAsync(e->d_segs.p, e->h_segs.p, ns * sizeof(ScatterSeg),
-                           hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(launch_scatter(e->d_segs.p, ns, (const uint32_t *)e->d_hits.p, 0, e->stream));
+    // Compact the claimed chunks into a dense device buffer (offsets and the
+    // copy computed on the device), then one D2H.
+    if ((rc = e->d_dense.reserve((size_t)e->last_hits)) ||
+        (rc = e->d_offsets.reserve((size_t)e->chunk_cap + 1)))
+        return rc;
+    HIP_TRY(launch_compact_records(e->d_alloc, e->d_fills.p, e->chunk_cap, e->d_offsets.p,
+                                   e->d_hits.p, e->d_dense.p, e->d_dense.cap, e->stream));
     HIP_TRY(hipMemcpyAsync(out, e->d_dense.p, (size_t)want * sizeof(gss_hit),
                            hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
