@@ -81,15 +81,17 @@ __global__ __launch_bounds__(256) void k_ceiling_gather(const uint64_t *__restri
     out[tid] = acc;
 }
 
-// Device buffers and the private stream and events of one measurement.
+// Device buffers, a pinned host buffer and the private stream and events of
+// one measurement.
 struct Scope {
-    void *a = nullptr, *b = nullptr;
+    void *a = nullptr, *b = nullptr, *host = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ~Scope() {
         if (s) (void)hipStreamSynchronize(s);
         if (a) (void)hipFree(a);
         if (b) (void)hipFree(b);
+        if (host) (void)hipHostFree(host);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (s) (void)hipStreamDestroy(s);
@@ -219,6 +221,26 @@ int gss_ceiling_gather(int device, int64_t table_bytes, int64_t n_reads, int unr
     };
     if ((rc = time_reps(sc, reps, launch, ms))) return rc;
     *gbps = 128.0 * (double)n_reads / (*ms * 1e-3) / 1e9;
+    return GSS_OK;
+}
+
+int gss_ceiling_pcie(int device, int64_t bytes, int reps, double *d2h_gbps, double *h2d_gbps) {
+    if (bytes <= 0 || reps < 1 || !d2h_gbps || !h2d_gbps)
+        return set_err(GSS_E_INVAL, "pcie: bytes > 0 and reps >= 1");
+    Scope sc;
+    int rc = open_scope(device, sc);
+    if (rc) return rc;
+    if (hipMalloc(&sc.a, (size_t)bytes) != hipSuccess ||
+        hipHostMalloc(&sc.host, (size_t)bytes, hipHostMallocDefault) != hipSuccess)
+        return set_err(GSS_E_NOMEM, "pcie: cannot allocate %lld bytes", (long long)bytes);
+    CEIL_HIP(hipMemsetAsync(sc.a, 0x3C, (size_t)bytes, sc.s));
+    double ms = 0;
+    auto d2h = [&]() { return hipMemcpyAsync(sc.host, sc.a, (size_t)bytes, hipMemcpyDeviceToHost, sc.s); };
+    if ((rc = time_reps(sc, reps, d2h, &ms))) return rc;
+    *d2h_gbps = (double)bytes / (ms * 1e-3) / 1e9;
+    auto h2d = [&]() { return hipMemcpyAsync(sc.a, sc.host, (size_t)bytes, hipMemcpyHostToDevice, sc.s); };
+    if ((rc = time_reps(sc, reps, h2d, &ms))) return rc;
+    *h2d_gbps = (double)bytes / (ms * 1e-3) / 1e9;
     return GSS_OK;
 }
 

@@ -49,6 +49,12 @@ int gss_ceiling_copy(int device, int64_t bytes, int reps, double *gbps, double *
 int gss_ceiling_gather(int device, int64_t table_bytes, int64_t n_reads, int unroll, int reps,
                        double *gbps, double *ms);
 
+/* Copies `bytes` (positive) between a device buffer and pinned host memory
+ * (hipHostMalloc) with hipMemcpyAsync, `reps` times each way after one
+ * untimed copy: *d2h_gbps and *h2d_gbps are bytes / average copy time.  The
+ * floor under any transfer of hit records to the host. */
+int gss_ceiling_pcie(int device, int64_t bytes, int reps, double *d2h_gbps, double *h2d_gbps);
+
 #ifdef __cplusplus
 }
 #endif

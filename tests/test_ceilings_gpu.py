@@ -20,6 +20,9 @@ def test_device_query_and_small_ceilings():
     gbps, ms = ctypes.c_double(), ctypes.c_double()
     _lib.call("gss_ceiling_copy", 0, 64 << 20, 3, ctypes.byref(gbps), ctypes.byref(ms))
     assert gbps.value > 0 and ms.value > 0
+    d2h, h2d = ctypes.c_double(), ctypes.c_double()
+    _lib.call("gss_ceiling_pcie", 0, 16 << 20, 2, ctypes.byref(d2h), ctypes.byref(h2d))
+    assert d2h.value > 0 and h2d.value > 0
     for unroll in (1, 2, 4, 8):
         _lib.call("gss_ceiling_gather", 0, 2 << 20, 1 << 20, unroll, 2, ctypes.byref(gbps),
                   ctypes.byref(ms))
