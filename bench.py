@@ -105,15 +105,16 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (sweep-1024: 50, batched-16: 196)")
+                    help="timed steps (sweep-1024: 50, batched-16: 196, perftest: 2000)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps (sweep-1024: 5, batched-16: 4, all checked)")
+                    help="untimed steps (sweep-1024: 5, batched-16: 4, perftest: 20)")
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lat-steps", type=int, default=30,
                     help="sweep-1024: sweeps timed for report latency after the timed steps")
     ap.add_argument("--workload", default="sweep-1024",
-                    choices=["sweep-1024", "batched-16", "serving-32", "residency", "ceilings"])
+                    choices=["sweep-1024", "batched-16", "serving-32", "residency", "ceilings",
+                             "perftest"])
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
@@ -128,9 +129,9 @@ def main() -> int:
                        (item.split("=", 1) for item in args.tuning.split(","))}
     batched = args.workload == "batched-16"
     if args.steps is None:
-        args.steps = 196 if batched else 50
+        args.steps = {"batched-16": 196, "perftest": 2000}.get(args.workload, 50)
     if args.warmup is None:
-        args.warmup = 4 if batched else 5
+        args.warmup = {"batched-16": 4, "perftest": 20}.get(args.workload, 5)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sweep is built and measured on a single "
               "MI355X (a multi-GPU run would shard the clause DB, not the slots); "
@@ -145,6 +146,8 @@ def main() -> int:
         return run_residency(args)
     if args.workload == "ceilings":
         return run_ceilings(args)
+    if args.workload == "perftest":
+        return run_perftest(args)
 
     from gpusharesat_amd import workloads
     from gpusharesat_amd.engine import Engine
@@ -439,6 +442,86 @@ def run_serving(args) -> int:
                        "max": float(lat.max())},
         "latency_by_arrival_10s": windows,
         "graph_ms": {"mean": float(np.mean(out.step_ms)), "max": float(np.max(out.step_ms))},
+    }), flush=True)
+    return 0
+
+
+def run_perftest(args) -> int:
+    """The reference's own benchmark shape (glucose-syrup/perftest/perfTest.cu:
+    testPerf) through the drop-in GpuClauseSharer interface: 2M clauses of
+    12..20 random literals over 500 variables, one solver; per round the
+    solver unsets its previous values, sets each variable true / false /
+    unset with probability 1/3 each, sends the assignment, the GPU runs
+    (execute = two gpuRun calls, as the reference's test helper) and the
+    solver imports its reported clauses.  Seeded numpy instance of that
+    shape (the exact golden-count instance is checked in the tests).
+    Reports the reference's two figures: executions per second (rounds over
+    GPU + import time) and reads per microsecond (rounds x clauses x (1 +
+    mean length) / GPU time)."""
+    import ctypes
+    from gpusharesat_amd import _lib
+    from gpusharesat_amd.sharer import GpuClauseSharer, default_options
+    rounds, warm = args.steps, args.warmup
+    V, n_cl, lo, hi = 500, 2_000_000, 12, 20
+    rng = np.random.default_rng(25)
+    lens = rng.integers(lo, hi + 1, size=n_cl).astype(np.int32)
+    total = int(lens.sum())
+    vars_ = rng.integers(0, V, size=total)
+    lits = (2 * vars_ + rng.integers(0, 2, size=total)).astype(np.int32)  # mkLit(var, sign)
+    off = np.zeros(n_cl + 1, np.int64)
+    off[1:] = np.cumsum(lens)
+    log(f"perftest: {n_cl} clauses ({total} literals) over {V} vars, {rounds} rounds")
+    s = GpuClauseSharer(default_options(min_gpu_latency_micros=0, verbosity=0))
+    try:
+        s.setVarCount(V)
+        s.setCpuSolverCount(1)
+        t0 = time.perf_counter()
+        base = lits.ctypes.data
+        cid = ctypes.c_int64()
+        add = _lib.load().gss_sharer_add_clause
+        for i in range(n_cl):
+            rc = add(s._h, -1, base + 4 * int(off[i]), int(lens[i]), ctypes.byref(cid))
+            if rc:
+                raise RuntimeError(f"addClause {i} failed: {_lib.load().gss_last_error()}")
+        log(f"added in {time.perf_counter() - t0:.1f}s")
+        prev = np.zeros(0, np.int32)
+        gpu_s = imp_s = 0.0
+        reports = 0
+        for r in range(warm + rounds):
+            p = rng.integers(0, 3, size=V)
+            vs = np.nonzero(p < 2)[0]
+            cur = (2 * vs + (p[vs] == 1)).astype(np.int32)  # p 0: true, 1: false
+            if prev.size:
+                s.unsetSolverValues(0, prev)
+            if not s.trySetSolverValues(0, cur):
+                raise RuntimeError("trySetSolverValues refused")
+            s.trySendAssignment(0)
+            prev = cur
+            t_a = time.perf_counter()
+            s.execute()
+            t_b = time.perf_counter()
+            got = s.popAll(0)
+            t_c = time.perf_counter()
+            if r >= warm:
+                gpu_s += t_b - t_a
+                imp_s += t_c - t_b
+                reports += len(got)
+        execs = rounds / (gpu_s + imp_s)
+        reads_us = rounds * n_cl * (1 + (lo + hi) / 2) / (gpu_s * 1e6)
+        log(f"{rounds} rounds: gpu {gpu_s * 1e3:.0f} ms, import {imp_s * 1e3:.0f} ms, "
+            f"{reports} reports; executions per second {execs:.0f}, reads per microsecond "
+            f"{reads_us:.0f}")
+    finally:
+        s.close()
+    print(json.dumps({
+        "metric": "perfTest executions per second (GpuClauseSharer: set, send, gpuRun x2, import)",
+        "value": execs, "unit": "rounds/s", "n_gpus": 1, "steps": rounds, "warmup": warm,
+        "ms_per_step": (gpu_s + imp_s) * 1e3 / rounds, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "perftest: 2M clauses len U[12,20] over 500 vars, 1 solver, "
+                               "1 assignment per round (1/3 true, 1/3 false, 1/3 unset)",
+                   "reports": reports},
+        "reads_per_microsecond": reads_us,
     }), flush=True)
     return 0
 
