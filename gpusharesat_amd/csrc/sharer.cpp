@@ -191,7 +191,12 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     ensure_table();
     check_rc(gss_assig_apply(engine_, (int64_t)run.updates.size(), run.updates.data()),
              "gss_assig_apply");
-    check_rc(gss_sweep_async(engine_, valid.data()), "gss_sweep_async");
+    // with no completed assignment in any slot the check cannot report
+    // anything (every slot starts dead): the updates still go to the table,
+    // the sweep is skipped
+    run.swept = false;
+    for (uint64_t v : valid) run.swept = run.swept || v != 0;
+    if (run.swept) check_rc(gss_sweep_async(engine_, valid.data()), "gss_sweep_async");
     check_rc(gss_assig_roll_forward_applied(engine_, run.last_pos.data()),
              "gss_assig_roll_forward_applied");
     run.clause_count = clauses_.clause_count();
@@ -200,7 +205,10 @@ bool EngineClauseSharer::start_run(InFlight &run) {
 }
 
 void EngineClauseSharer::collect(InFlight &run, std::vector<gss_hit> &hits) {
-    (void)run;
+    if (!run.swept) {
+        hits.clear();
+        return;
+    }
     int64_t n = 0;
     check_rc(gss_sweep_wait(engine_, &n), "gss_sweep_wait");
     hits.resize((size_t)n);

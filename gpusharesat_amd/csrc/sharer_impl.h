@@ -225,6 +225,7 @@ private:
         std::vector<gss_var_update> updates;
         std::vector<int8_t> last_pos;
         long clause_count = 0;
+        bool swept = false;  // a sweep was launched (some slot was valid)
     };
 
     void run_once(bool can_start);
