@@ -231,15 +231,15 @@ struct gss_engine {
     Pinned<uint64_t> h_valid;
     // plain sweep: what d_desc / d_valid hold (uploads are skipped when a
     // sweep's descriptors and slot mask are unchanged), and the pinned copy of
-    // the claim counter and chunk fills that k_stage_out writes after the sweep
+    // the claim counter and record count that k_count_records writes after
+    // the sweep
     std::vector<BucketDesc> desc_new, desc_up;
     const BucketDesc *desc_up_at = nullptr;
     std::vector<uint64_t> valid_up;
-    Pinned<uint32_t> h_out;  // [0] = chunks claimed, [1 + c] = fill of chunk c
+    Pinned<uint32_t> h_count;  // [0] = chunks claimed, [1], [2] = records (low, high word)
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     uint32_t *d_alloc = nullptr;
-    std::vector<uint32_t> fills;
     uint32_t chunk_cap = 0;
     int nbuckets = 0, ntasks = 0, task_clauses = 0, blocks = 0;
     bool pending = false, waited = false, have_sweep = false;
@@ -374,8 +374,8 @@ int launch_current_sweep(gss_engine *e) {
         HIP_TRY(launch_sweep(a, e->words, e->tune_unroll, e->blocks, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
-    // claim counter and fills to pinned memory: the wait reads them there
-    HIP_TRY(launch_stage_out(e->d_alloc, e->d_fills.p, e->chunk_cap, e->h_out.p, e->stream));
+    // claim counter and record count to pinned memory: the wait reads them there
+    HIP_TRY(launch_count_records(e->d_alloc, e->d_fills.p, e->chunk_cap, e->h_count.p, e->stream));
     return GSS_OK;
 }
 
@@ -384,9 +384,8 @@ int reserve_chunks(gss_engine *e, uint64_t chunks) {
     int rc;
     if ((rc = e->d_hits.reserve((size_t)chunks * kHitChunk))) return rc;
     if ((rc = e->d_fills.reserve((size_t)chunks))) return rc;
-    if ((rc = e->h_out.reserve((size_t)chunks + 1))) return rc;
-    e->chunk_cap = (uint32_t)std::min<size_t>(
-        std::min<size_t>(e->d_fills.cap, e->d_hits.cap / kHitChunk), e->h_out.cap - 1);
+    if ((rc = e->h_count.reserve(4))) return rc;
+    e->chunk_cap = (uint32_t)std::min<size_t>(e->d_fills.cap, e->d_hits.cap / kHitChunk);
     return GSS_OK;
 }
 
@@ -1119,7 +1118,7 @@ int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
     if (!e->pending) return set_err(GSS_E_STATE, "no sweep in flight");
     for (;;) {
         HIP_TRY(hipStreamSynchronize(e->stream));
-        const uint32_t used = e->h_out.p[0];
+        const uint32_t used = e->h_count.p[0];
         if (used <= e->chunk_cap) {
             e->last_chunks = used;
             break;
@@ -1141,9 +1140,7 @@ int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
     e->last_ms = ms;
     e->total_ms += ms;
     e->nsweeps++;
-    e->fills.assign(e->h_out.p + 1, e->h_out.p + 1 + e->last_chunks);
-    int64_t tot = 0;
-    for (uint32_t f : e->fills) tot += f;
+    const int64_t tot = (int64_t)((uint64_t)e->h_count.p[1] | ((uint64_t)e->h_count.p[2] << 32));
     e->last_hits = tot;
     if (n_hits) *n_hits = tot;
     return GSS_OK;

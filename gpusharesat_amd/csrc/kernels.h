@@ -124,6 +124,10 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
                                  int words, const int8_t *last_pos, hipStream_t s);
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s);
+// Chunks claimed and records of a sweep to pinned host memory: h_out[0] =
+// claims, h_out[1] / h_out[2] = low / high word of the record count.
+hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                uint32_t *h_out, hipStream_t s);
 // Dense copy of a sweep's records (chunk order, the records of a chunk in
 // place): offs = exclusive prefix of the fills of the claimed chunks (cap + 1
 // entries), dense = the records, when they all fit in dense_cap; counts are

@@ -655,6 +655,30 @@ __global__ __launch_bounds__(256) void k_gather_chunks(const uint32_t *alloc,
     }
 }
 
+// Claim counter and record total of a sweep to pinned host memory:
+// out[0] = chunks claimed, out[1] / out[2] = low / high word of the records
+// in the chunks that fit (one block).
+__global__ __launch_bounds__(1024) void k_count_records(const uint32_t *alloc,
+                                                        const uint32_t *__restrict__ fills,
+                                                        uint32_t cap, uint32_t *out) {
+    __shared__ uint64_t part[1024];
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    uint64_t sum = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) sum += fills[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t d = 512; d > 0; d >>= 1) {
+        if (threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = used;
+        out[1] = (uint32_t)part[0];
+        out[2] = (uint32_t)(part[0] >> 32);
+    }
+}
+
 // ---- tagged DB compaction --------------------------------------------------
 
 __device__ __forceinline__ int find_ref(const CompactRef *__restrict__ r, int nb, int64_t g) {
@@ -865,6 +889,12 @@ hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, ui
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s) {
     hipLaunchKernelGGL(k_stage_out, dim3(256), dim3(256), 0, s, alloc, fills, cap, h_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                uint32_t *h_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_records, dim3(1), dim3(1024), 0, s, alloc, fills, cap, h_out);
     return hipGetLastError();
 }
 
