@@ -9,6 +9,8 @@
   (glucose-syrup/test/GpuSolverTest.cu:343-391, 394-454, 678-724, 949-981).
 """
 import itertools
+import json
+import os
 
 import numpy as np
 import pytest
@@ -16,6 +18,9 @@ import pytest
 import _oracle as orc
 
 ALL = 0xFFFFFFFFFFFFFFFF
+# the reference tests' known answers, transcribed with their file:line sources
+KNOWN = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "reference_known_answers.json")))
 
 
 def planes_from_values(values: np.ndarray):
@@ -48,21 +53,27 @@ def counting_rule(lits, vals):
     return nf == len(states), nu == 1 and nf == len(states) - 1
 
 
+def _perftest(rounds, **kw):
+    p = KNOWN["perftest"]
+    return orc.perftest_counts(p["clauses"], rounds, min_size=p["min_size"],
+                               max_size=p["max_size"], nvars=p["vars"], **kw)
+
+
 def test_golden_perftest_debug_count(oracle):
-    c = orc.perftest_counts(2_000_000, 15)
-    assert c[-1] == 143
+    c = _perftest(15)
+    assert c[-1] == KNOWN["perftest"]["reports_after_15_rounds"]
 
 
 def test_golden_perftest_release_count(oracle):
-    c = orc.perftest_counts(2_000_000, 2000)
-    assert c[14] == 143
-    assert c[-1] == 19739
+    c = _perftest(2000)
+    assert c[14] == KNOWN["perftest"]["reports_after_15_rounds"]
+    assert c[-1] == KNOWN["perftest"]["reports_after_2000_rounds"]
 
 
 def test_golden_rejects_other_argument_order(oracle):
     # drawing the variable before the sign does not reproduce the golden count,
     # which pins the evaluation order the reference build used
-    assert orc.perftest_counts(2_000_000, 15, sign_first=0)[-1] != 143
+    assert _perftest(15, sign_first=0)[-1] != KNOWN["perftest"]["reports_after_15_rounds"]
 
 
 def test_drand_matches_closed_form(oracle):
@@ -134,11 +145,12 @@ def lit(v, neg=False):
 
 def test_kat_find_clauses(oracle):
     # GpuSolverTest.cu:394-454: assignment {x0: F, x1: T, x2: U}
-    vals = np.array([[1, 0, 2]])
+    k = KNOWN["find_clauses"]
+    vals = np.array([k["assignment"]])
     d, t = planes_from_values(vals)
-    clauses = [[lit(0), lit(1)], [lit(0), lit(1, True)], [lit(1, True), lit(2)]]
-    off, lits = _csr(clauses)
-    assert _sweep_set(off, lits, d, t, [1]) == {(1, 0, "false"), (2, 0, "unit")}
+    assert k["clauses"] == [[lit(0), lit(1)], [lit(0), lit(1, True)], [lit(1, True), lit(2)]]
+    off, lits = _csr(k["clauses"])
+    assert _sweep_set(off, lits, d, t, [1]) == {(c, 0, kind) for c, kind in k["reports"]}
 
 
 def test_kat_clauses_assigs_reported(oracle):
@@ -151,21 +163,25 @@ def test_kat_clauses_assigs_reported(oracle):
     vals[64] = [0, 1, 0]
     d, t = planes_from_values(vals)
     off, lits = _csr([[lit(0)], [lit(1)], [lit(2)]])
+    k = KNOWN["clauses_assigs_reported"]
+    assert k["clauses"] == [[lit(0)], [lit(1)], [lit(2)]]
+    assert k["solver_assignments"] == [[list(vals[0]), list(vals[1])], [list(vals[64])], []]
     got = _sweep_set(off, lits, d, t, [0b11, 0b1])
     assert got == {(0, 0, "false"), (1, 1, "false"), (2, 1, "false"), (1, 64, "false")}
     by_solver = [len({c for c, a, _ in got if a // 64 == s}) for s in range(3)]
-    assert by_solver == [3, 1, 0]
+    assert by_solver == k["reported_clause_counts_per_solver"]
 
 
 def test_kat_many_clauses_all_at_once(oracle):
     # GpuSolverTest.cu:678-724: 4000 unit clauses {x_i}, x_i true for i % 3 == 0,
     # others undefined -> 4000 - ceil(4000 / 3) unit reports.
-    n = 4000
-    vals = np.where(np.arange(n) % 3 == 0, 0, 2)[None, :]
+    k = KNOWN["many_clauses_all_at_once"]
+    n = k["unit_clauses"]
+    vals = np.where(np.arange(n) % k["true_every"] == 0, 0, 2)[None, :]
     d, t = planes_from_values(vals)
     off, lits = _csr([[lit(i)] for i in range(n)])
     got = _sweep_set(off, lits, d, t, [1])
-    assert len(got) == n - (n + 2) // 3
+    assert len(got) == k["reports"] == n - (n + 2) // 3
     assert all(k == "unit" for _, _, k in got)
 
 
@@ -179,6 +195,7 @@ def test_kat_many_assignments(oracle):
     off, lits = _csr([[lit(2 * i, True), lit(2 * i + 1)] for i in range(32)])
     got = _sweep_set(off, lits, d, t, [(1 << 32) - 1])
     assert got == {(i, i, "unit") for i in range(32)}
+    assert len(got) == KNOWN["many_assignments"]["reports"]
 
 
 def test_sweep_matches_rule_per_clause(oracle):
