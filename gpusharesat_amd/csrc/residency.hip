@@ -11,12 +11,13 @@
 // approxNthAct, gpuShareLib/Clauses.cu:484-525) and compacts the DB on the
 // device.
 //
-// The MLP kernels: one thread per sample evaluates the network with the
-// parameters staged in LDS (5057 floats) and writes the activations and the
-// back-propagated deltas; the weight gradient is a sum of per-sample outer
-// products, computed per (parameter, chunk of samples) and reduced in a fixed
-// order, so gradients are deterministic.  ReLU'(0) = 0.  Loss: weighted
-// logistic, mean over the batch.
+// The MLP kernels: training runs one sample per wave (lane i = hidden unit i
+// of both 64-unit layers, the parameters staged in LDS) and writes the
+// activations and the back-propagated deltas; the weight gradient is a sum of
+// per-sample outer products, computed per (parameter, chunk of 32 samples)
+// and reduced in a fixed order, so gradients are deterministic.  Scoring runs
+// one clause per thread.  ReLU'(0) = 0.  Loss: weighted logistic, mean over
+// the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,7 +47,7 @@ constexpr int OW1 = 0, OB1 = H * F, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 
 static_assert(OB3 + 1 == P, "parameter layout");
 constexpr int kBuckets = 20000;
 constexpr float kRescale = 1e19f;  // RESCALE_CONST, gpuShareLib/Clauses.cuh:39
-constexpr int kGradChunk = 256;    // samples per partial gradient
+constexpr int kGradChunk = 32;     // samples per partial gradient
 
 int set_err(int code, const char *fmt, ...) {
     char buf[512];
@@ -282,66 +283,84 @@ __device__ __forceinline__ float logistic_loss(float z, float y) {
     return fmaxf(z, 0.f) - z * y + log1pf(expf(-fabsf(z)));
 }
 
-// Forward + backward of one sample per thread.  Writes a1, a2 (activations),
-// d1, d2 (deltas at the hidden pre-activations), d3 (at the output logit),
-// already multiplied by the sample weight.
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+// Forward + backward, one sample per wave: lane i is hidden unit i of both
+// layers (H = 64 = the wave).  W2 is staged in LDS in both orientations, so
+// the forward (row j by lane j) and the backward (column i by lane i) read
+// consecutive words.  A wave hands its layer's outputs to all its lanes
+// through its own LDS row.  Writes a1, a2 (activations), d1, d2 (deltas at
+// the hidden pre-activations), d3 (at the output logit), already multiplied
+// by the sample weight.
 __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
     const float *__restrict__ params, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ wt, const uint32_t *__restrict__ idx, int64_t n, float w_pos,
     float w_neg, float *__restrict__ a1o, float *__restrict__ a2o, float *__restrict__ d1o,
     float *__restrict__ d2o, float *__restrict__ d3o, float *loss_sum) {
-    __shared__ float w[P];
-    __shared__ float blk;
-    if (threadIdx.x == 0) blk = 0.f;
-    stage_params(params, w);
-    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    static_assert(H == 64, "one hidden unit per lane of a wave");
+    __shared__ float w1[H * F], b1[H], w2[H * H], w2t[H * H], b2[H], w3[H];
+    __shared__ float row[4][H];
+    __shared__ float b3s, blk;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < H * F; i += blockDim.x) w1[i] = params[OW1 + i];
+    for (int i = tid; i < H * H; i += blockDim.x) {
+        const float v = params[OW2 + i];
+        w2[i] = v;
+        w2t[(i % H) * H + i / H] = v;
+    }
+    if (tid < H) {
+        b1[tid] = params[OB1 + tid];
+        b2[tid] = params[OB2 + tid];
+        w3[tid] = params[OW3 + tid];
+    }
+    if (tid == 0) {
+        b3s = params[OB3];
+        blk = 0.f;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
     float my_loss = 0.f;
-    if (b < n) {
+    for (int64_t b = (int64_t)blockIdx.x * 4 + wv; b < n; b += (int64_t)gridDim.x * 4) {
         const int64_t src = idx ? (int64_t)idx[b] : b;
-        float xv[F];
+        float z1 = b1[lane];
 #pragma unroll
-        for (int k = 0; k < F; k++) xv[k] = x[src * F + k];
+        for (int k = 0; k < F; k++) z1 = fmaf(w1[lane * F + k], x[src * F + k], z1);
+        const float a1 = z1 > 0.f ? z1 : 0.f;
+        a1o[b * H + lane] = a1;
+        row[wv][lane] = a1;
+        __builtin_amdgcn_wave_barrier();
+        float z2 = b2[lane];
+#pragma unroll 16
+        for (int i = 0; i < H; i++) z2 = fmaf(w2t[i * H + lane], row[wv][i], z2);
+        const float a2 = z2 > 0.f ? z2 : 0.f;
+        a2o[b * H + lane] = a2;
+        const float z3 = b3s + wave_sum(w3[lane] * a2);
         const float yv = y[src];
         const float sw = (wt ? wt[src] : 1.f) * (yv > 0.5f ? w_pos : w_neg);
-        float a1[H];
-#pragma unroll
-        for (int i = 0; i < H; i++) {
-            float z = w[OB1 + i];
-#pragma unroll
-            for (int k = 0; k < F; k++) z = fmaf(w[OW1 + i * F + k], xv[k], z);
-            a1[i] = z > 0.f ? z : 0.f;
-            a1o[b * H + i] = a1[i];
-        }
-        float z3 = w[OB3];
-#pragma unroll 1
-        for (int j = 0; j < H; j++) {
-            float z = w[OB2 + j];
-#pragma unroll
-            for (int i = 0; i < H; i++) z = fmaf(w[OW2 + j * H + i], a1[i], z);
-            const float a = z > 0.f ? z : 0.f;
-            a2o[b * H + j] = a;
-            z3 = fmaf(w[OW3 + j], a, z3);
-        }
-        my_loss = sw * logistic_loss(z3, yv);
         const float d3 = sw * (sigmoidf(z3) - yv);
-        d3o[b] = d3;
-        // d2_j = d3 w3_j relu'(z2_j); d1_i = relu'(z1_i) sum_j W2[j][i] d2_j
-        float acc[H];
-#pragma unroll
-        for (int i = 0; i < H; i++) acc[i] = 0.f;
-#pragma unroll 1
-        for (int j = 0; j < H; j++) {
-            const float d2 = a2o[b * H + j] > 0.f ? d3 * w[OW3 + j] : 0.f;
-            d2o[b * H + j] = d2;
-#pragma unroll
-            for (int i = 0; i < H; i++) acc[i] = fmaf(w[OW2 + j * H + i], d2, acc[i]);
+        if (lane == 0) {
+            d3o[b] = d3;
+            my_loss += sw * logistic_loss(z3, yv);
         }
-#pragma unroll
-        for (int i = 0; i < H; i++) d1o[b * H + i] = a1[i] > 0.f ? acc[i] : 0.f;
+        // d2_j = d3 w3_j relu'(z2_j); d1_i = relu'(z1_i) sum_j W2[j][i] d2_j
+        const float d2 = a2 > 0.f ? d3 * w3[lane] : 0.f;
+        d2o[b * H + lane] = d2;
+        __builtin_amdgcn_wave_barrier();  // every lane has read a1 from the row
+        row[wv][lane] = d2;
+        __builtin_amdgcn_wave_barrier();
+        float acc = 0.f;
+#pragma unroll 16
+        for (int j = 0; j < H; j++) acc = fmaf(w2[j * H + lane], row[wv][j], acc);
+        d1o[b * H + lane] = a1 > 0.f ? acc : 0.f;
+        __builtin_amdgcn_wave_barrier();  // the next sample rewrites the row
     }
     atomicAdd(&blk, my_loss);
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss_sum, blk);
+    if (tid == 0) atomicAdd(loss_sum, blk);
 }
 
 // Partial gradient of every parameter over one chunk of samples.
@@ -386,8 +405,17 @@ __global__ void k_mlp_reduce_step(const float *__restrict__ partial, int nchunks
                                   float *__restrict__ mom, float lr, float mu) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    // loads issued eight at a time, summed in chunk order (deterministic)
     float g = 0.f;
-    for (int c = 0; c < nchunks; c++) g += partial[(size_t)c * P + p];
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = partial[(size_t)(c + u) * P + p];
+#pragma unroll
+        for (int u = 0; u < 8; u++) g += v[u];
+    }
+    for (; c < nchunks; c++) g += partial[(size_t)c * P + p];
     g *= scale;
     grad[p] = g;
     if (lr != 0.f) {
@@ -535,7 +563,8 @@ int run_batch(gss_res *r, hipStream_t st, const float *x, const float *y, const 
         (rc = r->partial.reserve((size_t)nchunks * P)))
         return rc;
     RES_HIP(hipMemsetAsync(r->loss.p, 0, sizeof(float), st));
-    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(grid(n, 1u << 30)), dim3(256), 0, st, r->params.p, x,
+    const unsigned fb_blocks = (unsigned)std::min<int64_t>((n + 3) / 4, 4096);
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(std::max(fb_blocks, 1u)), dim3(256), 0, st, r->params.p, x,
                        y, wt, idx, n, w_pos, w_neg, r->a1.p, r->a2.p, r->d1.p, r->d2.p, r->d3.p,
                        r->loss.p);
     RES_HIP(hipGetLastError());
