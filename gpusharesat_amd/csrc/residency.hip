@@ -154,56 +154,115 @@ __global__ void k_res_init(TagState s, const uint32_t *__restrict__ tags,
     s.alive[t] = 1;
 }
 
-// One block per claimed chunk of hit records: record (len, row) -> tag.
+// Hit records -> tag state, one 256-thread block per quarter of a claimed
+// chunk (one record per thread: a sweep claims few chunks, so one block per
+// chunk would leave the kernel waiting on four dependent lookup chains in a
+// row).  A clause's records (one per assignment word that hit it) are adjacent in
+// its chunk, so each wave merges runs of equal (len, row): the run's first
+// lane ORs the run's word bits with a segmented suffix scan and issues one
+// atomicAdd of the run length and one atomicOr for the whole run.
+constexpr uint32_t kRecBlock = 256;  // k_res_records block size
+constexpr uint32_t kRecParts = kHitChunk / kRecBlock;
+static_assert(kHitChunk % kRecBlock == 0, "a chunk splits into whole blocks");
+
 __global__ void k_res_records(const gss_hit *__restrict__ hits, const uint32_t *__restrict__ fills,
                               uint32_t chunks, const uint32_t *const *__restrict__ tagptr,
                               int ntagptr, TagState s, unsigned long long *pairs) {
     __shared__ unsigned long long blk;
     if (threadIdx.x == 0) blk = 0;
     __syncthreads();
+    const int lane = threadIdx.x & 63;
     unsigned long long mine = 0;
-    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const uint32_t f = fills[c];
-        for (uint32_t i = threadIdx.x; i < f; i += blockDim.x) {
+    const uint64_t parts = (uint64_t)chunks * kRecParts;
+    for (uint64_t g = blockIdx.x; g < parts; g += gridDim.x) {
+        const uint32_t c = (uint32_t)(g / kRecParts);
+        const uint32_t f = fills[c];  // uniform in the block
+        const uint32_t base = (uint32_t)(g % kRecParts) * kRecBlock;
+        if (base >= f) continue;
+        const uint32_t i = base + threadIdx.x;
+        const bool on = i < f;
+        uint32_t row = 0, len = 0xffffffffu;
+        uint64_t bit = 0;
+        const uint32_t *tp = nullptr;
+        if (on) {
             const gss_hit h = hits[(size_t)c * kHitChunk + i];
-            if (h.len >= ntagptr) continue;
-            const uint32_t *tp = tagptr[h.len];
-            if (!tp) continue;
-            const uint32_t t = tp[h.row];
-            atomicAdd(&s.cnt[t], 1u);
-            atomicOr((unsigned long long *)&s.solv[t], 1ull << (h.word & 63));
-            mine += (unsigned long long)__popcll(h.false_mask | h.unit_mask);
+            row = h.row;
+            len = h.len;
+            bit = 1ull << (h.word & 63);
+            tp = (int)len < ntagptr ? tagptr[len] : nullptr;
+            if (tp) mine += (unsigned long long)__popcll(h.false_mask | h.unit_mask);
+        }
+        const uint32_t prow = (uint32_t)__shfl_up((int)row, 1);
+        const uint32_t plen = (uint32_t)__shfl_up((int)len, 1);
+        const bool head = on && (lane == 0 || prow != row || plen != len);
+        // a run ends at the next head or at the first lane past the fill
+        const uint64_t bound = __ballot(head || !on);
+        const uint64_t above = lane == 63 ? 0ull : bound & (~0ull << (lane + 1));
+        const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;
+        uint64_t acc = bit;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t lo = (uint32_t)__shfl_down((int)(uint32_t)acc, d);
+            const uint32_t hi = (uint32_t)__shfl_down((int)(uint32_t)(acc >> 32), d);
+            if (lane + d < end) acc |= ((uint64_t)hi << 32) | lo;
+        }
+        if (head && tp) {
+            const uint32_t t = tp[row];
+            atomicAdd(&s.cnt[t], (uint32_t)(end - lane));
+            atomicOr((unsigned long long *)&s.solv[t], (unsigned long long)acc);
         }
     }
-    atomicAdd(&blk, mine);
+    // Device-scope atomics on one address are serialised (~10 ns each):
+    // blocks with nothing to add issue none.
+    if (mine) atomicAdd(&blk, mine);
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(pairs, blk);
+    if (threadIdx.x == 0 && blk) atomicAdd(pairs, blk);
 }
 
+// All of a tag's fields are read before any is written: the state arrays
+// are separate allocations the compiler cannot prove apart, so interleaved
+// reads and writes would each wait for the store before them.
 __global__ void k_res_update(TagState s, int64_t cap, int32_t round, float incr,
                              unsigned int *max_act_bits) {
+    unsigned int mx = 0;  // largest activity seen (positive floats order as uints)
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cap;
          t += (int64_t)gridDim.x * blockDim.x) {
         if (!s.alive[t]) continue;
         const uint32_t c = s.cnt[t];
-        const bool hit = c > 0;
-        s.ema_f[t] = 0.5f * s.ema_f[t] + (hit ? 0.5f : 0.0f);
-        s.ema_s[t] = 0.95f * s.ema_s[t] + (hit ? 0.05f : 0.0f);
-        s.tot[t] += (float)c;
-        if (hit) {
-            s.last_hit[t] = round;
-            s.hit_rounds[t] += 1;
-        }
-        s.solv_cum[t] |= s.solv[t];
-        s.cnt_prev[t] = c;
-        // one bump per record, added one at a time as the reference does
+        const float ef = s.ema_f[t], es = s.ema_s[t], tot = s.tot[t];
+        const int32_t lh = s.last_hit[t];
+        const uint32_t hr = s.hit_rounds[t];
+        const uint64_t sc = s.solv_cum[t], sv = s.solv[t];
         float a = s.act[t];
+        const bool hit = c > 0;
+        // one bump per record, added one at a time as the reference does
         for (uint32_t k = 0; k < c; k++) a += incr;
+        s.ema_f[t] = 0.5f * ef + (hit ? 0.5f : 0.0f);
+        s.ema_s[t] = 0.95f * es + (hit ? 0.05f : 0.0f);
+        s.tot[t] = tot + (float)c;
+        s.last_hit[t] = hit ? round : lh;
+        s.hit_rounds[t] = hr + (hit ? 1u : 0u);
+        s.solv_cum[t] = sc | sv;
+        s.cnt_prev[t] = c;
         s.act[t] = a;
         s.cnt[t] = 0;
         s.solv[t] = 0;
-        atomicMax(max_act_bits, __float_as_uint(a));  // positive floats order as uints
+        mx = max(mx, __float_as_uint(a));
     }
+    // Device-scope atomics on one address are serialised (~10 ns each), so
+    // the block's maximum is formed in LDS and sent only when it exceeds
+    // the value already there.  A stale read can only be lower than the
+    // true value, so skipping when it is not lower is exact.
+    __shared__ unsigned int bmx;
+    if (threadIdx.x == 0) bmx = 0;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) mx = max(mx, (unsigned int)__shfl_xor((int)mx, d));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(&bmx, mx);
+    __syncthreads();
+    if (threadIdx.x == 0 && bmx > __hip_atomic_load(max_act_bits, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(max_act_bits, bmx);
 }
 
 __global__ void k_res_rescale(TagState s, int64_t cap) {
@@ -734,7 +793,9 @@ int gss_res_account(gss_res *r, int32_t round, int64_t *hit_pairs) {
     RES_HIP(hipMemsetAsync(r->counters.p, 0, 2 * sizeof(unsigned long long), st));
     RES_HIP(hipMemsetAsync(r->maxbits.p, 0, 4, st));
     if (chunks) {
-        hipLaunchKernelGGL(k_res_records, dim3(std::min<uint32_t>(chunks, 65536)), dim3(256), 0,
+        const uint64_t parts = (uint64_t)chunks * kRecParts;
+        hipLaunchKernelGGL(k_res_records, dim3((unsigned)std::min<uint64_t>(parts, 1u << 20)),
+                           dim3(kRecBlock), 0,
                            st, hits, fills, chunks, r->tagptr.p, (int)tp.size(), r->state(),
                            r->counters.p);
         RES_HIP(hipGetLastError());

@@ -67,6 +67,56 @@ def sweep_tags(e, valid):
     return hits, counts
 
 
+def test_record_accounting_with_64_words(engine_factory):
+    """Short clauses against 64 dense words: most clauses have a record in
+    nearly every word, so the per-wave merge of a clause's adjacent records
+    sees runs of up to 64.  Counts, word bits, bumps and hit pairs are exact."""
+    rng = np.random.default_rng(21)
+    V, W = 200, 64
+    e, r = make_res(engine_factory, max_tags=4000)
+    decay = np.float32(0.99999)
+    e.reset_assignments(V, W)
+    d = workloads.bernoulli_words(rng, V, W * 64, 0.9)
+    t = workloads.bernoulli_words(rng, V, W * 64, 0.3) & d
+    e.upload(d, t)
+    valid = np.full(W, ALL)
+    n = 2000
+    lens = rng.integers(1, 4, size=n).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    tags = rng.permutation(4000)[:n].astype(np.uint32)
+    r.add(lits, lens, tags, 0)
+    incr = np.float32(1.0)
+    act = {}
+    for tg in tags:
+        incr = np.float32(incr / decay)
+        act[int(tg)] = incr
+    words_cum = {}
+    for rnd in range(2):
+        hits, counts = sweep_tags(e, valid)
+        assert max(counts.values()) == W  # some runs span the whole wave
+        tag_of = {int(L): e.read_tags(int(L)) for L in np.unique(hits["len"])}
+        words = {}
+        for h in hits:
+            tg = int(tag_of[int(h["len"])][int(h["row"])])
+            words[tg] = words.get(tg, 0) | (1 << int(h["word"]))
+        pairs_exp = int(np.bitwise_count(hits["false_mask"] | hits["unit_mask"]).sum())
+        assert r.account(rnd) == pairs_exp
+        for tg, c in counts.items():
+            a = act[tg]
+            for _ in range(c):
+                a = np.float32(a + incr)
+            act[tg] = a
+            words_cum[tg] = words_cum.get(tg, 0) | words[tg]
+        tg_all = np.array(sorted(act), dtype=np.uint32)
+        got = r.activity(tg_all)
+        assert np.array_equal(got, np.array([act[int(x)] for x in tg_all], np.float32))
+        f = r.features(rnd, tg_all)
+        cnt = np.array([counts.get(int(x), 0) for x in tg_all])
+        assert np.allclose(f[:, 4], np.log1p(cnt) / 6, atol=1e-6)
+        pc = np.array([bin(words_cum.get(int(x), 0)).count("1") for x in tg_all])
+        assert np.array_equal(f[:, 10], (pc / 64).astype(np.float32))
+
+
 def test_activity_policy_and_kept_db_parity(engine_factory):
     rng = np.random.default_rng(9)
     V, W = 300, 2
