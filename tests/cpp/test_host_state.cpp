@@ -10,6 +10,8 @@
 //                             act-only removal threshold, rescale
 //   GpuSolverTest.cu:1037-1050, Reported.cu:105-158
 //                             batch pop order, dedup, release, own clauses
+//   ConcurrentQueueTest.cc:29-70
+//                             batches handed out in publication order
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -313,6 +315,31 @@ static void report_queue() {
     CHECK(!q.pop(lits, count, id, st));
 }
 
+// Batches are handed out in the order they were published, and one published
+// after the earlier ones were drained comes next (ConcurrentQueueTest.cc:29-70:
+// iteration and removal both follow insertion order).
+static void report_queue_fifo() {
+    std::atomic<unsigned long> st[6];
+    for (auto &s : st) s = 0;
+    ReportQueue q;
+    int *lits;
+    int count;
+    long id;
+    for (long a = 0; a < 4; a++) q.assig_was_sent(a);
+    q.publish(batch(q, 0, 1, {{lit(3)}}, 3));
+    q.publish(batch(q, 1, 1, {{lit(2)}}, 2));
+    q.publish(batch(q, 2, 1, {{lit(5)}}, 5));
+    CHECK(q.pop(lits, count, id, st) && id == 3 && lits[0] == lit(3));
+    CHECK(q.pop(lits, count, id, st) && id == 2 && lits[0] == lit(2));
+    CHECK(q.pop(lits, count, id, st) && id == 5 && lits[0] == lit(5));
+    CHECK(!q.pop(lits, count, id, st));
+    q.publish(batch(q, 3, 1, {{lit(9)}}, 9));
+    CHECK(q.pop(lits, count, id, st) && id == 9 && lits[0] == lit(9));
+    CHECK(!q.pop(lits, count, id, st));
+    CHECK(q.last_all_reported() == 4);
+    CHECK(st[reportedClauses] == 4);
+}
+
 int main() {
     two_assignments();
     two_solvers();
@@ -326,6 +353,7 @@ int main() {
     rescale();
     reduce_keeps_active();
     report_queue();
+    report_queue_fifo();
     std::printf("%d checks, %d failed\n", checks, failures);
     return failures ? 1 : 0;
 }

@@ -10,6 +10,10 @@ Reference tests (glucose-syrup/test/GpuSolverTest.cu unless noted):
   SolverHasManyClausesReportedAllAtOnce :678-724, testReduceDb :889-935,
   testSolverPassesManyAssignments :949-981, testManyAssignments :157-181,
   SolverCanReimportClause (rel-newtech/test/GpuSolverTest.cu:577-596),
+  TwoSolverImportBinary :605-632, SolverUnsets :634-654,
+  SolverImportUnary :656-676, SolverHasManyClausesReported :678-694,
+  OneInstanceTwoAssignments :727-765, findConflict :858-887 and
+  testSendClauseToGpu :999-1035 (the parts visible through the sharer),
   ClauseActivityLbdTest.cu:59-95 (activity init / bump / decay).
 """
 import numpy as np
@@ -132,6 +136,93 @@ def test_same_assignment_resent_reports_again():
         assert s.trySendAssignment(0) >= 0
         s.execute()
         assert len(s.popAll(0)) == 1
+
+
+def test_two_solvers_each_get_their_binary():
+    with sharer(3, 2) as s:
+        assert s.trySetSolverValues(0, [T(0)])
+        assert s.trySetSolverValues(1, [F(0)])
+        s.trySendAssignment(0)
+        s.trySendAssignment(1)
+        a = s.addClause(-1, [F(0), mkLit(1)])
+        b = s.addClause(-1, [mkLit(0), F(1)])
+        s.execute()
+        assert s.popAll(0) == [([F(0), mkLit(1)], a)]
+        assert s.popAll(1) == [([mkLit(0), F(1)], b)]
+
+
+def test_unset_values_are_unset_on_the_device():
+    with sharer(2, 1) as s:
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert s.popAll(0) == []
+        s.unsetSolverValues(0, [T(1)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        cid = s.addClause(-1, [F(0), mkLit(1)])
+        s.execute()
+        # reported because x1 is no longer set
+        assert s.popAll(0) == [([F(0), mkLit(1)], cid)]
+
+
+def test_unary_clause_reported_as_unit():
+    with sharer(2, 1) as s:
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        cid = s.addClause(-1, [mkLit(0)])
+        s.execute()
+        assert s.popAll(0) == [([mkLit(0)], cid)]
+        assert s.getOneSolverStat(0, "reportedClausesUnit") == 1
+        assert s.getOneSolverStat(0, "reportedClauses") == 1
+
+
+def test_one_clause_reported_per_run():
+    # the solver imports each unit clause and sends its new trail: only the
+    # newest clause is unit under the next assignment
+    n = 20
+    with sharer(n, 1) as s:
+        for i in range(n):
+            cid = s.addClause(-1, [mkLit(i)])
+            s.trySendAssignment(0)
+            s.execute()
+            assert s.popAll(0) == [([mkLit(i)], cid)]
+            assert s.trySetSolverValues(0, [T(i)])
+        assert s.getOneSolverStat(0, "reportedClauses") == n
+
+
+def test_clauses_useful_for_earlier_assignments_are_reported():
+    with sharer(4, 1) as s:
+        a = s.addClause(-1, [F(0), mkLit(2)])
+        b = s.addClause(-1, [F(1), mkLit(3)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.unsetSolverValues(0, [T(0)])
+        assert s.trySetSolverValues(0, [T(1)])
+        s.trySendAssignment(0)
+        s.unsetSolverValues(0, [T(1)])
+        assert s.getOneSolverStat(0, "reportedClauses") == 0
+        s.execute()
+        assert sorted(c for _, c in s.popAll(0)) == sorted([a, b])
+        assert s.getOneSolverStat(0, "reportedClauses") == 2
+
+
+def test_clause_counts_after_add_and_own_clause():
+    with sharer(3, 1) as s:
+        s.addClause(-1, [F(0), F(1)])
+        s.gpuRun()
+        assert s.getGlobalStat("gpuClauses") == 1
+        assert s.getGlobalStat("gpuClauseLengthSum") == 2
+    # a clause the solver learned itself is not handed back to it
+    with sharer(3, 1) as s:
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.addClause(0, [F(0), F(1)])
+        s.execute()
+        assert s.getGlobalStat("gpuClauses") == 1
+        assert s.getGlobalStat("gpuClauseLengthSum") == 2
+        assert s.popAll(0) == []
+        assert s.getOneSolverStat(0, "reportedClauses") == 0
 
 
 def test_many_clauses_reported_all_at_once():

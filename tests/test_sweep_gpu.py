@@ -160,6 +160,25 @@ def test_parity_maximum_clause_length(engine_factory):
     assert_same(got, exp)
 
 
+def test_second_sweep_holds_only_its_own_records(engine_factory):
+    """A sweep's records replace the previous sweep's: the reference's
+    cleared reporter holds only the new reports
+    (rel-newtech/test/OtherTests.cu:142-177)."""
+    e = engine_factory()
+    lens = np.ones(4, np.int32)
+    rows = e.append(np.array([0, 2, 4, 6], np.uint32), lens)  # {x0} .. {x3}
+    e.reset_assignments(4, 1)
+    for false_vars in ((0, 1), (2, 3), ()):
+        is_def = np.full((4, 1), ALL)
+        is_true = is_def.copy()
+        is_true[list(false_vars), 0] = 0
+        e.upload(is_def, is_true)
+        e.sweep(np.full(1, ALL))
+        got = gpu_records(e, lens, rows)
+        assert sorted(got["clause"].tolist()) == list(false_vars)
+        assert (got["false_mask"] == ALL).all() and (got["unit_mask"] == 0).all()
+
+
 def test_empty_db_and_empty_masks(engine_factory):
     e = engine_factory()
     e.reset_assignments(10, 1)
