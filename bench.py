@@ -110,6 +110,8 @@ def main() -> int:
                     help="untimed steps (sweep-1024: 5, batched-16: 4, perftest: 20)")
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="batched-16: steps also timed through the plain entry points")
     ap.add_argument("--lat-steps", type=int, default=30,
                     help="sweep-1024: sweeps timed for report latency after the timed steps")
     ap.add_argument("--workload", default="sweep-1024",
@@ -312,6 +314,8 @@ def run_batched(args) -> int:
     ref.append(init_lits, init_lens)
     ref.reset_assignments(gen.V, gen.words)
     ref.upload(d0, t0p)
+    if tuned:
+        ref.apply_tuning(tuned)
     del d0, t0p
     verified = 0
     for k in range(args.warmup):
@@ -328,12 +332,33 @@ def run_batched(args) -> int:
             log(f"step {k}: batched records differ from the plain replay")
             return 1
         verified += 1
+    # the plain form on the steps the batched run times first: one step at a
+    # time through the plain entry points, records copied to the host
+    n_plain = min(args.plain_steps, args.steps)
+    plain_checks = []
+    lits_ref = ref.totals()[1]
+    ref.sync()
+    t_plain = time.perf_counter()
+    for k in range(args.warmup, args.warmup + n_plain):
+        st = plan[k]
+        ref.append(st.lits, st.lens)
+        ref.apply(st.ups)
+        ref.sweep(st.valid)
+        ref.hits()
+        ref.roll_forward(st.ups, st.last_pos)
+        lits_ref += int(st.lits.size)
+        plain_checks.append(lits_ref * st.assignments)
+    ref.sync()
+    plain_s = time.perf_counter() - t_plain
     ref.close()
-    log(f"{verified} warmup steps identical to the plain replay; timing {args.steps} steps")
+    log(f"{verified} warmup steps identical to the plain replay; plain form "
+        f"{sum(plain_checks) / plain_s:.3e} checks/s over {n_plain} steps; "
+        f"timing {args.steps} steps")
     lits_db = e.totals()[1]
     checks = 0
     lat = []
     step_ms = []
+    done_t = []
     hits_total = 0
     pending = []
     e.sync()
@@ -357,7 +382,8 @@ def run_batched(args) -> int:
         if todo:
             enqueue_next()
         hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
-        lat.append(time.perf_counter() - pending.pop(0))
+        done_t.append(time.perf_counter())
+        lat.append(done_t[-1] - pending.pop(0))
         step_ms.append(ms)
     e.sync()
     elapsed = time.perf_counter() - t_start
@@ -392,6 +418,16 @@ def run_batched(args) -> int:
                        "max": float(lat_ms.max())},
         "graph_ms": {"mean": float(np.mean(step_ms)), "max": float(np.max(step_ms))},
     }
+    if n_plain:
+        plain_rate = sum(plain_checks) / plain_s
+        batched_rate = sum(plain_checks) / (done_t[n_plain - 1] - t_start)
+        out["plain_form"] = {
+            "steps": n_plain, "checks_per_s": plain_rate,
+            "batched_checks_per_s_same_steps": batched_rate,
+            "batched_over_plain": batched_rate / plain_rate,
+            "what": "the first timed steps replayed one at a time through the plain entry "
+                    "points (append, apply, sweep, records to the host, roll forward), "
+                    "same tuning"}
     print(json.dumps(out), flush=True)
     e.close()
     return 0
