@@ -1306,7 +1306,7 @@ static int step_blocks(gss_engine *e, int64_t tasks) {
 // slot's pinned h_hits on the slot's stream: one DMA copy when the graph made
 // them dense, else a compaction first (and the next capture's dense buffer is
 // sized for this volume).
-static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32_t used) {
+static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
     int rc;
     if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
     if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
@@ -1723,7 +1723,7 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     e->last_step_hits = tot;
     e->last_step_chunks = used;
     e->last_step_ms = ms;
-    if (tot > 0 && (rc = queue_records_copy(e, sl, tot, used))) return rc;
+    if (tot > 0 && (rc = queue_records_copy(sl, tot, used))) return rc;
     sl.results = true;
     if (n_hits) *n_hits = tot;
     if (step_ms) *step_ms = ms;

@@ -14,6 +14,8 @@ Reference tests (glucose-syrup/test/GpuSolverTest.cu unless noted):
   SolverImportUnary :656-676, SolverHasManyClausesReported :678-694,
   OneInstanceTwoAssignments :727-765, findConflict :858-887 and
   testSendClauseToGpu :999-1035 (the parts visible through the sharer),
+  the threading contract of GpuClauseSharer.h:78-158 (solver threads and
+  the GPU thread at once),
   ClauseActivityLbdTest.cu:59-95 (activity init / bump / decay).
 """
 import numpy as np
@@ -440,3 +442,82 @@ def test_snapshot_semantics_match_oracle(solvers, seed):
                     for hs in _clause_hits(clauses, snaps[so]):
                         exp |= hs
                 assert got == sorted(ids[i] for i in exp), (rnd, so)
+
+
+def _useful(lits, assig):
+    """Falsified or unit under a var -> bool assignment (unset = undefined)."""
+    undef = 0
+    for x in lits:
+        v, neg = x >> 1, x & 1
+        if v not in assig:
+            undef += 1
+        elif assig[v] != bool(neg):
+            return False  # a true literal
+    return undef <= 1
+
+
+def test_solver_threads_and_gpu_thread_concurrently():
+    """The threading contract (GpuClauseSharer.h:78-158): each solver thread
+    sets, unsets, sends, pops and adds clauses while one thread loops
+    gpuRun.  Every clause handed to a solver is falsified or unit under one
+    of the assignments that solver sent, and nothing is lost or crashes."""
+    import threading
+    V, S, ROUNDS = 40, 4, 150
+    rng0 = np.random.default_rng(17)
+    with sharer(V, S) as s:
+        for _ in range(300):
+            k = int(rng0.integers(2, 5))
+            vs = rng0.choice(V, size=k, replace=False)
+            s.addClause(-1, [mkLit(int(v), bool(rng0.integers(2))) for v in vs])
+        stop = threading.Event()
+        errors = []
+        sent = [[] for _ in range(S)]
+        got = [[] for _ in range(S)]
+
+        def gpu():
+            try:
+                while not stop.is_set():
+                    s.gpuRun()
+            except Exception as ex:  # reported after the join
+                errors.append(ex)
+
+        def solver(sid):
+            try:
+                rng = np.random.default_rng(100 + sid)
+                cur = {}
+                for _ in range(ROUNDS):
+                    if cur:
+                        s.unsetSolverValues(sid, [mkLit(v, not val) for v, val in cur.items()])
+                        cur = {}
+                    vs = rng.choice(V, size=int(0.6 * V), replace=False)
+                    new = {int(v): bool(rng.integers(2)) for v in vs}
+                    if s.trySetSolverValues(sid, [mkLit(v, not val) for v, val in new.items()]):
+                        cur = new
+                    if s.trySendAssignment(sid) >= 0:
+                        sent[sid].append(dict(cur))
+                    got[sid].extend(l for l, _ in s.popAll(sid))
+                    if rng.random() < 0.1:
+                        vs = rng.choice(V, size=3, replace=False)
+                        s.addClause(sid, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+            except Exception as ex:
+                errors.append(ex)
+
+        g = threading.Thread(target=gpu, daemon=True)
+        g.start()
+        ts = [threading.Thread(target=solver, args=(i,), daemon=True) for i in range(S)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=60)
+        stop.set()
+        g.join(timeout=60)
+        assert not g.is_alive() and not any(t.is_alive() for t in ts)
+        assert not errors, errors
+        s.execute()
+        for sid in range(S):
+            got[sid].extend(l for l, _ in s.popAll(sid))
+        print("sent", [len(x) for x in sent], "reported", [len(x) for x in got], flush=True)
+        assert sum(len(x) for x in sent) > 0 and sum(len(x) for x in got) > 0
+        for sid in range(S):
+            for lits in got[sid]:
+                assert any(_useful(lits, a) for a in sent[sid]), (sid, lits)
