@@ -208,6 +208,7 @@ def main() -> int:
     latency = None
     if lat:
         latency = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                   "p999": float(np.percentile(lat, 99.9)),
                    "max": float(max(lat)), "sweeps": len(lat),
                    "what": "submit -> all hit records in pinned host memory"}
         log(f"report latency p50 {latency['p50']:.2f} ms, p99 {latency['p99']:.2f} ms "
@@ -415,6 +416,7 @@ def run_batched(args) -> int:
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),
+                       "p999": float(np.percentile(lat_ms, 99.9)),
                        "max": float(lat_ms.max())},
         "graph_ms": {"mean": float(np.mean(step_ms)), "max": float(np.max(step_ms))},
     }
