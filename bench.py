@@ -110,6 +110,8 @@ def main() -> int:
                     help="untimed steps (sweep-1024: 5, batched-16: 4, perftest: 20)")
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--inflight", type=int, default=2, choices=(1, 2),
+                    help="batched-16: steps in flight (2 = throughput, 1 = latency)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="batched-16: steps also timed through the plain entry points")
     ap.add_argument("--lat-steps", type=int, default=30,
@@ -374,18 +376,21 @@ def run_batched(args) -> int:
         lits_db += int(st.lits.size)
         checks += lits_db * st.assignments
 
-    # two steps in flight; once the older one ends its records travel to the
-    # host while the next step is staged, then they are read
-    while todo and len(pending) < 2:
+    # two steps in flight (default); once the older one ends its records
+    # travel to the host while the next step is staged, then they are read.
+    # With --inflight 1 each step is staged, run and read before the next.
+    while todo and len(pending) < args.inflight:
         enqueue_next()
     while pending:
         _, ms = e.step_wait()
-        if todo:
+        if todo and args.inflight > 1:
             enqueue_next()
         hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
         done_t.append(time.perf_counter())
         lat.append(done_t[-1] - pending.pop(0))
         step_ms.append(ms)
+        if todo and args.inflight == 1:
+            enqueue_next()
     e.sync()
     elapsed = time.perf_counter() - t_start
     clauses_end = e.totals()[0]
@@ -412,7 +417,7 @@ def run_batched(args) -> int:
                         "Poisson(16x2800) Pareto(1.5) clauses per step, DB 1M -> ~10M",
             "clauses_end": int(clauses_end), "literals_end": int(lits_db),
             "hit_records": int(hits_total), "tuned": tuned,
-            "verified_steps": verified,
+            "verified_steps": verified, "steps_in_flight": args.inflight,
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),
