@@ -378,19 +378,19 @@ def run_batched(args) -> int:
 
     # two steps in flight (default); once the older one ends its records
     # travel to the host while the next step is staged, then they are read.
-    # With --inflight 1 each step is staged, run and read before the next.
+    # With --inflight 1 a step is submitted only once the previous one has
+    # left the device (its records still travelling), so no step waits
+    # behind another's graph.
     while todo and len(pending) < args.inflight:
         enqueue_next()
     while pending:
         _, ms = e.step_wait()
-        if todo and args.inflight > 1:
+        if todo:
             enqueue_next()
         hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
         done_t.append(time.perf_counter())
         lat.append(done_t[-1] - pending.pop(0))
         step_ms.append(ms)
-        if todo and args.inflight == 1:
-            enqueue_next()
     e.sync()
     elapsed = time.perf_counter() - t_start
     clauses_end = e.totals()[0]
