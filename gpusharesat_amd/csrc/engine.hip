@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -210,6 +211,10 @@ struct gss_engine {
     // replacement, clear); appends leave existing rows in place
     uint64_t rows_version = 0, sweep_rows_version = 0;
     int64_t total_clauses = 0, total_lits = 0;
+    // Cap on the device bytes of the clause buckets (their capacities), from
+    // GPUSHARESAT_DB_LIMIT_BYTES at creation; 0 = only the device's memory.
+    // Growth past it fails with GSS_E_NOMEM exactly as a full device would.
+    int64_t db_limit = 0;
     // Largest literal ever stored (running max, never lowered); a sweep is
     // refused unless the table has a row for it, so the kernel's table reads
     // stay in bounds.
@@ -318,11 +323,27 @@ int log2_exact(int w) {
     return (1 << l) == w ? l : -1;
 }
 
-int ensure_bucket_cap(gss_engine *e, int len, int64_t need) {
+// With shrink, an array more than twice the need is replaced by one of the
+// need (a bucket being replaced, count already 0): a reduced DB gives its
+// memory back.  The new array is allocated before the old one is released.
+int ensure_bucket_cap(gss_engine *e, int len, int64_t need, bool shrink = false) {
     if ((int)e->buckets.size() <= len) e->buckets.resize(len + 1);
     Bucket &b = e->buckets[len];
-    if (need <= b.cap) return GSS_OK;
-    int64_t nc = std::max<int64_t>(std::max<int64_t>(need, b.cap * 2), 256);
+    const bool smaller = shrink && b.d && std::max<int64_t>(need, 256) < b.cap / 2;
+    if (need <= b.cap && !smaller) return GSS_OK;
+    int64_t nc = smaller ? std::max<int64_t>(need, 256)
+                         : std::max<int64_t>(std::max<int64_t>(need, b.cap * 2), 256);
+    if (e->db_limit > 0) {
+        const int64_t per_row = (int64_t)len * 4 + (e->tagged ? 4 : 0);
+        int64_t used = 0;
+        for (size_t l = 1; l < e->buckets.size(); l++)
+            used += e->buckets[l].cap * ((int64_t)l * 4 + (e->tagged ? 4 : 0));
+        const int64_t room = e->db_limit - (used - b.cap * per_row);
+        if (nc * per_row > room) nc = need;  // no headroom rather than no room
+        if (nc * per_row > room)
+            return set_err(GSS_E_NOMEM, "bucket %d: %lld rows exceed the clause DB limit of %lld bytes",
+                           len, (long long)nc, (long long)e->db_limit);
+    }
     // Stream-ordered: the new array is allocated, filled and the old one
     // released in the engine stream's order, after every queued sweep or step
     // that still reads the old one, so growth never stalls the host.
@@ -406,6 +427,8 @@ int gss_engine_create(int device, gss_engine **out) {
     gss_engine *e = new (std::nothrow) gss_engine();
     if (!e) return set_err(GSS_E_NOMEM, "engine allocation failed");
     e->device = device;
+    if (const char *lim = std::getenv("GPUSHARESAT_DB_LIMIT_BYTES"))
+        e->db_limit = std::max<long long>(0, std::atoll(lim));
     int rc = GSS_OK;
     auto fail = [&](int code) {
         gss_engine_destroy(e);
@@ -775,7 +798,7 @@ int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
     const int64_t old = b.count;
     e->rows_version++;
     b.count = 0;  // nothing to preserve when growing
-    if ((rc = ensure_bucket_cap(e, len, n))) {
+    if ((rc = ensure_bucket_cap(e, len, n, true))) {
         b.count = old;
         return rc;
     }

@@ -279,10 +279,22 @@ void EngineClauseSharer::reduce_db_now() {
     added_at_last_reduce_ = clauses_.added();
     global_stats_[gpuReduceDbs]++;
     log(2, "c Reducing gpu clause db, clause count is " + std::to_string(clauses_.clause_count()));
-    for (int len : clauses_.reduce(limits_.max_clause_size)) {
+    auto upload = [&](int len) {
         const auto &lits = clauses_.bucket_lits(len);
         check_rc(gss_db_set_bucket(engine_, len, (int64_t)(lits.size() / len), lits.data()),
                  "gss_db_set_bucket");
+    };
+    // the reduced buckets first (they shrink), then every bucket the device
+    // does not hold as the mirror does: after an append that ran out of
+    // memory the mirror already has that batch's clauses, in buckets the
+    // reduction may not touch (lengths 1 and 2 are always kept)
+    const std::vector<int> changed = clauses_.reduce(limits_.max_clause_size);
+    for (int len : changed) upload(len);
+    for (int len = 1; len <= clauses_.max_len(); len++) {
+        if (std::find(changed.begin(), changed.end(), len) != changed.end()) continue;
+        int64_t on_device = 0;
+        check_rc(gss_db_bucket_count(engine_, len, &on_device), "gss_db_bucket_count");
+        if (on_device != clauses_.rows(len)) upload(len);
     }
     global_stats_[gpuClauses] = (unsigned long)clauses_.clause_count();
     global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();

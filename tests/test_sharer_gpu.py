@@ -363,6 +363,55 @@ def test_write_clauses_in_cnf(tmp_path):
         assert p.read_text().splitlines() == ["p cnf 4 2", "2 0", "-1 4 0"]
 
 
+def test_out_of_device_memory_halves_the_db(tmp_path, monkeypatch):
+    """An append that does not fit makes gpuRun reduce the DB and carry on,
+    and hasRunOutOfGpuMemoryOnce() turns true (GpuRunner.cu:243-260).  The
+    condition is forced from the host by capping the clause DB's device
+    bytes (GPUSHARESAT_DB_LIMIT_BYTES, read when the engine is created).
+    Afterwards the device holds exactly the host's clauses, those of the
+    failed batch in the always-kept lengths 1-2 included: an assignment's
+    reports equal the clauses of the written CNF that it makes falsified or
+    unit."""
+    from collections import Counter
+    monkeypatch.setenv("GPUSHARESAT_DB_LIMIT_BYTES", str(256 << 10))
+    rng = np.random.default_rng(5)
+    V = 400
+
+    def add(s, n, lo, hi):
+        for _ in range(n):
+            vs = rng.choice(V, size=int(rng.integers(lo, hi)), replace=False)
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+
+    with sharer(V, 1) as s:
+        add(s, 2000, 3, 7)
+        add(s, 300, 1, 3)
+        s.execute()
+        assert not s.hasRunOutOfGpuMemoryOnce()
+        add(s, 12000, 3, 7)  # ~216 KB more of clauses: past the cap
+        add(s, 2000, 1, 3)
+        s.gpuRun()
+        assert s.hasRunOutOfGpuMemoryOnce()
+        assert s.getGlobalStat("gpuReduceDbs") == 1
+        kept = s.getGlobalStat("gpuClauses")
+        assert 2300 <= kept < 16300
+        assig = {int(v): bool(rng.integers(2)) for v in rng.choice(V, size=V // 2, replace=False)}
+        assert s.trySetSolverValues(0, [mkLit(v, not val) for v, val in assig.items()])
+        assert s.trySendAssignment(0) >= 0
+        s.execute()
+        got = Counter(tuple(sorted(l)) for l, _ in s.popAll(0))
+        p = tmp_path / "db.cnf"
+        s.writeClausesInCnf(str(p))
+        lines = p.read_text().splitlines()
+        assert lines[0].split()[3] == str(kept)
+        exp = Counter()
+        for line in lines[1:]:
+            lits = [mkLit(abs(int(x)) - 1, int(x) < 0) for x in line.split()[:-1]]
+            if _useful(lits, assig):
+                exp[tuple(sorted(lits))] += 1
+        assert sum(exp.values()) > 0
+        assert got == exp
+
+
 def test_rejected_clauses_and_bad_ids():
     from gpusharesat_amd._lib import GssError
     with sharer(4, 1) as s:
