@@ -158,9 +158,12 @@ bool EngineClauseSharer::start_run(InFlight &run) {
         int rc = gss_db_append(engine_, (int64_t)b.lens.size(), b.lits.data(), b.lens.data(),
                                rows.data());
         if (rc == GSS_E_NOMEM) {
-            // out of device memory: halve the DB and re-upload (GpuRunner.cu:243-260)
+            // out of device memory: the DB is halved and re-uploaded
+            // (GpuRunner.cu:243-260), but only after the run in flight has
+            // been delivered: its hits name clauses by row, and the
+            // reduction moves rows (GpuRunner.cu:251-253)
             ran_out_of_memory_ = true;
-            reduce_db_now();
+            reduce_pending_ = true;
             return false;
         }
         check_rc(rc, "gss_db_append");
@@ -264,6 +267,10 @@ void EngineClauseSharer::run_once(bool can_start) {
     }
     if (can_start) start_run(in_flight_);
     if (had) deliver(prev, hit_buf_);
+    if (reduce_pending_) {
+        reduce_pending_ = false;
+        reduce_db_now();
+    }
 }
 
 void EngineClauseSharer::gpuRun() {

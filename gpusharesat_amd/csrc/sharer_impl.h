@@ -260,6 +260,7 @@ private:
     std::vector<gss_hit> hit_buf_;
     long added_at_last_reduce_ = 0;
     bool ran_out_of_memory_ = false;
+    bool reduce_pending_ = false;  // out of memory: reduce once the run in flight is delivered
 };
 
 }  // namespace GpuShare

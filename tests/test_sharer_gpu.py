@@ -368,34 +368,53 @@ def test_out_of_device_memory_halves_the_db(tmp_path, monkeypatch):
     and hasRunOutOfGpuMemoryOnce() turns true (GpuRunner.cu:243-260).  The
     condition is forced from the host by capping the clause DB's device
     bytes (GPUSHARESAT_DB_LIMIT_BYTES, read when the engine is created).
-    Afterwards the device holds exactly the host's clauses, those of the
-    failed batch in the always-kept lengths 1-2 included: an assignment's
-    reports equal the clauses of the written CNF that it makes falsified or
-    unit."""
+    The append fails while a sweep with hits is in flight: those hits are
+    delivered as the clauses they named before the reduction moved rows
+    (GpuRunner.cu:251-253).  Afterwards the device holds exactly the host's
+    clauses, those of the failed batch in the always-kept lengths 1-2
+    included: an assignment's reports equal the clauses of the written CNF
+    that it makes falsified or unit."""
     from collections import Counter
     monkeypatch.setenv("GPUSHARESAT_DB_LIMIT_BYTES", str(256 << 10))
     rng = np.random.default_rng(5)
     V = 400
 
     def add(s, n, lo, hi):
+        out = []
         for _ in range(n):
             vs = rng.choice(V, size=int(rng.integers(lo, hi)), replace=False)
-            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+            out.append([mkLit(int(v), bool(rng.integers(2))) for v in vs])
+            s.addClause(-1, out[-1])
+        return out
+
+    def assignment():
+        return {int(v): bool(rng.integers(2)) for v in rng.choice(V, size=V // 2, replace=False)}
+
+    def useful(clauses, assig):
+        return Counter(tuple(sorted(c)) for c in clauses if _useful(c, assig))
 
     with sharer(V, 1) as s:
-        add(s, 2000, 3, 7)
-        add(s, 300, 1, 3)
+        first = add(s, 2000, 3, 7) + add(s, 300, 1, 3)
         s.execute()
         assert not s.hasRunOutOfGpuMemoryOnce()
+        a1 = assignment()
+        trail = [mkLit(v, not val) for v, val in a1.items()]
+        assert s.trySetSolverValues(0, trail)
+        assert s.trySendAssignment(0) >= 0
+        s.gpuRun()  # a sweep of the first clauses against a1 is in flight
         add(s, 12000, 3, 7)  # ~216 KB more of clauses: past the cap
         add(s, 2000, 1, 3)
-        s.gpuRun()
+        s.gpuRun()  # a1's hits collected, the append fails, they are delivered, the DB reduced
         assert s.hasRunOutOfGpuMemoryOnce()
         assert s.getGlobalStat("gpuReduceDbs") == 1
+        got1 = Counter(tuple(sorted(l)) for l, _ in s.popAll(0))
+        exp1 = useful(first, a1)
+        assert sum(exp1.values()) > 0 and got1 == exp1
         kept = s.getGlobalStat("gpuClauses")
         assert 2300 <= kept < 16300
-        assig = {int(v): bool(rng.integers(2)) for v in rng.choice(V, size=V // 2, replace=False)}
-        assert s.trySetSolverValues(0, [mkLit(v, not val) for v, val in assig.items()])
+        s.unsetSolverValues(0, trail)
+        a2 = assignment()
+        assert s.trySetSolverValues(0, [mkLit(v, not val) for v, val in a2.items()])
         assert s.trySendAssignment(0) >= 0
         s.execute()
         got = Counter(tuple(sorted(l)) for l, _ in s.popAll(0))
@@ -403,11 +422,8 @@ def test_out_of_device_memory_halves_the_db(tmp_path, monkeypatch):
         s.writeClausesInCnf(str(p))
         lines = p.read_text().splitlines()
         assert lines[0].split()[3] == str(kept)
-        exp = Counter()
-        for line in lines[1:]:
-            lits = [mkLit(abs(int(x)) - 1, int(x) < 0) for x in line.split()[:-1]]
-            if _useful(lits, assig):
-                exp[tuple(sorted(lits))] += 1
+        db = [[mkLit(abs(int(x)) - 1, int(x) < 0) for x in line.split()[:-1]] for line in lines[1:]]
+        exp = useful(db, a2)
         assert sum(exp.values()) > 0
         assert got == exp
 
