@@ -44,6 +44,9 @@ constexpr int kDefaultFuseLen = 4;
 // byte offset of the literal's row in a 16-word table); see kernels.hip.
 constexpr int kLitShift = 7;
 constexpr uint32_t kMaxLit = (1u << (32 - kLitShift)) - 1;
+// a table of GSS_MAX_VARS variables has 2 * GSS_MAX_VARS + 2 rows (the two
+// sentinels included), the last at offset kMaxLit
+static_assert(2ull * GSS_MAX_VARS + 1 == kMaxLit, "the exported variable limit follows the row offsets");
 
 thread_local std::string g_err;
 

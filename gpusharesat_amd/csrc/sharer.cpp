@@ -378,7 +378,7 @@ void EngineClauseSharer::setVarCount(int n) {
 long EngineClauseSharer::addClause(int solverId, int *lits, int count) {
     if (count < 1 || count > limits_.max_clause_size || !lits) return -1;
     for (int i = 0; i < count; i++)
-        if (lits[i] < 0) return -1;
+        if (lits[i] < 0 || (unsigned)(lits[i] >> 1) >= GSS_MAX_VARS) return -1;
     const long id = clauses_.stage(lits, count, count);  // lbd = size (GpuClauseSharerImpl.cu:125)
     if (solverId >= 0 && solverId < solver_count()) reports_[solverId]->clause_was_added(id);
     return id;

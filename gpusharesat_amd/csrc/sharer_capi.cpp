@@ -192,7 +192,8 @@ int gss_sharer_get_one_solver_stat_count(gss_sharer *s, int *out) {
 int gss_sharer_try_set_solver_values(gss_sharer *s, int solver_id, const int *lits, int count,
                                      int *ok) {
     NEED(solver_ok(s, solver_id) && ok && count >= 0 && (count == 0 || lits), "bad argument");
-    for (int i = 0; i < count; i++) NEED(lits[i] >= 0, "negative literal");
+    for (int i = 0; i < count; i++)
+        NEED(lits[i] >= 0 && (unsigned)(lits[i] >> 1) < GSS_MAX_VARS, "literal out of range");
     return guard([&] {
         *ok = s->impl->trySetSolverValues(solver_id, const_cast<int *>(lits), count) ? 1 : 0;
     });
@@ -200,7 +201,8 @@ int gss_sharer_try_set_solver_values(gss_sharer *s, int solver_id, const int *li
 
 int gss_sharer_unset_solver_values(gss_sharer *s, int solver_id, const int *lits, int count) {
     NEED(solver_ok(s, solver_id) && count >= 0 && (count == 0 || lits), "bad argument");
-    for (int i = 0; i < count; i++) NEED(lits[i] >= 0, "negative literal");
+    for (int i = 0; i < count; i++)
+        NEED(lits[i] >= 0 && (unsigned)(lits[i] >> 1) < GSS_MAX_VARS, "literal out of range");
     return guard([&] { s->impl->unsetSolverValues(solver_id, const_cast<int *>(lits), count); });
 }
 

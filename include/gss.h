@@ -53,6 +53,9 @@ extern "C" {
 
 #define GSS_MAX_WORDS 64    /* max uint64 words per literal row (4096 slots) */
 #define GSS_MAX_CLAUSE_LEN 65535
+/* variables 0 .. GSS_MAX_VARS - 1 (fewer for tables wider than 16 words,
+ * whose size is capped at 4 GiB) */
+#define GSS_MAX_VARS ((1u << 24) - 1)
 
 typedef struct gss_engine gss_engine;
 

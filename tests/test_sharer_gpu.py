@@ -437,6 +437,17 @@ def test_rejected_clauses_and_bad_ids():
             s.trySendAssignment(5)
         with pytest.raises(GssError):
             s.addClause(3, [0])
+        # literals past the device's variable range (2^24 - 1 variables) are
+        # refused where they enter, before any state changes
+        assert s.addClause(-1, [mkLit((1 << 24) - 1)]) == -1
+        with pytest.raises(GssError):
+            s.trySetSolverValues(0, [mkLit((1 << 24) - 1)])
+        with pytest.raises(GssError):
+            s.unsetSolverValues(0, [mkLit((1 << 24) - 1)])
+        cid = s.addClause(-1, [mkLit((1 << 24) - 2)])  # the largest variable is fine
+        assert cid >= 0
+        s.execute()
+        assert s.getGlobalStat("gpuClauses") == 1
 
 
 def _clause_hits(clauses, snapshots):
