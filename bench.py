@@ -111,7 +111,8 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0, help="instance scale (1.0 = headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--inflight", type=int, default=2, choices=(1, 2),
-                    help="batched-16: steps in flight (2 = throughput, 1 = latency)")
+                    help="batched-16, serving-32: steps on the device at once "
+                         "(2 = throughput, 1 = latency)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="batched-16: steps also timed through the plain entry points")
     ap.add_argument("--lat-steps", type=int, default=30,
@@ -442,7 +443,7 @@ def run_batched(args) -> int:
 
 def run_serving(args) -> int:
     from gpusharesat_amd import serving as sv
-    cfg = sv.ServeConfig(duration=args.duration, rate=args.rate)
+    cfg = sv.ServeConfig(duration=args.duration, rate=args.rate, max_inflight=args.inflight)
     log(f"serving-32: {cfg}")
     srv = sv.Server(cfg, log=log)
     try:
@@ -480,7 +481,8 @@ def run_serving(args) -> int:
                    "duration_s": cfg.duration, "requests": out.requests,
                    "served_per_s": out.requests / out.elapsed,
                    "clauses_end": out.clauses_end, "literals_end": out.literals_end,
-                   "records": out.records, "max_queue": out.max_queue, "tuned": tuned},
+                   "records": out.records, "max_queue": out.max_queue, "tuned": tuned,
+                   "steps_in_flight": cfg.max_inflight},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
                        "max": float(lat.max())},
         "latency_by_arrival_10s": windows,

@@ -144,6 +144,7 @@ class ServeConfig:
     lit_true: float = 0.9           # a clause literal is true under the planted assignment
     pool: int = 48                  # distinct request payloads per size
     clause_pool: int = 2_000_000
+    max_inflight: int = 2           # steps on the device at once (1: each step starts once the previous has left it)
     seed: int = 5
 
 
@@ -405,7 +406,7 @@ class Server:
             arrivals(now)
             if ai >= len(reqs) and not pending and not inflight:
                 break
-            if len(inflight) == 2 or (inflight and not pending):
+            if len(inflight) >= c.max_inflight or (inflight and not pending):
                 finish()
                 continue
             if not pending:
