@@ -73,8 +73,10 @@ long SolverWindow::done() {
     return current_id_++;
 }
 
-void SolverWindow::current_assignment(uint8_t *out) const {
-    if (!last_val_.empty()) std::memcpy(out, last_val_.data(), last_val_.size());
+void SolverWindow::current_assignment(uint8_t *out, int n) const {
+    const int known = std::min<int>(n, (int)last_val_.size());
+    if (known > 0) std::memcpy(out, last_val_.data(), (size_t)known);
+    if (n > known) std::memset(out + known, kValUndef, (size_t)(n - known));
 }
 
 SolverWindow::Taken SolverWindow::take(std::vector<gss_var_update> &out, int word) {

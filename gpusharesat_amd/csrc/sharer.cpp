@@ -456,13 +456,17 @@ long EngineClauseSharer::getLastAssigAllReported(int solver) {
     return reports_[solver]->last_all_reported();
 }
 
+// One byte per variable of the sharer's count (setVarCount), as the caller
+// sizes the buffer; a solver may have set variables beyond it.
 void EngineClauseSharer::getCurrentAssignment(int solver, uint8_t *assig) {
     SolverWindow &w = *windows_[solver];
+    const int n = var_count_;
     {
         std::lock_guard<std::mutex> g(w.lock);
-        w.current_assignment(assig);
+        w.current_assignment(assig, n);
     }
-    for (int l : to_unset_[solver]) assig[l >> 1] = kValUndef;
+    for (int l : to_unset_[solver])
+        if ((l >> 1) < n) assig[l >> 1] = kValUndef;
 }
 
 GpuClauseSharer *makeGpuClauseSharerPtr(GpuClauseSharerOptions opts,

@@ -50,7 +50,8 @@ public:
     void set_var(int var, uint8_t val);
     bool available() const { return current_id_ != first_id_ + slots_; }
     long done();
-    void current_assignment(uint8_t *out) const;
+    // values of variables [0, n): those the window never saw are undefined
+    void current_assignment(uint8_t *out, int n) const;
     int var_count() const { return (int)last_val_.size(); }
 
     struct Taken {

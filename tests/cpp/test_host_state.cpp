@@ -109,8 +109,18 @@ static void many_assignments(int slots) {
     const gss_var_update *u7 = find(ups, 7);
     CHECK(u7 && (u7->is_def & 1) == 0);
     uint8_t cur[64];
-    w.current_assignment(cur);
+    w.current_assignment(cur, 8);
     CHECK(cur[7] == kValUndef && cur[6] == kValTrue && cur[5] == kValFalse);
+    // exactly n values: variables the window never saw are undefined, and
+    // nothing is written past n
+    std::vector<uint8_t> big((size_t)slots + 4, 0xAB);
+    w.current_assignment(big.data(), slots + 2);
+    CHECK(big[6] == kValTrue && big[slots] == kValUndef && big[slots + 1] == kValUndef &&
+          big[slots + 2] == 0xAB);
+    uint8_t small[4];
+    std::memset(small, 0xAB, sizeof small);
+    w.current_assignment(small, 3);
+    CHECK(small[2] == kValTrue && small[1] == kValFalse && small[3] == 0xAB);
 }
 
 // carry-forward: a value changed after an assignment was completed keeps the

@@ -282,6 +282,12 @@ def test_current_assignment_values():
         s.unsetSolverValues(1, [F(2)])
         assert list(s.getCurrentAssignment(1, 5)) == [0, 2, 2, 2, 0]
         assert list(s.getCurrentAssignment(0, 5)) == [2] * 5
+        # a variable past the sharer's count: the buffer still gets exactly
+        # one byte per counted variable
+        assert s.trySetSolverValues(0, [T(1), F(9)])
+        assert list(s.getCurrentAssignment(0, 5)) == [2, 0, 2, 2, 2]
+        s.unsetSolverValues(0, [F(9)])
+        assert list(s.getCurrentAssignment(0, 5)) == [2, 0, 2, 2, 2]
 
 
 def test_own_clause_not_reported_back():
