@@ -77,7 +77,9 @@ int gss_sharer_try_send_assignment(gss_sharer *s, int solver_id, int64_t *out_id
 int gss_sharer_pop_reported_clause(gss_sharer *s, int solver_id, const int **lits, int *count,
                                    int64_t *gpu_clause_id, int *ok);
 int gss_sharer_get_last_assig_all_reported(gss_sharer *s, int solver_id, int64_t *out);
-/* assig must hold one byte per variable (0 true, 1 false, 2 undef). */
+/* assig must hold one byte per variable of the sharer's count (the last
+ * gss_sharer_set_var_count): 0 true, 1 false, 2 undef; exactly that many
+ * bytes are written. */
 int gss_sharer_get_current_assignment(gss_sharer *s, int solver_id, uint8_t *assig);
 int gss_sharer_get_one_solver_stat(gss_sharer *s, int solver_id, int stat, int64_t *out);
 
