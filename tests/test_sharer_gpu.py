@@ -275,6 +275,29 @@ def test_slot_window_exhaustion_and_deferred_unsets():
         assert len(s.popAll(0)) == 1  # {x3} unit in every assignment
 
 
+def test_more_solvers_after_runs_keep_each_solvers_table_word():
+    """Growing the solver count re-lays the assignment table out (1 -> 4
+    words); solver 0's word keeps its values, so its next assignment, sent
+    as the changes since the last one, is checked against all of them."""
+    with sharer(4, 1) as s:
+        a = s.addClause(-1, [F(0), mkLit(1)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert s.popAll(0) == [([F(0), mkLit(1)], a)]
+        s.setCpuSolverCount(3)
+        assert list(s.getCurrentAssignment(0, 4)) == [0, 2, 2, 2]
+        b = s.addClause(-1, [F(0), F(2), mkLit(3)])
+        assert s.trySetSolverValues(0, [T(2)])  # x0 still true from before
+        s.trySendAssignment(0)
+        assert s.trySetSolverValues(2, [T(0), T(2)])
+        s.trySendAssignment(2)
+        s.execute()
+        assert sorted(c for _, c in s.popAll(0)) == [a, b]
+        assert s.popAll(1) == []
+        assert sorted(c for _, c in s.popAll(2)) == [a, b]
+
+
 def test_current_assignment_values():
     with sharer(5, 2) as s:
         s.trySetSolverValues(1, [T(0), F(2), T(4)])
