@@ -298,6 +298,24 @@ def test_more_solvers_after_runs_keep_each_solvers_table_word():
         assert sorted(c for _, c in s.popAll(2)) == [a, b]
 
 
+def test_more_variables_after_runs():
+    """The table grows in place when the variable count grows, and when a
+    clause names a variable past it; earlier values stay."""
+    with sharer(3, 1) as s:
+        a = s.addClause(-1, [F(0), mkLit(1)])
+        assert s.trySetSolverValues(0, [T(0)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert [c for _, c in s.popAll(0)] == [a]
+        s.setVarCount(10)
+        b = s.addClause(-1, [F(0), F(8), mkLit(9)])
+        c = s.addClause(-1, [F(0), mkLit(20)])  # past the variable count
+        assert s.trySetSolverValues(0, [T(8)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert sorted(x for _, x in s.popAll(0)) == [a, b, c]
+
+
 def test_current_assignment_values():
     with sharer(5, 2) as s:
         s.trySetSolverValues(1, [T(0), F(2), T(4)])
