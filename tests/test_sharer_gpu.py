@@ -370,6 +370,26 @@ def test_reduce_db_keeps_active_clauses():
         assert s.getAddedClauseCount() == 3
 
 
+def test_reduce_db_with_a_run_in_flight():
+    """reduceDb delivers the run in flight before rows move
+    (GpuClauseSharerImpl.cu:105-110): its reports are the clauses they
+    named, though the reduction then removes some of them."""
+    with sharer(6, 1) as s:
+        ids = [s.addClause(-1, [F(0), mkLit(v), mkLit(v + 1)]) for v in (1, 2, 3, 4)]
+        assert s.trySetSolverValues(0, [T(0), F(2), F(3)])
+        s.trySendAssignment(0)
+        s.gpuRun()  # the sweep is in flight
+        s.reduceDb()
+        got = {c: l for l, c in s.popAll(0)}
+        # x0 true, x2 and x3 false: (~x0 v x1 v x2) and (~x0 v x3 v x4) are
+        # unit, (~x0 v x2 v x3) is falsified, (~x0 v x4 v x5) has two
+        # undefined literals
+        assert sorted(got) == ids[:3]
+        for i, v in enumerate((1, 2, 3)):
+            assert got[ids[i]] == [F(0), mkLit(v), mkLit(v + 1)]
+        assert s.getGlobalStat("gpuReduceDbs") == 1
+
+
 def test_activity_init_bump_decay():
     # ClauseActivityLbdTest.cu:59-95 with decay 0.5: a new clause starts at the
     # increment after one decay (2.0); a report bumps it by the increment.
