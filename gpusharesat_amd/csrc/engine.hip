@@ -213,6 +213,11 @@ struct gss_engine {
     // bumped whenever existing rows move or vanish (compaction, bucket
     // replacement, clear); appends leave existing rows in place
     uint64_t rows_version = 0, sweep_rows_version = 0;
+    // bumped by every call that changes the clause DB or the assignment
+    // table; a plain sweep whose hit buffer overflowed is re-run only if
+    // nothing changed since it was queued (else the re-run would check a
+    // different DB or table than the caller asked for)
+    uint64_t state_version = 0, sweep_state_version = 0;
     int64_t total_clauses = 0, total_lits = 0;
     // Cap on the device bytes of the clause buckets (their capacities), from
     // GPUSHARESAT_DB_LIMIT_BYTES at creation; 0 = only the device's memory.
@@ -634,6 +639,7 @@ static int db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32
     e->total_clauses += n;
     e->total_lits += total;
     e->max_lit = std::max<int64_t>(e->max_lit, mx);
+    e->state_version++;
     return GSS_OK;
 }
 
@@ -737,6 +743,7 @@ int gss::db_compact_device(gss_engine *e, const uint8_t *d_keep, int64_t ntags) 
     HIP_TRY(launch_compact_move(e->d_cref.p, nb, N, e->d_flags.p, e->d_pos.p, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->rows_version++;
+    e->state_version++;
     for (int i = 0; i < nb; i++) {
         Bucket &b = e->buckets[lens[i]];
         (void)hipFreeAsync(b.d, e->stream);
@@ -800,6 +807,18 @@ int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
     Bucket &b = e->buckets[len];
     const int64_t old = b.count;
     e->rows_version++;
+    e->state_version++;
+    if (n == 0) {
+        // emptying a bucket allocates nothing (out-of-memory recovery relies
+        // on it): its array goes back to the pool in stream order
+        if (b.d) HIP_TRY(hipFreeAsync(b.d, e->stream));
+        b.d = nullptr;
+        b.cap = 0;
+        b.count = 0;
+        e->total_clauses -= old;
+        e->total_lits -= old * len;
+        return GSS_OK;
+    }
     b.count = 0;  // nothing to preserve when growing
     if ((rc = ensure_bucket_cap(e, len, n, true))) {
         b.count = old;
@@ -852,6 +871,7 @@ int gss_db_clear(gss_engine *e) {
     if (rc) return rc;
     for (Bucket &b : e->buckets) b.count = 0;
     e->rows_version++;
+    e->state_version++;
     e->total_clauses = e->total_lits = 0;
     e->max_lit = -1;
     return GSS_OK;
@@ -903,6 +923,7 @@ int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
         if ((rc = e->table.reserve(n))) return rc;
     }
     HIP_TRY(launch_fill_u64(e->table.p, (int64_t)n, ~0ull, e->stream));  // all undefined
+    e->state_version++;
     e->vars = var_count;
     e->words = words;
     if ((rc = write_sentinels(e))) return rc;
@@ -921,6 +942,7 @@ int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
     const size_t n = table_rows(var_count) * e->words;
     if ((rc = e->table.reserve(n, true, e->stream))) return rc;
     HIP_TRY(launch_fill_u64(e->table.p + old_n, (int64_t)(n - old_n), ~0ull, e->stream));
+    e->state_version++;
     e->vars = var_count;
     if ((rc = write_sentinels(e))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -935,6 +957,7 @@ int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_t
     const int64_t V = e->vars;
     // stage per block of variables
     const int64_t blk = 1 << 16;
+    e->state_version++;
     Pinned<uint64_t> st;
     if ((rc = st.reserve((size_t)blk * 2 * W))) return rc;
     for (int64_t v0 = 0; v0 < V; v0 += blk) {
@@ -1000,6 +1023,7 @@ int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups) {
     if ((rc = upload_updates(e, n, ups))) return rc;
     HIP_TRY(launch_assig_apply(e->d_ups.p, n, e->table.p, e->words, e->stream));
     e->applied = n;
+    e->state_version++;
     return GSS_OK;
 }
 
@@ -1018,6 +1042,7 @@ int gss_assig_roll_forward_applied(gss_engine *e, const int8_t *last_pos) {
     HIP_TRY(launch_assig_roll(e->d_ups.p, e->applied, e->table.p, e->words, e->d_lastpos,
                               e->stream));
     e->applied = 0;
+    e->state_version++;
     return GSS_OK;
 }
 
@@ -1034,6 +1059,7 @@ int gss_assig_roll_forward(gss_engine *e, int64_t n, const gss_var_update *ups,
     HIP_TRY(hipMemcpyAsync(e->d_lastpos, e->h_lastpos.p, e->words, hipMemcpyHostToDevice,
                            e->stream));
     HIP_TRY(launch_assig_roll(e->d_ups.p, n, e->table.p, e->words, e->d_lastpos, e->stream));
+    e->state_version++;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }
@@ -1069,7 +1095,10 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
         const size_t len = e->tune_schedule == 2 ? i : nlen - i;
         const Bucket &b = e->buckets[len];
         if (!b.count) continue;
-        if (b.count > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
+        // the kernel does row arithmetic on the count as a signed 32-bit value
+        if (b.count > 0x7FFFFFFFll)
+            return set_err(GSS_E_INVAL, "bucket %zu holds %lld rows (at most 2^31 - 1)", len,
+                           (long long)b.count);
         BucketDesc d;
         d.lits = b.d;
         d.len = (uint32_t)len;
@@ -1132,6 +1161,7 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     if (chunks > e->chunk_cap && (rc = reserve_chunks(e, chunks))) return rc;
     if ((rc = launch_current_sweep(e))) return rc;
     e->sweep_rows_version = e->rows_version;
+    e->sweep_state_version = e->state_version;
     e->pending = true;
     e->waited = false;
     e->have_sweep = true;
@@ -1149,7 +1179,16 @@ int gss_sweep_wait(gss_engine *e, int64_t *n_hits) {
             e->last_chunks = used;
             break;
         }
-        // The buffer was too small: grow to what the claims asked for, re-run.
+        // The buffer was too small: grow to what the claims asked for, re-run
+        // -- on the same DB and table only.
+        if (e->state_version != e->sweep_state_version) {
+            e->pending = false;
+            return set_err(GSS_E_NOMEM,
+                           "the sweep's hit buffer overflowed (%u chunks claimed, %u held) and the "
+                           "clause DB or assignment table changed after the sweep was queued, so "
+                           "it cannot be re-run; wait for a sweep before changing them",
+                           used, e->chunk_cap);
+        }
         if ((rc = reserve_chunks(e, (uint64_t)used + used / 8 + 16))) {
             e->pending = false;
             return rc;
@@ -1584,7 +1623,9 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     int64_t tasks = 0;
     for (size_t len = 1; len < nlen; len++) {
         const int64_t c = rows_after(len);
-        if (c > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "bucket too large");
+        if (c > 0x7FFFFFFFll)
+            return set_err(GSS_E_INVAL, "bucket %zu would hold %lld rows (at most 2^31 - 1)", len,
+                           (long long)c);
         if (c) nb++;
         tasks += (c + tc - 1) / tc;
     }
@@ -1713,6 +1754,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         h.valid[w] = w < W ? valid[w] : 0;
         h.last_pos[w] = (w < W && last_pos) ? last_pos[w] : -1;
     }
+    e->state_version++;
     HIP_TRY(hipEventRecord(sl.t0, e->stream));
     HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));

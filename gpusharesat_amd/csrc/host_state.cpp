@@ -37,6 +37,7 @@ void SolverWindow::set_var_count(int n) {
     if (n > (int)last_val_.size()) {
         last_val_.resize(n, kValUndef);
         update_pos_.resize(n, -1);
+        var_count_.store(n, std::memory_order_release);
     }
 }
 
@@ -232,6 +233,15 @@ std::vector<int> ClauseMirror::reduce(int max_lbd) {
         }
     }
     return changed;
+}
+
+void ClauseMirror::drop_bucket(int len) {
+    if (len < 1 || len >= (int)meta_.size()) return;
+    const long n = (long)meta_[len].size();
+    count_ -= n;
+    len_sum_ -= n * len;
+    meta_[len].clear();
+    lits_[len].clear();
 }
 
 /* ---- ReportQueue (gpuShareLib/Reported.cu:97-204) ------------------------- */

@@ -161,9 +161,11 @@ bool EngineClauseSharer::start_run(InFlight &run) {
             // out of device memory: the DB is halved and re-uploaded
             // (GpuRunner.cu:243-260), but only after the run in flight has
             // been delivered: its hits name clauses by row, and the
-            // reduction moves rows (GpuRunner.cu:251-253)
+            // reduction moves rows (GpuRunner.cu:251-253).  The device
+            // holds none of this batch, which the mirror already has.
             ran_out_of_memory_ = true;
             reduce_pending_ = true;
+            dirty_.insert(b.lens.begin(), b.lens.end());
             return false;
         }
         check_rc(rc, "gss_db_append");
@@ -200,20 +202,27 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     run.swept = false;
     for (uint64_t v : valid) run.swept = run.swept || v != 0;
     if (run.swept) check_rc(gss_sweep_async(engine_, valid.data()), "gss_sweep_async");
-    check_rc(gss_assig_roll_forward_applied(engine_, run.last_pos.data()),
-             "gss_assig_roll_forward_applied");
+    // the roll-forward is queued once the sweep has been waited for (collect):
+    // a sweep whose hit buffer overflows is re-run on the table it was
+    // queued with
     run.clause_count = clauses_.clause_count();
     run.active = true;
     return true;
 }
 
 void EngineClauseSharer::collect(InFlight &run, std::vector<gss_hit> &hits) {
+    auto roll = [&] {
+        check_rc(gss_assig_roll_forward_applied(engine_, run.last_pos.data()),
+                 "gss_assig_roll_forward_applied");
+    };
     if (!run.swept) {
+        roll();
         hits.clear();
         return;
     }
     int64_t n = 0;
     check_rc(gss_sweep_wait(engine_, &n), "gss_sweep_wait");
+    roll();
     hits.resize((size_t)n);
     if (n) check_rc(gss_hits_copy(engine_, hits.data(), n, &n), "gss_hits_copy");
     if (opts_.quickProf) {
@@ -286,22 +295,40 @@ void EngineClauseSharer::reduce_db_now() {
     added_at_last_reduce_ = clauses_.added();
     global_stats_[gpuReduceDbs]++;
     log(2, "c Reducing gpu clause db, clause count is " + std::to_string(clauses_.clause_count()));
-    auto upload = [&](int len) {
-        const auto &lits = clauses_.bucket_lits(len);
-        check_rc(gss_db_set_bucket(engine_, len, (int64_t)(lits.size() / len), lits.data()),
-                 "gss_db_set_bucket");
-    };
-    // the reduced buckets first (they shrink), then every bucket the device
-    // does not hold as the mirror does: after an append that ran out of
-    // memory the mirror already has that batch's clauses, in buckets the
-    // reduction may not touch (lengths 1 and 2 are always kept)
-    const std::vector<int> changed = clauses_.reduce(limits_.max_clause_size);
-    for (int len : changed) upload(len);
-    for (int len = 1; len <= clauses_.max_len(); len++) {
-        if (std::find(changed.begin(), changed.end(), len) != changed.end()) continue;
-        int64_t on_device = 0;
-        check_rc(gss_db_bucket_count(engine_, len, &on_device), "gss_db_bucket_count");
-        if (on_device != clauses_.rows(len)) upload(len);
+    // Reduce, then upload every bucket whose device rows differ from the
+    // mirror's: the reduced ones first (they shrink and give memory back),
+    // then those a failed append left behind (the mirror already holds that
+    // batch, also in the lengths 1 and 2 the reduction never touches).  As
+    // the reference (Clauses.cu reduceDb: reduce and retry until the copy
+    // fits), an upload that still does not fit reduces again; when nothing
+    // is left to reduce, the buckets the device cannot hold are emptied on
+    // both sides, so device and mirror always agree.
+    for (;;) {
+        const std::vector<int> changed = clauses_.reduce(limits_.max_clause_size);
+        std::vector<int> order(changed);
+        for (int len : dirty_)
+            if (std::find(changed.begin(), changed.end(), len) == changed.end()) order.push_back(len);
+        dirty_.insert(changed.begin(), changed.end());
+        int rc = GSS_OK;
+        for (int len : order) {
+            const auto &lits = clauses_.bucket_lits(len);
+            rc = gss_db_set_bucket(engine_, len, (int64_t)(lits.size() / len), lits.data());
+            if (rc == GSS_E_NOMEM) break;
+            check_rc(rc, "gss_db_set_bucket");
+            dirty_.erase(len);
+        }
+        if (rc == GSS_OK) break;
+        ran_out_of_memory_ = true;
+        log(1, "c GPU memory still short after a reduction: " + std::string(gss_last_error()));
+        if (changed.empty()) {
+            const std::vector<int> left(dirty_.begin(), dirty_.end());
+            for (int len : left) {
+                clauses_.drop_bucket(len);
+                check_rc(gss_db_set_bucket(engine_, len, 0, nullptr), "gss_db_set_bucket");
+                dirty_.erase(len);
+            }
+            break;
+        }
     }
     global_stats_[gpuClauses] = (unsigned long)clauses_.clause_count();
     global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();

@@ -52,7 +52,8 @@ public:
     long done();
     // values of variables [0, n): those the window never saw are undefined
     void current_assignment(uint8_t *out, int n) const;
-    int var_count() const { return (int)last_val_.size(); }
+    // readable without the lock (the GPU thread sizes the table from it)
+    int var_count() const { return var_count_.load(std::memory_order_acquire); }
 
     struct Taken {
         long start_id = 0;
@@ -71,6 +72,7 @@ private:
     long current_id_ = 0;
     long first_id_ = 0;
     std::vector<uint8_t> last_val_;
+    std::atomic<int> var_count_{0};  // last_val_.size(), published after a resize
     std::vector<int> update_pos_;
     std::vector<gss_var_update> updates_;
 };
@@ -99,6 +101,8 @@ public:
     void bump(int len, uint32_t row);
     // keeps clauses per the reference policy; returns the buckets changed
     std::vector<int> reduce(int max_lbd);
+    // removes every clause of bucket len
+    void drop_bucket(int len);
     float approx_nth_activity(int min_lbd, int max_lbd, long n) const;
 
     long clause_count() const { return count_; }
@@ -262,6 +266,7 @@ private:
     long added_at_last_reduce_ = 0;
     bool ran_out_of_memory_ = false;
     bool reduce_pending_ = false;  // out of memory: reduce once the run in flight is delivered
+    std::set<int> dirty_;          // buckets whose device rows differ from the mirror's
 };
 
 }  // namespace GpuShare

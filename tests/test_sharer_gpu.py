@@ -495,6 +495,71 @@ def test_out_of_device_memory_halves_the_db(tmp_path, monkeypatch):
         assert got == exp
 
 
+def _db_matches_cnf(s, tmp_path, V, rng):
+    """The device holds exactly the clauses of the written CNF: one
+    assignment's reports equal the CNF clauses it makes falsified or unit."""
+    from collections import Counter
+    p = tmp_path / "db.cnf"
+    s.writeClausesInCnf(str(p))
+    lines = p.read_text().splitlines()
+    db = [[mkLit(abs(int(x)) - 1, int(x) < 0) for x in line.split()[:-1]] for line in lines[1:]]
+    assig = {int(v): bool(rng.integers(2)) for v in rng.choice(V, size=V // 2, replace=False)}
+    s.unsetSolverValues(0, [mkLit(v) for v in range(V)])
+    assert s.trySetSolverValues(0, [mkLit(v, not val) for v, val in assig.items()])
+    assert s.trySendAssignment(0) >= 0
+    s.execute()
+    got = Counter(tuple(sorted(l)) for l, _ in s.popAll(0))
+    exp = Counter(tuple(sorted(c)) for c in db if _useful(c, assig))
+    assert got == exp
+    return len(db)
+
+
+def test_out_of_memory_reduces_until_the_db_fits(tmp_path, monkeypatch):
+    """An append far past the DB byte cap: one halving still does not fit, so
+    the reduction repeats (as the reference's reduceDb loop) until the upload
+    fits; gpuRun carries on and the device holds exactly the mirror."""
+    monkeypatch.setenv("GPUSHARESAT_DB_LIMIT_BYTES", str(128 << 10))
+    rng = np.random.default_rng(8)
+    V = 500
+    with sharer(V, 1) as s:
+        for _ in range(16000):  # ~320 KB of literals of lengths 3-7
+            vs = rng.choice(V, size=int(rng.integers(3, 8)), replace=False)
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+        s.gpuRun()
+        assert s.hasRunOutOfGpuMemoryOnce()
+        kept = s.getGlobalStat("gpuClauses")
+        assert 0 < kept <= 16000 // 4 + 200  # at least two halvings
+        assert _db_matches_cnf(s, tmp_path, V, rng) == kept
+
+
+def test_out_of_memory_with_nothing_left_to_reduce(tmp_path, monkeypatch):
+    """Binary clauses are never reduced (Clauses.cu:255): when they alone do
+    not fit, the buckets the device cannot hold are emptied on both sides and
+    the sharer keeps working with device and mirror equal."""
+    monkeypatch.setenv("GPUSHARESAT_DB_LIMIT_BYTES", str(16 << 10))
+    rng = np.random.default_rng(9)
+    V = 300
+    def add(n, size):
+        for _ in range(n):
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2)))
+                             for v in rng.choice(V, size=size, replace=False)])
+
+    with sharer(V, 1) as s:
+        add(100, 1)
+        s.gpuRun()
+        assert not s.hasRunOutOfGpuMemoryOnce()
+        add(4000, 2)  # 32 KB of binary clauses
+        s.gpuRun()
+        assert s.hasRunOutOfGpuMemoryOnce()
+        # the binary bucket could not be held; the unary one is untouched
+        assert s.getGlobalStat("gpuClauses") == 100
+        assert _db_matches_cnf(s, tmp_path, V, rng) == 100
+        add(50, 2)  # fits again
+        s.gpuRun()
+        assert s.getGlobalStat("gpuClauses") == 150
+        assert _db_matches_cnf(s, tmp_path, V, rng) == 150
+
+
 def test_rejected_clauses_and_bad_ids():
     from gpusharesat_amd._lib import GssError
     with sharer(4, 1) as s:

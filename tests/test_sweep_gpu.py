@@ -240,6 +240,33 @@ def test_hit_buffer_regrowth_drops_nothing(engine_factory):
     assert_same(got, exp)
 
 
+def test_overflow_rerun_refused_after_the_table_changed(engine_factory):
+    """A sweep whose hit buffer overflows is re-run by the wait only when
+    neither the DB nor the table changed after it was queued; with an update
+    queued behind it, the wait fails instead of checking the updated table.
+    Without a change the re-run's records equal the oracle's."""
+    from gpusharesat_amd._lib import GssError
+    inst = workloads.plain_64(seed=9, scale=0.2)
+    e = engine_factory()
+    e.set_tuning(TUNE_MAX_HIT_CHUNKS, 1)
+    rows = e.append(inst.lits, inst.lens)
+    e.reset_assignments(inst.var_count, inst.words)
+    e.upload(inst.is_def, inst.is_true)
+    e.sweep_async(inst.valid)
+    ups = np.zeros(1, VAR_UPDATE_DTYPE)
+    ups["var"], ups["word"], ups["is_def"] = 0, 0, ALL
+    e.apply(ups)  # queued behind the sweep: the table changes
+    with pytest.raises(GssError, match="cannot be re-run"):
+        e.sweep_wait()
+    # the same sweep with nothing queued behind it re-runs and drops nothing
+    d, t = e.download()
+    e.sweep_async(inst.valid)
+    n = e.sweep_wait()
+    exp = orc.sweep(inst.offsets, inst.lits, d, t, inst.valid)
+    assert n == exp.size > 2000
+    assert_same(gpu_records(e, inst.lens, rows), exp)
+
+
 def test_tuning_never_changes_results(engine_factory):
     inst = workloads.plain_64(seed=6, scale=0.3)
     e = engine_factory()
