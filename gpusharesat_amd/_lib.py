@@ -57,6 +57,7 @@ SIGNATURES: dict[str, list] = {
     "gss_sweep_wait": [c_p, c_p],
     "gss_sweep": [c_p, c_p, c_p],
     "gss_hits_copy": [c_p, c_p, c_i64, c_p],
+    "gss_hits_gather": [c_p, c_p, c_i64, c_p],
     "gss_sweep_kernel_ms": [c_p, c_p],
     "gss_sweep_kernel_stats": [c_p, c_p, c_p, c_int],
     "gss_set_tuning": [c_p, c_int, c_int],

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -34,7 +35,23 @@
 
 using namespace gss;
 
+// Scratch of the no-allocation record gather (gss_hits_gather): the record
+// offset of every hit chunk the last sweep claimed, then the total.  One copy
+// per device, in this library's code object, held until the process ends.
+// Host code never uses the array's name as an address: every kernel argument
+// takes the per-device address that hipGetSymbolAddress returned for it
+// (scratch_address below, called once per device by gss_engine_create with
+// that device current).  Every call on one device shares the array, and
+// k_chunk_offsets rewrites entries [0, chunks] each time, so nothing in it
+// needs clearing; two gathers that overlap on one device race.
+static constexpr uint32_t kGatherMaxChunks = 262144;  // 2 MiB + 8 B per device
+static __device__ __align__(256) uint64_t scratch[kGatherMaxChunks + 1];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t *g_scratch_ptr[kMaxDevices] = {};
 
 // Clauses of at most this many literals read both rows of each literal in one
 // pass (see kernels.hip).
@@ -67,6 +84,24 @@ int set_err(int code, const char *fmt, ...) {
             return set_err(_e == hipErrorOutOfMemory ? GSS_E_NOMEM : GSS_E_HIP, "%s: %s", \
                            #expr, hipGetErrorString(_e));                                  \
     } while (0)
+
+// Device address of `scratch` on `device`, which must be current: taken once
+// per device, then reused.
+int scratch_address(int device, uint64_t **out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices)
+        return set_err(GSS_E_INVAL, "device %d: at most %d devices", device, kMaxDevices);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (he != hipSuccess || !p)
+            return set_err(GSS_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                           hipGetErrorString(he));
+        g_scratch_ptr[device] = (uint64_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return GSS_OK;
+}
 
 struct Bucket {
     uint32_t *d = nullptr;
@@ -250,6 +285,7 @@ struct gss_engine {
     const BucketDesc *desc_up_at = nullptr;
     std::vector<uint64_t> valid_up;
     Pinned<uint32_t> h_count;  // [0] = chunks claimed, [1], [2] = records (low, high word)
+    uint64_t *gather_scratch = nullptr;  // this device's address of `scratch`
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     uint32_t *d_alloc = nullptr;
@@ -444,6 +480,7 @@ int gss_engine_create(int device, gss_engine **out) {
     };
     if (hipDeviceGetAttribute(&e->cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "cannot query CU count"));
+    if ((rc = scratch_address(device, &e->gather_scratch))) return fail(rc);
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(GSS_E_HIP, "stream creation failed"));
     if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
@@ -1234,6 +1271,30 @@ int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
                                    e->d_hits.p, e->d_dense.p, e->d_dense.cap, e->stream));
     HIP_TRY(hipMemcpyAsync(out, e->d_dense.p, (size_t)want * sizeof(gss_hit),
                            hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_hits_gather(gss_engine *e, gss_hit *d_out, int64_t cap, int64_t *n) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!e->waited) return set_err(GSS_E_STATE, "no waited sweep");
+    if (n) *n = e->last_hits;
+    if (!d_out && cap == 0) return GSS_OK;  // the count only
+    if (!d_out || cap < 0) return set_err(GSS_E_INVAL, "null output");
+    if (cap < e->last_hits)
+        return set_err(GSS_E_INVAL, "the sweep has %lld records; the output holds %lld",
+                       (long long)e->last_hits, (long long)cap);
+    if (e->last_chunks > kGatherMaxChunks)
+        return set_err(GSS_E_INVAL,
+                       "the sweep claimed %u hit chunks; gss_hits_gather handles at most %u "
+                       "(gss_hits_copy has no such limit)",
+                       e->last_chunks, kGatherMaxChunks);
+    if (!e->last_hits) return GSS_OK;
+    // the same two kernels as gss_hits_copy: chunk offsets into the scratch,
+    // records straight into the caller's array
+    HIP_TRY(launch_compact_records(e->d_alloc, e->d_fills.p, e->chunk_cap, e->gather_scratch,
+                                   e->d_hits.p, d_out, (uint64_t)cap, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }

@@ -187,6 +187,20 @@ int gss_sweep(gss_engine *e, const uint64_t *valid, int64_t *n_hits);
 /* Copy the hit records of the last (waited) sweep to host: up to cap records,
  * *n = total.  Record order is unspecified. */
 int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
+/* No-allocation form of gss_hits_copy (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Writes the records of
+ * the last waited sweep densely into d_out, caller DEVICE memory of cap
+ * records (record order unspecified, as gss_hits_copy), and *n = their
+ * number; with d_out NULL and cap 0 only *n is set.  Calls no hipMalloc,
+ * hipFree or other allocating routine: its only working memory is a static
+ * per-device array of 262,145 uint64 (2 MiB, held by every device the library
+ * runs on until the process ends) for the chunk offsets, so a sweep that
+ * claimed more than 262,144 hit chunks of 1024 records is refused with
+ * GSS_E_INVAL, as is cap below the record count.  Synchronous.  Every call on
+ * one device shares that array: calls that overlap on one device (from two
+ * threads or two engines) race. */
+int gss_hits_gather(gss_engine *e, gss_hit *d_out, int64_t cap, int64_t *n);
 /* Duration in ms of the sweep kernel of the last waited sweep, measured with
  * HIP events recorded on the engine's stream around the kernel launch. */
 int gss_sweep_kernel_ms(gss_engine *e, float *ms);
