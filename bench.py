@@ -18,11 +18,15 @@ before timing starts.  Prints one JSON line.
                  beside it the kernel's traffic past the L2 (two PMC passes)
                  as a rate, and the device's random 128-byte line gather rate
                  on a table of the same size, measured in the same run
-  cpu_baseline   the CPU oracle (bit-sliced rule, all host threads) on a
-                 bounded sample of the same instance
-  latency_ms     after the timed region: submit -> every hit record of the
-                 sweep in pinned host memory, one sweep in flight, through
-                 the step path (graph replay, device compaction, PCIe copy)
+  cpu_baseline   the CPU oracle (bit-sliced rule) on the whole instance, with
+                 the threads of this process's CPU share (the box's share
+                 for one GPU); nproc and the CPU model are recorded beside it
+  latency_ms     after the timed region: submit -> every hit of the sweep in
+                 pinned host memory as the solvers receive it (clause and
+                 solver word, 8 bytes per record), one sweep in flight,
+                 through the step path (graph replay, device compaction,
+                 PCIe copy); latency_full_records_ms the same with the
+                 24-byte records that also carry the masks
 
 --workload batched-16 measures the batched form instead: 16 solver streams
 over 1M vars, each step inserting Poisson(16 x 2800) clauses of Pareto(1.5)
@@ -49,56 +53,105 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = ("lit×assig checks/s + %HBM roofline @4M cl×1024 asg; p99 report latency")
-# HBM bytes per sweep launch of the default kernel on this workload, from two
-# rocprofv3 PMC passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, KiB, the
-# factors calibrated on the clause-upload copy kernel of known size); the
-# commands and raw figures are in profiles/k_sweep_sweep1024.md.  None when
-# the instance is not the headline one.
-MEASURED_TRAFFIC_KIB = {("k_sweep<16,1,4,true>", 1.0): 2 * 2_069_510 + 453_794}
+# HBM bytes per sweep launch of the default kernel on the headline instance
+# come from two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) recorded in this
+# profile, together with the SHA-256 of the kernel source they were measured
+# on; a profile of another source is not used (traffic is then null).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_sweep_sweep1024.md")
+KERNEL_SOURCE = os.path.join(ROOT, "gpusharesat_amd", "csrc", "kernels.hip")
+
+
+def measured_traffic(kernel: str) -> dict:
+    """Traffic of `kernel` on the headline instance from the committed
+    profile, if that profile was taken on the current kernel source."""
+    import hashlib
+    import re
+    src = hashlib.sha256(open(KERNEL_SOURCE, "rb").read()).hexdigest()
+    out = {"bytes": None, "profile": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+           "kernel_source_sha256": src}
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        out["note"] = "no traffic profile"
+        return out
+    m = re.search(r"traffic-record: kernel=(\S+) source_sha256=([0-9a-f]{64}) "
+                  r"fetch_kib=(\d+) fetch_factor=([\d.]+) write_kib=(\d+) "
+                  r"write_factor=([\d.]+) date=(\S+)", text)
+    if not m:
+        out["note"] = "the profile holds no traffic record"
+        return out
+    k, sha, fk, ff, wk, wf, date = m.groups()
+    out.update({"profiled_kernel": k, "profiled_source_sha256": sha, "date": date})
+    if k != kernel or sha != src:
+        out["note"] = "the profile was taken on another kernel or kernel source"
+        return out
+    out["bytes"] = int((int(fk) * float(ff) + int(wk) * float(wf)) * 1024)
+    return out
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(inst, target_s: float = 12.0) -> dict:
-    """Times the oracle (a port of the reference rule) on the host cores."""
+def cpu_share() -> tuple[int, dict]:
+    """Threads of this process's CPU share, and what the host reports.  The
+    share is the CPUs this process may run on, bounded by OMP_NUM_THREADS
+    where the environment sets it (the box gives one GPU's process 16)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    share = affinity
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share = min(share, int(env))
+    info = {"nproc": nproc, "affinity_cpus": affinity, "omp_num_threads": env or None}
+    try:
+        fields = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "vendor_id", "cpu family", "model", "stepping",
+                         "cpu MHz", "cache size") and k not in fields:
+                    fields[k] = v
+        info["cpu_model"] = fields.get("model name", "")
+        info["cpuinfo"] = fields
+    except OSError:
+        pass
+    return max(1, share), info
+
+
+def cpu_baseline(inst, target_s: float = 10.0) -> dict:
+    """Times the oracle (a port of the reference rule, bit-sliced, 64
+    assignments per word) on the whole instance with every thread of the
+    process's CPU share, repeated for about target_s; one thread is timed on
+    the first 100k clauses."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    nthreads, info = cpu_share()
     off = inst.offsets
-    # bounded sample: a prefix of the clauses, all 1024 assignments
-    n_sample = min(inst.lens.size, 1_000_000)
-    sub_off = off[: n_sample + 1]
-    lits = inst.lits[: sub_off[-1]]
-    checks_per_run = int(sub_off[-1]) * inst.assignments
+    checks_per_run = int(off[-1]) * inst.assignments
     t0 = time.perf_counter()
     runs = 0
     while True:
-        orc.sweep(sub_off, lits, inst.is_def, inst.is_true, inst.valid, nthreads=nthreads)
+        orc.sweep(off, inst.lits, inst.is_def, inst.is_true, inst.valid, nthreads=nthreads)
         runs += 1
         el = time.perf_counter() - t0
         if el >= target_s or runs >= 50:
             break
     all_cores = checks_per_run * runs / el
-    # one thread, smaller prefix
-    n1 = min(n_sample, 100_000)
+    n1 = min(inst.lens.size, 100_000)
     t0 = time.perf_counter()
     orc.sweep(off[: n1 + 1], inst.lits[: off[n1]], inst.is_def, inst.is_true, inst.valid,
               nthreads=1)
     one = int(off[n1]) * inst.assignments / (time.perf_counter() - t0)
-    model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": all_cores, "unit": "checks/s", "cores": nthreads, "kind": "port",
-            "sample": f"first {n_sample} clauses ({int(sub_off[-1])} literals) x "
-                      f"{inst.assignments} assignments, {runs} runs, {nthreads} threads",
-            "value_1core": one, "cpu_model": model}
+            "sample": f"the whole instance ({inst.lens.size} clauses, {int(off[-1])} literals) "
+                      f"x {inst.assignments} assignments, {runs} runs in {el:.1f} s, "
+                      f"{nthreads} threads",
+            "value_1core": one, "sample_1core": f"first {n1} clauses, 1 thread",
+            "seconds_per_sweep": el / runs, **info}
 
 
 def main() -> int:
@@ -194,32 +247,54 @@ def main() -> int:
     achieved = alg_bytes / k_avg_s / 1e9
     log(f"{ms_per_step:.3f} ms/step, kernel {k_avg_s * 1e3:.3f} ms, {hits} hit records, "
         f"{achieved:.0f} GB/s algorithmic")
-    # Report latency (outside the timed region): submit -> every record of the
+    # Report latency (outside the timed region): submit -> every hit of the
     # sweep in pinned host memory, one sweep in flight, through the step path
-    # (graph replay: sweep, compaction, copy over PCIe).
-    lat = []
-    for k in range(args.lat_steps + 2):
-        t_sub = time.perf_counter()
-        e.step_enqueue(valid=valid)
-        n_step, _ = e.step_wait()
-        view = e.step_hits_view()
-        t_rep = time.perf_counter()
-        if n_step != hits or view.size != hits:
-            raise RuntimeError(f"step path gave {n_step} records, the sweep {hits}")
-        if k >= 2:  # the first replays capture the graph and size the buffers
-            lat.append((t_rep - t_sub) * 1e3)
-    latency = None
-    if lat:
-        latency = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                   "p999": float(np.percentile(lat, 99.9)),
-                   "max": float(max(lat)), "sweeps": len(lat),
-                   "what": "submit -> all hit records in pinned host memory"}
-        log(f"report latency p50 {latency['p50']:.2f} ms, p99 {latency['p99']:.2f} ms "
-            f"({hits * 24 / 1e6:.0f} MB of records per sweep)")
+    # (graph replay: sweep, compaction, copy over PCIe); first with the
+    # records the solvers receive (clause and solver word), then with the
+    # full records.
+    from gpusharesat_amd import serving as sv
+    first = None
+
+    def latency_of(records: int) -> dict | None:
+        nonlocal first
+        lat = []
+        for k in range(args.lat_steps + 2):
+            t_sub = time.perf_counter()
+            sv.submit(e, valid, records=records)
+            n_step, _ = e.step_wait()
+            view = e.step_ids_view() if records == sv.RECORDS_IDS else e.step_hits_view()
+            t_rep = time.perf_counter()
+            if n_step != hits or view.size != hits:
+                raise RuntimeError(f"step path gave {n_step} records, the sweep {hits}")
+            if records == sv.RECORDS_IDS and first is None:
+                first = view.copy()
+            if k >= 2:  # the first replays capture the graph and size the buffers
+                lat.append((t_rep - t_sub) * 1e3)
+        if not lat:
+            return None
+        size = 8 if records == sv.RECORDS_IDS else 24
+        return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                "p999": float(np.percentile(lat, 99.9)), "max": float(max(lat)),
+                "sweeps": len(lat), "bytes_per_record": size,
+                "what": "submit -> every hit record of the sweep in pinned host memory"}
+
+    latency = latency_of(sv.RECORDS_IDS)
+    latency_full = latency_of(sv.RECORDS_FULL)
+    if latency:
+        log(f"report latency (8-byte records) p50 {latency['p50']:.2f} ms, p99 "
+            f"{latency['p99']:.2f} ms; 24-byte records p50 {latency_full['p50']:.2f} ms, p99 "
+            f"{latency_full['p99']:.2f} ms ({hits * 8 / 1e6:.0f} / {hits * 24 / 1e6:.0f} MB)")
+    # the ids the solvers receive are the first word of the full records
+    full = e.step_hits_view()
+    if first is not None and not np.array_equal(
+            np.sort(first.view(np.uint64)), np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0])):
+        raise RuntimeError("id records differ from the full records' first words")
     # The device's own ceiling for this access pattern: random 128-byte line
     # gathers (independent, 4 in flight per lane) from a table of the same size.
-    traffic = (MEASURED_TRAFFIC_KIB[("k_sweep<16,1,4,true>", args.scale)] * 1024
-               if ("k_sweep<16,1,4,true>", args.scale) in MEASURED_TRAFFIC_KIB else None)
+    kernel_name = "k_sweep<16,1,4,true>"
+    tr = measured_traffic(kernel_name) if args.scale == 1.0 else {
+        "bytes": None, "note": "not the headline instance"}
+    traffic = tr["bytes"]
     table_bytes = (2 * inst.var_count + 2) * W * 8
     gather = _gather_ceiling((table_bytes + 127) // 128 * 128)
     log(f"random-line gather ceiling on a {table_bytes / 1e6:.0f} MB table: {gather:.0f} GB/s"
@@ -258,6 +333,7 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_unit": "bytes per launch",
+            "traffic_source": tr,
             "traffic_gbps": traffic / k_avg_s / 1e9 if traffic else None,
             "gather_ceiling_gbps": gather,
             "kernel": "k_sweep<16,1,4,true>",
@@ -266,6 +342,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "latency_ms": latency,
+        "latency_full_records_ms": latency_full,
     }
     print(json.dumps(out), flush=True)
     e.close()

@@ -66,6 +66,8 @@ SIGNATURES: dict[str, list] = {
     "gss_step_wait": [c_p, c_p, c_p],
     "gss_step_hits_copy": [c_p, c_p, c_i64, c_p],
     "gss_step_hits_map": [c_p, c_p, c_p],
+    "gss_step_ids_map": [c_p, c_p, c_p],
+    "gss_db_reserve": [c_p, c_i64, c_p],
     "gss_step_submit": [c_p, c_p],
     "gss_bases_reset": [c_p, c_int],
     # include/gss_residency.h

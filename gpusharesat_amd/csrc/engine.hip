@@ -199,7 +199,7 @@ struct StepSlot {
     DevBuf<gss_var_patch> d_patches;
     DevBuf<gss_base_update> d_bups;
     DevBuf<gss_word_load> d_loads;
-    DevBuf<gss_hit> d_hits;
+    DevBuf<uint64_t> d_hits;     // chunk_cap chunks of kHitChunk records of rec_words words
     DevBuf<uint32_t> d_fills;
     DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
     DevBuf<uint64_t> d_offs;     // record offset of each chunk (the graph's compaction)
@@ -226,6 +226,9 @@ struct StepSlot {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool busy = false;
     bool results = false;  // h_out / d_hits hold the results of the last waited step
+    int rec_words = 3;     // record form of the slot's last submitted step (3: gss_hit,
+                           // 1: gss_hit_id)
+    int res_words = 3;     // the same, of its last waited step (the results)
     std::vector<const void *> key;  // everything baked into the captured graph
 };
 
@@ -436,6 +439,7 @@ int launch_current_sweep(gss_engine *e) {
         a.alloc = e->d_alloc;
         a.chunk_cap = e->chunk_cap;
         a.fuse_len = fuse_of(e);
+        a.rec_words = 3;
         HIP_TRY(launch_sweep(a, e->words, e->tune_unroll, e->blocks, e->stream));
     }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
@@ -1151,6 +1155,7 @@ int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     int active_words = 0;
     for (int w = 0; w < W; w++)
         if (valid[w]) active_words++;
+    if (!active_words) e->ntasks = 0;  // every clause starts dead: nothing to check
     // Uploads only when what the device holds differs (the previous sweep has
     // been waited for, so its copies out of the pinned staging are done).
     const bool valid_same = (int)e->valid_up.size() == W &&
@@ -1268,7 +1273,7 @@ int gss_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n) {
         (rc = e->d_offsets.reserve((size_t)e->chunk_cap + 1)))
         return rc;
     HIP_TRY(launch_compact_records(e->d_alloc, e->d_fills.p, e->chunk_cap, e->d_offsets.p,
-                                   e->d_hits.p, e->d_dense.p, e->d_dense.cap, e->stream));
+                                   e->d_hits.p, e->d_dense.p, e->d_dense.cap, 3, e->stream));
     HIP_TRY(hipMemcpyAsync(out, e->d_dense.p, (size_t)want * sizeof(gss_hit),
                            hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1294,7 +1299,7 @@ int gss_hits_gather(gss_engine *e, gss_hit *d_out, int64_t cap, int64_t *n) {
     // the same two kernels as gss_hits_copy: chunk offsets into the scratch,
     // records straight into the caller's array
     HIP_TRY(launch_compact_records(e->d_alloc, e->d_fills.p, e->chunk_cap, e->gather_scratch,
-                                   e->d_hits.p, d_out, (uint64_t)cap, e->stream));
+                                   e->d_hits.p, d_out, (uint64_t)cap, 3, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }
@@ -1434,6 +1439,7 @@ static int step_blocks(gss_engine *e, int64_t tasks) {
 // sized for this volume).
 static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
     int rc;
+    const int rw = sl.res_words;  // 8-byte words per record
     if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
     if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
     hipStream_t cs = sl.copy;
@@ -1454,9 +1460,9 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
             const uint32_t f = sl.h_out.p[1 + c];
             if (!f) continue;
             ScatterSeg &sg = sl.h_cseg.p[ns++];
-            sg.dst = (uint32_t *)(sl.d_dense.p + acc);
-            sg.src = (uint64_t)c * kHitChunk * (sizeof(gss_hit) / 4);
-            sg.count = (uint64_t)f * (sizeof(gss_hit) / 4);
+            sg.dst = (uint32_t *)((uint64_t *)sl.d_dense.p + acc * rw);
+            sg.src = (uint64_t)c * kHitChunk * rw * 2;
+            sg.count = (uint64_t)f * rw * 2;
             acc += f;
         }
         HIP_TRY(hipMemcpyAsync(sl.d_cseg.p, sl.h_cseg.p, ns * sizeof(ScatterSeg),
@@ -1464,7 +1470,7 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
         HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
         sl.dense_want = std::max<uint64_t>(sl.dense_want, 2 * (uint64_t)total);
     }
-    HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * sizeof(gss_hit),
+    HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * rw * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, cs));
     HIP_TRY(hipEventRecord(sl.ev_copied, cs));
     sl.copy_queued = true;
@@ -1505,12 +1511,13 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     a.task_clauses = tc;
     a.table = e->table.p;
     a.valid = dh->valid;
-    a.hits = sl.d_hits.p;
+    a.hits = (gss_hit *)sl.d_hits.p;
     a.fills = sl.d_fills.p;
     a.alloc = sl.d_alloc;
     a.chunk_cap = sl.chunk_cap;
     a.fuse_len = fuse;
     a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
+    a.rec_words = sl.rec_words;
     hipError_t he = hipSuccess;
     if (he == hipSuccess) he = launch_stage_in(st, cs);
     if (he == hipSuccess)
@@ -1535,7 +1542,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     // step's sweep)
     if (he == hipSuccess && sl.d_dense.cap > 0)
         he = launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_offs.p,
-                                    sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap, cs);
+                                    sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap, sl.rec_words, cs);
     hipGraph_t g = nullptr;
     hipError_t he2 = hipStreamEndCapture(cs, &g);
     if (he != hipSuccess) {
@@ -1596,6 +1603,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         n_patches < 0 || (n_patches > 0 && !desc->patches) || n_bups < 0 ||
         (n_bups > 0 && !desc->base_ups) || n_loads < 0 || (n_loads > 0 && !desc->loads))
         return set_err(GSS_E_INVAL, "bad step arguments");
+    if (desc->records != GSS_RECORDS_FULL && desc->records != GSS_RECORDS_IDS)
+        return set_err(GSS_E_INVAL, "records must be GSS_RECORDS_FULL or GSS_RECORDS_IDS");
     if (e->pending) return set_err(GSS_E_STATE, "a plain sweep is in flight");
     if (e->tagged && n > 0) return set_err(GSS_E_STATE, "tagged DB: append with gss_db_append_tagged");
     const int S = e->next_slot;
@@ -1717,15 +1726,22 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         (rc = sl.d_bups.reserve_async(sl.h_bups.cap, es)) ||
         (rc = sl.d_loads.reserve_async(sl.h_loads.cap, es)))
         return rc;
-    if (chunks > sl.chunk_cap) {
+    // the hit buffer holds chunks of records of this step's form
+    const int rw = desc->records == GSS_RECORDS_IDS ? 1 : 3;
+    auto chunk_room = [&] {
+        return (uint32_t)std::min<size_t>(
+            std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / ((size_t)kHitChunk * rw)),
+            sl.h_out.cap ? sl.h_out.cap - 1 : 0);
+    };
+    if (chunks > chunk_room()) {
         chunks *= 2;  // headroom for the DB growing over the next steps
-        if ((rc = sl.d_hits.reserve_async((size_t)chunks * kHitChunk, e->stream)) ||
+        if ((rc = sl.d_hits.reserve_async((size_t)chunks * kHitChunk * rw, e->stream)) ||
             (rc = sl.d_fills.reserve_async((size_t)chunks, e->stream)) ||
             (rc = sl.h_out.reserve(chunks + 1)))
             return rc;
-        sl.chunk_cap = (uint32_t)std::min<size_t>(
-            std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / kHitChunk), sl.h_out.cap - 1);
     }
+    sl.chunk_cap = chunk_room();
+    sl.rec_words = rw;
     if ((rc = sl.d_offs.reserve_async((size_t)sl.chunk_cap + 1, es))) return rc;
     if (sl.dense_want > sl.d_dense.cap && (rc = sl.d_dense.reserve_async(sl.dense_want, es)))
         return rc;
@@ -1744,7 +1760,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         (const void *)(intptr_t)W, (const void *)(intptr_t)e->vars,
         (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
-        (const void *)(intptr_t)sl.chunk_cap};
+        (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
         if ((rc = build_step_graph(e, sl, blocks, tc, fuse))) return rc;
@@ -1797,7 +1813,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     StepHeader &h = *sl.h_hdr.p;
     std::memset(&h, 0, sizeof h);
     h.nbuckets = k;
-    h.ntasks = (int32_t)t;
+    // with no valid slot every clause starts dead: the sweep has no work
+    h.ntasks = active_words ? (int32_t)t : 0;
     h.nclauses = (int32_t)n;
     h.nups = n_ups;
     h.npatches = n_patches;
@@ -1848,6 +1865,7 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     sl.results = false;
     sl.host_ready = false;
     sl.rec_total = tot;
+    sl.res_words = sl.rec_words;
     e->last_step_slot = S;
     e->last_step_hits = tot;
     e->last_step_chunks = used;
@@ -1880,11 +1898,52 @@ int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n) {
     if (e->last_step_slot < 0 || !e->slots[e->last_step_slot].results)
         return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
     StepSlot &sl = e->slots[e->last_step_slot];
+    if (sl.res_words != 3)
+        return set_err(GSS_E_STATE, "the last waited step recorded ids only (gss_step_ids_map)");
     const int64_t total = e->last_step_hits;
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
     *out = sl.h_hits.p;
+    return GSS_OK;
+}
+
+int gss_step_ids_map(gss_engine *e, const gss_hit_id **out, int64_t *n) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    *out = nullptr;
+    if (e->last_step_slot < 0 || !e->slots[e->last_step_slot].results)
+        return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
+    StepSlot &sl = e->slots[e->last_step_slot];
+    if (sl.res_words != 1)
+        return set_err(GSS_E_STATE, "the last waited step recorded full hits (gss_step_hits_map)");
+    const int64_t total = e->last_step_hits;
+    if (n) *n = total;
+    if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
+    sl.host_ready = true;
+    *out = (const gss_hit_id *)sl.h_hits.p;
+    return GSS_OK;
+}
+
+int gss_db_reserve(gss_engine *e, int64_t n, const int32_t *lens) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !lens)) return set_err(GSS_E_INVAL, "bad clause batch");
+    if (e->tagged && n > 0) return set_err(GSS_E_STATE, "tagged DB");
+    int maxlen = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
+            return set_err(GSS_E_INVAL, "clause %lld: length %d outside [1, %d]", (long long)i,
+                           lens[i], GSS_MAX_CLAUSE_LEN);
+        maxlen = std::max(maxlen, (int)lens[i]);
+    }
+    std::vector<int64_t> per(maxlen + 1, 0);
+    for (int64_t i = 0; i < n; i++) per[lens[i]]++;
+    if ((int)e->buckets.size() <= maxlen) e->buckets.resize(maxlen + 1);
+    for (int len = 1; len <= maxlen; len++)
+        if (per[len] && (rc = ensure_bucket_cap(e, len, e->buckets[len].count + per[len])))
+            return rc;
     return GSS_OK;
 }
 

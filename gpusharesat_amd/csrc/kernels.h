@@ -36,6 +36,7 @@ struct SweepArgs {
                              // (<= 0: none, and the kernel without that path runs)
     uint32_t sentinel;       // stored row offset of the sentinel literal
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
+    int rec_words;           // 8-byte words per record: 3 (gss_hit) or 1 (gss_hit_id)
 };
 
 // Staged regions of a batched step, copied host -> device by k_stage_in.
@@ -130,11 +131,12 @@ hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, ui
                                 uint32_t *h_out, hipStream_t s);
 // Dense copy of a sweep's records (chunk order, the records of a chunk in
 // place): offs = exclusive prefix of the fills of the claimed chunks (cap + 1
-// entries), dense = the records, when they all fit in dense_cap; counts are
-// read on the device.
+// entries), dense = the records, when they all fit in dense_cap records;
+// counts are read on the device.  Records are rec_words 8-byte words (3: a
+// gss_hit, 1: a gss_hit_id); chunk c starts at word c * kHitChunk * rec_words.
 hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
-                                  uint64_t *offs, const gss_hit *hits, gss_hit *dense,
-                                  uint64_t dense_cap, hipStream_t s);
+                                  uint64_t *offs, const void *hits, void *dense,
+                                  uint64_t dense_cap, int rec_words, hipStream_t s);
 // Solver base planes: bases[(2 b) * vw + i] / bases[(2 b + 1) * vw + i] hold
 // bit (v mod 64) of word i = v / 64 of base b's is_def / is_true.
 hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uint64_t *bases,

@@ -49,11 +49,13 @@
 // a run of clauses of one length bucket (its parameters are wave-uniform and
 // live in scalar registers); waves are persistent and take tasks round-robin.
 //
-// Hits.  A clause/word with a report becomes one 24-byte record.  Each wave
-// compacts its records with a ballot + mbcnt prefix and appends them to a
-// private chunk of kHitChunk records claimed with one atomic (non-temporal
-// stores: records are never re-read by the sweep); a chunk is abandoned (fill
-// recorded) when the next pass does not fit.  Capacity is
+// Hits.  A clause/word with a report becomes one 24-byte record (gss_hit:
+// clause, word and both masks), or, for delivery to the solvers, which never
+// read the masks, one 8-byte record (gss_hit_id: the first word of the
+// gss_hit).  Each wave compacts its records with a ballot + mbcnt prefix and
+// appends them to a private chunk of kHitChunk records claimed with one
+// atomic (non-temporal stores: records are never re-read by the sweep); a
+// chunk is abandoned (fill recorded) when the next pass does not fit.  Capacity is
 // sized on the host so that nothing is dropped (the reference drops on
 // overflow, gpuShareLib/Reporter.cuh:46-48); if it ever is exceeded the claim
 // counter still counts and the host re-runs the sweep.
@@ -73,6 +75,9 @@ static_assert(sizeof(gss_hit) == 24 && offsetof(gss_hit, len) == 4 &&
                   offsetof(gss_hit, word) == 6 && offsetof(gss_hit, false_mask) == 8 &&
                   offsetof(gss_hit, unit_mask) == 16,
               "gss_hit layout");
+static_assert(sizeof(gss_hit_id) == 8 && offsetof(gss_hit_id, len) == 4 &&
+                  offsetof(gss_hit_id, word) == 6,
+              "gss_hit_id is the first word of a gss_hit");
 
 namespace {
 
@@ -407,22 +412,18 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                         fill = 0;
                     }
                     if (hit && (uint32_t)chunk < a.chunk_cap) {
-                        gss_hit h;
-                        h.row = (uint32_t)row;
-                        h.len = (uint16_t)len;
-                        h.word = (uint16_t)(lw * K + k);
-                        h.false_mask = fm[k];
-                        h.unit_mask = um[k];
                         // Records are streamed out and never read by the kernel:
                         // non-temporal stores keep them from displacing table
                         // rows in L2 and the memory-side cache.
-                        uint64_t *dst = (uint64_t *)(a.hits + (size_t)chunk * kHitChunk + fill +
-                                                     prefix_count(bal));
-                        __builtin_nontemporal_store((uint64_t)h.row | ((uint64_t)h.len << 32) |
-                                                        ((uint64_t)h.word << 48),
-                                                    dst);
-                        __builtin_nontemporal_store(h.false_mask, dst + 1);
-                        __builtin_nontemporal_store(h.unit_mask, dst + 2);
+                        const uint64_t id = (uint64_t)(uint32_t)row | ((uint64_t)len << 32) |
+                                            ((uint64_t)(lw * K + k) << 48);
+                        const size_t r = (size_t)chunk * kHitChunk + fill + prefix_count(bal);
+                        uint64_t *dst = (uint64_t *)a.hits + r * (size_t)a.rec_words;
+                        __builtin_nontemporal_store(id, dst);
+                        if (a.rec_words == 3) {
+                            __builtin_nontemporal_store(fm[k], dst + 1);
+                            __builtin_nontemporal_store(um[k], dst + 2);
+                        }
                     }
                     fill += n;
                 }
@@ -643,14 +644,14 @@ __global__ __launch_bounds__(256) void k_gather_chunks(const uint32_t *alloc,
                                                        const uint64_t *__restrict__ offs,
                                                        const uint64_t *__restrict__ hits,
                                                        uint64_t *__restrict__ dense,
-                                                       uint64_t dense_cap) {
+                                                       uint64_t dense_cap, uint32_t rec_words) {
     const uint32_t used = *alloc;
     const uint32_t n = used < cap ? used : cap;
     if (offs[n] > dense_cap) return;
     for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
-        const uint32_t words = fills[c] * 3u;
-        const uint64_t *src = hits + (uint64_t)c * kHitChunk * 3u;
-        uint64_t *dst = dense + offs[c] * 3u;
+        const uint32_t words = fills[c] * rec_words;
+        const uint64_t *src = hits + (uint64_t)c * kHitChunk * rec_words;
+        uint64_t *dst = dense + offs[c] * rec_words;
         for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
     }
 }
@@ -899,14 +900,15 @@ hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, ui
 }
 
 hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
-                                  uint64_t *offs, const gss_hit *hits, gss_hit *dense,
-                                  uint64_t dense_cap, hipStream_t s) {
+                                  uint64_t *offs, const void *hits, void *dense,
+                                  uint64_t dense_cap, int rec_words, hipStream_t s) {
+    if (rec_words != 1 && rec_words != 3) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, s, alloc, fills, cap, offs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_chunks, dim3(2048), dim3(256), 0, s, alloc, fills, cap,
                        (const uint64_t *)offs, (const uint64_t *)hits, (uint64_t *)dense,
-                       dense_cap);
+                       dense_cap, (uint32_t)rec_words);
     return hipGetLastError();
 }
 

@@ -146,6 +146,14 @@ void EngineClauseSharer::ensure_table() {
     }
 }
 
+// A run (GpuRunner::startGpuRunAsync, GpuRunner.cu:274-351) is one batched
+// step: the run's new clauses, the solvers' assignment updates, the valid
+// slots and the roll-forward positions are staged into the step's pinned slot
+// (one host-to-device copy, made by the step's first kernel), and one HIP
+// graph appends, applies, checks, rolls forward and compacts the records.
+// The records are gss_hit_id (clause and solver word, no masks: delivery never
+// reads them); gss_step_wait starts their copy to pinned host memory on the
+// slot's own stream, and deliver() reads them there.
 bool EngineClauseSharer::start_run(InFlight &run) {
     ClauseMirror::Batch b;
     clauses_.commit_staged(b);
@@ -154,24 +162,20 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();
     if (!b.lens.empty()) {
         clause_vars_ = std::max<int64_t>(clause_vars_, (b.max_lit >> 1) + 1);
-        std::vector<uint32_t> rows(b.lens.size());
-        int rc = gss_db_append(engine_, (int64_t)b.lens.size(), b.lits.data(), b.lens.data(),
-                               rows.data());
+        // room for the batch first, before any window is taken: out of
+        // device memory, the DB is halved and re-uploaded
+        // (GpuRunner.cu:243-260), but only after the run in flight has been
+        // delivered: its hits name clauses by row, and the reduction moves
+        // rows (GpuRunner.cu:251-253).  The device then holds none of this
+        // batch, which the mirror already has.
+        const int rc = gss_db_reserve(engine_, (int64_t)b.lens.size(), b.lens.data());
         if (rc == GSS_E_NOMEM) {
-            // out of device memory: the DB is halved and re-uploaded
-            // (GpuRunner.cu:243-260), but only after the run in flight has
-            // been delivered: its hits name clauses by row, and the
-            // reduction moves rows (GpuRunner.cu:251-253).  The device
-            // holds none of this batch, which the mirror already has.
             ran_out_of_memory_ = true;
             reduce_pending_ = true;
             dirty_.insert(b.lens.begin(), b.lens.end());
             return false;
         }
-        check_rc(rc, "gss_db_append");
-        for (size_t i = 0; i < rows.size(); i++)
-            if (rows[i] != b.rows[i])
-                throw std::logic_error("device clause rows diverged from the host mirror");
+        check_rc(rc, "gss_db_reserve");
     }
     if (clauses_.clause_count() == 0) return false;  // GpuRunner.cu:281-286
     ensure_table();
@@ -194,45 +198,38 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     }
     // table vars may have grown through the windows while we held no lock
     ensure_table();
-    check_rc(gss_assig_apply(engine_, (int64_t)run.updates.size(), run.updates.data()),
-             "gss_assig_apply");
-    // with no completed assignment in any slot the check cannot report
-    // anything (every slot starts dead): the updates still go to the table,
-    // the sweep is skipped
-    run.swept = false;
-    for (uint64_t v : valid) run.swept = run.swept || v != 0;
-    if (run.swept) check_rc(gss_sweep_async(engine_, valid.data()), "gss_sweep_async");
-    // the roll-forward is queued once the sweep has been waited for (collect):
-    // a sweep whose hit buffer overflows is re-run on the table it was
-    // queued with
+    std::vector<uint32_t> rows(b.lens.size());
+    gss_step_desc d{};
+    d.n_clauses = (int64_t)b.lens.size();
+    d.lits = b.lits.data();
+    d.lens = b.lens.data();
+    d.out_rows = rows.data();
+    d.n_ups = (int64_t)run.updates.size();
+    d.ups = run.updates.data();
+    d.valid = valid.data();
+    d.last_pos = run.last_pos.data();
+    d.records = GSS_RECORDS_IDS;
+    check_rc(gss_step_submit(engine_, &d), "gss_step_submit");
+    for (size_t i = 0; i < rows.size(); i++)
+        if (rows[i] != b.rows[i])
+            throw std::logic_error("device clause rows diverged from the host mirror");
     run.clause_count = clauses_.clause_count();
     run.active = true;
     return true;
 }
 
-void EngineClauseSharer::collect(InFlight &run, std::vector<gss_hit> &hits) {
-    auto roll = [&] {
-        check_rc(gss_assig_roll_forward_applied(engine_, run.last_pos.data()),
-                 "gss_assig_roll_forward_applied");
-    };
-    if (!run.swept) {
-        roll();
-        hits.clear();
-        return;
-    }
+void EngineClauseSharer::collect(InFlight &run) {
     int64_t n = 0;
-    check_rc(gss_sweep_wait(engine_, &n), "gss_sweep_wait");
-    roll();
-    hits.resize((size_t)n);
-    if (n) check_rc(gss_hits_copy(engine_, hits.data(), n, &n), "gss_hits_copy");
-    if (opts_.quickProf) {
-        float ms = 0;
-        gss_sweep_kernel_ms(engine_, &ms);
-        global_stats_[timeSpentTestingClauses] += (unsigned long)(ms * 1000.0f);
-    }
+    float ms = 0.f;
+    check_rc(gss_step_wait(engine_, &n, &ms), "gss_step_wait");
+    run.records = n;
+    if (opts_.quickProf) global_stats_[timeSpentTestingClauses] += (unsigned long)(ms * 1000.0f);
 }
 
-void EngineClauseSharer::deliver(InFlight &run, std::vector<gss_hit> &hits) {
+void EngineClauseSharer::deliver(InFlight &run) {
+    const gss_hit_id *ids = nullptr;
+    int64_t n = 0;
+    if (run.records) check_rc(gss_step_ids_map(engine_, &ids, &n), "gss_step_ids_map");
     global_stats_[gpuRuns]++;
     long assigs = 0, solvers_with = 0;
     for (const auto &t : run.taken) {
@@ -243,8 +240,8 @@ void EngineClauseSharer::deliver(InFlight &run, std::vector<gss_hit> &hits) {
     global_stats_[totalAssigClauseTested] += cl * (unsigned long)assigs;
     global_stats_[clauseTestsOnGroups] += cl;
     global_stats_[clauseTestsOnAssigs] += cl * (unsigned long)solvers_with;
-    global_stats_[gpuReports] += hits.size();
-    for (const gss_hit &h : hits) clauses_.bump(h.len, h.row);
+    global_stats_[gpuReports] += (unsigned long)n;
+    for (int64_t i = 0; i < n; i++) clauses_.bump(ids[i].len, ids[i].row);
     Gauge g(global_stats_[timeSpentFillingReported], opts_.quickProf);
     const int S = (int)run.taken.size();
     std::vector<ClauseBatch *> batch(S, nullptr);
@@ -254,7 +251,8 @@ void EngineClauseSharer::deliver(InFlight &run, std::vector<gss_hit> &hits) {
         batch[s]->start_assig = run.taken[s].start_id;
         batch[s]->assig_count = run.taken[s].count;
     }
-    for (const gss_hit &h : hits) {
+    for (int64_t i = 0; i < n; i++) {
+        const gss_hit_id &h = ids[i];
         const int s = h.word;
         if (s >= S || !batch[s]) continue;  // cannot happen: hits only in valid slots
         ClauseBatch &b = *batch[s];
@@ -266,16 +264,20 @@ void EngineClauseSharer::deliver(InFlight &run, std::vector<gss_hit> &hits) {
         if (batch[s]) reports_[s]->publish(batch[s]);
 }
 
+// One run in flight, as the reference's pipeline (GpuRunner.cu:211-261): the
+// run in flight is waited for (its records start to travel to the host), the
+// next run is staged and submitted, then the waited run's records are
+// delivered while the new one is on the device.
 void EngineClauseSharer::run_once(bool can_start) {
     InFlight prev;
     const bool had = in_flight_.active;
     if (had) {
         prev = std::move(in_flight_);
         in_flight_ = InFlight();
-        collect(prev, hit_buf_);
+        collect(prev);
     }
     if (can_start) start_run(in_flight_);
-    if (had) deliver(prev, hit_buf_);
+    if (had) deliver(prev);
     if (reduce_pending_) {
         reduce_pending_ = false;
         reduce_db_now();

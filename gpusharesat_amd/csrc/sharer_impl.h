@@ -10,8 +10,9 @@
 //                   (gpuShareLib/Clauses.cu:200-291,318-356,391-525)
 //   ReportQueues    per-solver batches of reported clauses, deduplication and
 //                   own-clause suppression (gpuShareLib/Reported.cu:35-204)
-//   run pipeline    one run in flight: its hits are collected by the next
-//                   gpuRun() (gpuShareLib/GpuRunner.cu:211-383)
+//   run pipeline    one run in flight, as one batched step (gss_step_submit):
+//                   its hits are collected by the next gpuRun()
+//                   (gpuShareLib/GpuRunner.cu:211-383)
 #pragma once
 
 #include <atomic>
@@ -230,14 +231,14 @@ private:
         std::vector<gss_var_update> updates;
         std::vector<int8_t> last_pos;
         long clause_count = 0;
-        bool swept = false;  // a sweep was launched (some slot was valid)
+        int64_t records = 0;  // hit records of the waited step
     };
 
     void run_once(bool can_start);
     void reduce_db_now();
     bool start_run(InFlight &run);
-    void collect(InFlight &run, std::vector<gss_hit> &hits);
-    void deliver(InFlight &run, std::vector<gss_hit> &hits);
+    void collect(InFlight &run);
+    void deliver(InFlight &run);
     void unset_pending_locked(int solver);
     void ensure_table();
     void log(int level, const std::string &msg);
@@ -262,7 +263,6 @@ private:
     std::vector<std::string> global_names_, solver_names_;
 
     InFlight in_flight_;
-    std::vector<gss_hit> hit_buf_;
     long added_at_last_reduce_ = 0;
     bool ran_out_of_memory_ = false;
     bool reduce_pending_ = false;  // out of memory: reduce once the run in flight is delivered

@@ -53,18 +53,24 @@ class _StepDesc(ctypes.Structure):
                 ("n_patches", ctypes.c_int64), ("patches", ctypes.c_void_p),
                 ("n_base_ups", ctypes.c_int64), ("base_ups", ctypes.c_void_p),
                 ("n_loads", ctypes.c_int64), ("loads", ctypes.c_void_p),
-                ("valid", ctypes.c_void_p), ("last_pos", ctypes.c_void_p)]
+                ("valid", ctypes.c_void_p), ("last_pos", ctypes.c_void_p),
+                ("records", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def _p(a):
     return a.ctypes.data if a is not None and a.size else None
 
 
+RECORDS_FULL = 0
+RECORDS_IDS = 1
+
+
 def submit(e: Engine, valid, lits=None, lens=None, patches=None, base_ups=None, loads=None,
-           ups=None, last_pos=None) -> None:
+           ups=None, last_pos=None, records: int = RECORDS_FULL) -> None:
     """gss_step_submit with numpy arrays (kept alive by the caller until the
     call returns; the engine copies them into its pinned staging)."""
     d = _StepDesc()
+    d.records = records
     v = np.zeros(e.words, np.uint64)
     v[:len(valid)] = valid
     arrs = {"valid": v}

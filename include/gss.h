@@ -67,6 +67,17 @@ typedef struct gss_hit {
     uint64_t unit_mask;  /* slots where exactly one literal is undefined, rest false */
 } gss_hit;
 
+/* The first word of a gss_hit: which clause and which assignment word (the
+ * solver), without the masks.  A batched step can record hits in this form
+ * (GSS_RECORDS_IDS) for delivery: the reference's host side never reads a
+ * report's slot mask either (Reported.cu:190-204 folds it into a field that is
+ * never read), so a third of the bytes carries everything a solver receives. */
+typedef struct gss_hit_id {
+    uint32_t row;
+    uint16_t len;
+    uint16_t word;
+} gss_hit_id;
+
 typedef struct gss_var_update {
     uint32_t var;
     uint32_t word;
@@ -273,7 +284,14 @@ typedef struct gss_step_desc {
     const gss_word_load *loads;
     const uint64_t *valid;             /* [words] */
     const int8_t *last_pos;            /* [words] or NULL */
+    int32_t records;                   /* GSS_RECORDS_FULL (0): gss_hit records, read with
+                                          gss_step_hits_map / gss_step_hits_copy;
+                                          GSS_RECORDS_IDS (1): gss_hit_id records, read
+                                          with gss_step_ids_map */
+    int32_t reserved;                  /* 0 */
 } gss_step_desc;
+#define GSS_RECORDS_FULL 0
+#define GSS_RECORDS_IDS 1
 int gss_step_submit(gss_engine *e, const gss_step_desc *d);
 /* (Re)creates nbases solver base planes over the current variable count,
  * every variable undefined (2 bits per variable per base).  A table that
@@ -298,6 +316,17 @@ int gss_step_hits_copy(gss_engine *e, gss_hit *out, int64_t cap, int64_t *n);
  * gss_step_wait of that step's slot.  Same source as gss_step_hits_copy;
  * replaces Reporter::getCopiedToHost (Reporter.cuh:103). */
 int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n);
+/* The same for a step submitted with GSS_RECORDS_IDS: *out points at its
+ * gss_hit_id records in pinned host memory, valid as long as
+ * gss_step_hits_map's.  Each is the first word of the gss_hit the full form
+ * records for the same (clause, word). */
+int gss_step_ids_map(gss_engine *e, const gss_hit_id **out, int64_t *n);
+/* Makes room for n more clauses (lens[i] literals each) in their length
+ * buckets without adding any: a step that appends them then needs no bucket
+ * growth.  GSS_E_NOMEM when the device (or the DB byte cap) cannot hold them,
+ * with nothing else changed.  Stands in for the capacity check the reference
+ * makes before a run's clause copy (GpuRunner.cu:243-260). */
+int gss_db_reserve(gss_engine *e, int64_t n, const int32_t *lens);
 
 #ifdef __cplusplus
 }

@@ -253,3 +253,83 @@ def test_sweep_only_step_and_errors(engine_factory):
     a.step_wait()
     a.step_wait()
     assert a.totals() == b.totals()
+
+
+def test_id_records_are_the_full_records_first_words(engine_factory):
+    """A step submitted with GSS_RECORDS_IDS records, for every (clause, word)
+    with a report, exactly the first 8-byte word of the gss_hit the full form
+    records (row, length, word): the ids of a step equal the full step's
+    records without their masks, on the same DB and table, for steps that
+    append, update and roll forward in between; each form is refused by the
+    other form's reader."""
+    from gpusharesat_amd import serving as sv
+    from gpusharesat_amd.engine import HIT_ID_DTYPE
+    rng = np.random.default_rng(720)
+    V, W = 1500, 16
+    a, b = engine_factory(), engine_factory()
+    for e in (a, b):
+        e.reset_assignments(V, W)
+    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    a.upload(d0, t0)
+    b.upload(d0, t0)
+    plain = Plain(b)
+    lens_a, rows_a = np.zeros(0, np.int32), np.zeros(0, np.uint32)
+    for k in range(5):
+        lits, lens, ups, valid, last_pos = make_step(rng, V, W, int(rng.integers(200, 900)), 25,
+                                                     600)
+        exp = plain.step(lits, lens, ups, valid, last_pos)
+        rows = np.empty(lens.size, np.uint32)
+        sv.submit(a, valid, lits=lits, lens=lens, ups=ups, last_pos=last_pos,
+                  records=sv.RECORDS_IDS)
+        n, _ = a.step_wait()
+        ids = a.step_ids_view().copy()
+        with pytest.raises(GssError, match="ids only"):
+            a.step_hits_view()
+        print(f"step {k}: {n} id records", flush=True)
+        # the rows the step gave the new clauses: read back from the DB order
+        lens_a = np.concatenate([lens_a, lens])
+        rows_a = np.concatenate([rows_a, plain.rows[-lens.size:]])
+        assert ids.size == n == exp.size
+        # expected ids from the oracle's records: (row, length, word)
+        want = np.empty(exp.size, HIT_ID_DTYPE)
+        want["row"] = rows_a[exp["clause"]]
+        want["len"] = lens_a[exp["clause"]]
+        want["word"] = exp["word"]
+        assert np.array_equal(np.sort(ids.view(np.uint64)), np.sort(want.view(np.uint64)))
+    # the same state through a full-record step: its first words are the ids
+    sv.submit(a, valid, records=sv.RECORDS_IDS)
+    a.step_wait()
+    ids = np.sort(a.step_ids_view().view(np.uint64))
+    a.step_enqueue(valid=valid)
+    a.step_wait()
+    full = a.step_hits_view()
+    with pytest.raises(GssError, match="full hits"):
+        a.step_ids_view()
+    assert np.array_equal(ids, np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0]))
+
+
+def test_step_with_no_valid_slot_still_appends_and_updates(engine_factory):
+    """A step whose slots are all invalid checks nothing (no records) but
+    still appends its clauses, applies its updates and rolls forward: the
+    next step's records equal the oracle's on the table it then holds."""
+    rng = np.random.default_rng(730)
+    V, W = 900, 4
+    a, b = engine_factory(), engine_factory()
+    for e in (a, b):
+        e.reset_assignments(V, W)
+    plain = Plain(b)
+    lits, lens, ups, valid, last_pos = make_step(rng, V, W, 700, 12, 500)
+    plain.step(lits, lens, ups, np.zeros(W, np.uint64), last_pos)
+    rows = a.step_enqueue(lits, lens, ups, np.zeros(W, np.uint64), last_pos)
+    n, _ = a.step_wait()
+    assert n == 0
+    da, ta = a.download()
+    db, tb = b.download()
+    assert np.array_equal(da, db) and np.array_equal(ta, tb)
+    exp = plain.step(np.zeros(0, np.uint32), np.zeros(0, np.int32), ups[:0], valid,
+                     np.full(W, -1, np.int8))
+    a.step_enqueue(valid=valid)
+    n, _ = a.step_wait()
+    assert n == exp.size > 0
+    assert_same(to_records(a.step_hits(), lens, rows), exp, "after the empty-slot step")
