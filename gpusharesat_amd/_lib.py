@@ -28,7 +28,7 @@ class GssError(RuntimeError):
 
 
 # Every symbol exported by the headers under include/ (gss.h, gss_sharer.h,
-# gss_residency.h, gss_ceilings.h), with its argument types (all return int status unless listed in _RESTYPES).
+# gss_residency.h, gss_ceilings.h, gss_serve.h), with its argument types (all return int status unless listed in _RESTYPES).
 SIGNATURES: dict[str, list] = {
     "gss_last_error": [],
     "gss_engine_create": [c_int, c_p],
@@ -68,6 +68,17 @@ SIGNATURES: dict[str, list] = {
     "gss_step_hits_map": [c_p, c_p, c_p],
     "gss_step_ids_map": [c_p, c_p, c_p],
     "gss_db_reserve": [c_p, c_i64, c_p],
+    "gss_server_default_config": [c_p],
+    "gss_server_create": [c_int, c_p, c_p],
+    "gss_server_destroy": [c_p],
+    "gss_server_engine": [c_p, c_p],
+    "gss_server_update_base": [c_p, c_i64, c_p],
+    "gss_server_add_clauses": [c_p, c_i64, c_p, c_p, c_p],
+    "gss_server_submit": [c_p, c_int, c_int, c_i64, c_p, c_p],
+    "gss_server_poll": [c_p, c_i64, c_p, c_p, c_i64],
+    "gss_server_result": [c_p, c_i64, c_p],
+    "gss_server_release": [c_p, c_i64],
+    "gss_server_stats_get": [c_p, c_p],
     "gss_step_submit": [c_p, c_p],
     "gss_bases_reset": [c_p, c_int],
     # include/gss_residency.h

@@ -30,6 +30,7 @@ SOURCES = [
     "host_state.cpp",
     "sharer.cpp",
     "sharer_capi.cpp",
+    "serve.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -74,7 +75,9 @@ def build(verbose: bool = False) -> str:
             cmd = [HIPCC, *COMMON, *DEVICE, "-c", path, "-o", obj]
         else:
             # host-only C++ that calls the HIP runtime API
-            cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-x", "c++", "-c", path, "-o", obj]
+            rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+            cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include", "-x", "c++",
+                   "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)

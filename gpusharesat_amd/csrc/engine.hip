@@ -225,6 +225,8 @@ struct StepSlot {
     hipGraphExec_t exec = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool busy = false;
+    bool serve = false;    // the slot's last step admitted requests from the serving ring
+    Pinned<ServeReport> h_report;  // what that step admitted (serving steps)
     bool results = false;  // h_out / d_hits hold the results of the last waited step
     int rec_words = 3;     // record form of the slot's last submitted step (3: gss_hit,
                            // 1: gss_hit_id)
@@ -329,6 +331,13 @@ struct gss_engine {
     DevBuf<uint32_t> d_flags, d_pos, d_kept;
     DevBuf<uint8_t> d_scan_temp, d_keep;
     Pinned<uint32_t> h_kept;
+
+    // serving ring (gss_server): requests admitted on the device by serving steps
+    bool serve_on = false;
+    ServeRing serve{};
+    DevBuf<uint64_t> serve_tail;
+    DevBuf<ServeAdmitted> serve_adm;
+    DevBuf<uint32_t> serve_cnt;
 
     // batched steps (gss_step_*)
     StepSlot slots[2];
@@ -550,6 +559,9 @@ int gss_engine_destroy(gss_engine *e) {
     e->d_ups.release();
     e->d_offsets.release();
     e->d_dense.release();
+    e->serve_tail.release();
+    e->serve_adm.release();
+    e->serve_cnt.release();
     if (e->d_valid) (void)hipFree(e->d_valid);
     if (e->d_alloc) (void)hipFree(e->d_alloc);
     if (e->d_lastpos) (void)hipFree(e->d_lastpos);
@@ -895,6 +907,7 @@ int gss_db_totals(gss_engine *e, int64_t *clauses, int64_t *literals) {
 int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if (n == 0 && row0 == 0 && len >= 1) return GSS_OK;  // nothing to read
     if (len < 1 || len >= (int)e->buckets.size() || row0 < 0 || n < 0 ||
         row0 + n > e->buckets[len].count || (n && !out))
         return set_err(GSS_E_INVAL, "bad bucket read");
@@ -1478,7 +1491,8 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
 }
 
 // Captures the step's fixed launch sequence over this slot's buffers.
-static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse) {
+static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse,
+                            bool serve) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
     sl.exec = nullptr;
@@ -1524,12 +1538,18 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
         he = launch_scatter_clauses(sl.d_cls.p, &dh->nclauses, sl.d_lits.p, kLitShift, cs);
     if (he == hipSuccess)
         he = launch_base_apply_dyn(sl.d_bups.p, &dh->nbups, e->bases.p, e->bases_vw, cs);
+    // serving: the published requests that fit are admitted here, on the
+    // device, when the step starts (their loads, valid slots and counts)
+    if (he == hipSuccess && serve)
+        he = launch_serve_admit(e->serve, dh, sl.d_loads.p, sl.h_report.p, e->words, cs);
     if (he == hipSuccess)
         he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
                                    e->table.p, e->words, e->vars, cs);
     if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
     if (he == hipSuccess)
-        he = launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words, cs);
+        he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, cs)
+                   : launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words,
+                                            cs);
     if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
     if (he == hipSuccess) he = launch_sweep(a, e->words, e->tune_unroll, blocks, cs);
     if (he == hipSuccess)
@@ -1588,9 +1608,53 @@ int gss_bases_reset(gss_engine *e, int nbases) {
 }
 
 int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
+    return gss::step_submit_ex(e, desc, false);
+}
+
+}  // extern "C"
+
+int gss::engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,
+                             const gss_var_patch *patches, uint64_t patch_cap,
+                             const uint64_t *head) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!reqs || !patches || !head || !req_cap || !patch_cap)
+        return set_err(GSS_E_INVAL, "bad serving ring");
+    if (!e->inflight.empty()) return set_err(GSS_E_STATE, "steps in flight");
+    if ((rc = e->serve_tail.reserve(1)) || (rc = e->serve_adm.reserve(kServeMaxRequests)) ||
+        (rc = e->serve_cnt.reserve(1)))
+        return rc;
+    HIP_TRY(launch_fill_u64(e->serve_tail.p, 1, 0ull, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    ServeRing &r = e->serve;
+    r.reqs = reqs;
+    r.patches = patches;
+    r.head = head;
+    r.tail = e->serve_tail.p;
+    r.adm = e->serve_adm.p;
+    r.adm_count = e->serve_cnt.p;
+    r.req_cap = req_cap;
+    r.patch_cap = patch_cap;
+    e->serve_on = true;
+    return GSS_OK;
+}
+
+int gss::step_serve_report(gss_engine *e, ServeReport *out) {
+    if (!e || !out) return set_err(GSS_E_INVAL, "null argument");
+    if (e->last_step_slot < 0) return set_err(GSS_E_STATE, "no waited step");
+    const StepSlot &sl = e->slots[e->last_step_slot];
+    if (!sl.serve || !sl.h_report.p) return set_err(GSS_E_STATE, "not a serving step");
+    *out = *sl.h_report.p;
+    return GSS_OK;
+}
+
+int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     int rc = check_engine(e);
     if (rc) return rc;
     if (!desc) return set_err(GSS_E_INVAL, "null step");
+    if (serve && !e->serve_on) return set_err(GSS_E_STATE, "no serving ring attached");
+    if (serve && (desc->n_loads || desc->n_patches))
+        return set_err(GSS_E_INVAL, "a serving step takes its loads and patches from the ring");
     const int64_t n = desc->n_clauses, n_ups = desc->n_ups, n_patches = desc->n_patches,
                   n_bups = desc->n_base_ups, n_loads = desc->n_loads;
     const uint32_t *lits = desc->lits;
@@ -1643,6 +1707,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         if (last_pos && last_pos[w] > 63) return set_err(GSS_E_INVAL, "last_pos > 63");
         if (valid[w]) active_words++;
     }
+    if (serve) active_words = W;  // any word may be admitted on the device
     int maxlen = 0;
     int64_t total = 0;
     for (int64_t i = 0; i < n; i++) {
@@ -1718,13 +1783,14 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     // this step, frees of replaced buffers included, runs there.
     hipStream_t es = e->stream;
     if (sl.copy_queued) HIP_TRY(hipStreamWaitEvent(es, sl.ev_copied, 0));
+    if (serve && (rc = sl.h_report.reserve(1))) return rc;
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
         (rc = sl.d_cls.reserve_async(sl.h_cls.cap, es)) ||
         (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
         (rc = sl.d_desc.reserve_async(sl.h_desc.cap, es)) ||
         (rc = sl.d_patches.reserve_async(sl.h_patches.cap, es)) ||
         (rc = sl.d_bups.reserve_async(sl.h_bups.cap, es)) ||
-        (rc = sl.d_loads.reserve_async(sl.h_loads.cap, es)))
+        (rc = sl.d_loads.reserve_async(std::max<size_t>(sl.h_loads.cap, GSS_MAX_WORDS), es)))
         return rc;
     // the hit buffer holds chunks of records of this step's form
     const int rw = desc->records == GSS_RECORDS_IDS ? 1 : 3;
@@ -1760,10 +1826,11 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
         (const void *)(intptr_t)W, (const void *)(intptr_t)e->vars,
         (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
-        (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words};
+        (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words,
+        (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
-        if ((rc = build_step_graph(e, sl, blocks, tc, fuse))) return rc;
+        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve))) return rc;
         sl.key = key;
         sl.graph_dense = sl.d_dense.cap;
     }
@@ -1813,7 +1880,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     StepHeader &h = *sl.h_hdr.p;
     std::memset(&h, 0, sizeof h);
     h.nbuckets = k;
-    // with no valid slot every clause starts dead: the sweep has no work
+    // with no valid slot every clause starts dead: the sweep has no work (a
+    // serving step's admission decides that on the device)
     h.ntasks = active_words ? (int32_t)t : 0;
     h.nclauses = (int32_t)n;
     h.nups = n_ups;
@@ -1837,6 +1905,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));
     sl.busy = true;
+    sl.serve = serve;
     // the previous records of this slot stay readable only if their copy
     // was queued (or there were none)
     if (sl.rec_total > 0 && !sl.copy_queued) sl.results = false;
@@ -1844,6 +1913,8 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     e->next_slot ^= 1;
     return GSS_OK;
 }
+
+extern "C" {
 
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     int rc = check_engine(e);

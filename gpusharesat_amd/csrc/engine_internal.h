@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "gss.h"
+#include "kernels.h"
 
 namespace gss {
 
@@ -22,6 +23,17 @@ int db_tag_arrays(gss_engine *e, std::vector<const uint32_t *> &tags);
 int sweep_records(gss_engine *e, const gss_hit **hits, const uint32_t **fills,
                   uint32_t *chunks);
 hipStream_t engine_stream(gss_engine *e);
+// Serving (gss_server): attaches the request and patch rings (fine-grained
+// pinned host memory the host publishes into, head = requests published) to
+// the engine; the device-side admission cursor starts at request 0.
+int engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,
+                        const gss_var_patch *patches, uint64_t patch_cap, const uint64_t *head);
+// gss_step_submit, or with serve a step that admits requests from the ring
+// on the device (the descriptor then carries no loads, patches or valid
+// slots of its own).
+int step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve);
+// What the last waited serving step admitted.
+int step_serve_report(gss_engine *e, ServeReport *out);
 // Makes the engine's device current (GSS_OK or an error code).
 int engine_activate(gss_engine *e);
 

@@ -84,6 +84,60 @@ struct ClauseSeg {
     uint32_t len;
 };
 
+// ---- serving: requests admitted on the device -------------------------------
+
+// One request of the serving ring (fine-grained pinned host memory, written
+// by the host before it publishes the request).
+struct ServeReq {
+    uint32_t solver;     // base the request's words are loaded from
+    uint32_t k;          // assignments: slots [0, k) of words [0, ceil(k / 64))
+    uint64_t patch_off;  // first patch (running index; ring position = modulo capacity)
+    uint32_t n_patches;
+    uint32_t pad;
+    uint64_t seq;        // request sequence number
+};
+
+// A request admitted by a step: where it landed and where its patches are.
+struct ServeAdmitted {
+    uint64_t patch_off;
+    uint64_t patch_pfx;  // patches of the step's earlier admitted requests
+    uint32_t n_patches;
+    uint32_t w0;         // first table word
+};
+
+// What a step admitted (pinned host memory, read after the step).
+struct ServeReport {
+    uint64_t first;  // sequence number of the first admitted request
+    uint32_t count;  // requests admitted (in sequence order)
+    uint32_t words;  // table words they took
+};
+
+// The serving kernels' view of the rings and the device-side cursor.
+struct ServeRing {
+    const ServeReq *reqs;          // pinned ring, req_cap entries
+    const gss_var_patch *patches;  // pinned ring, patch_cap entries (words relative to the request)
+    const uint64_t *head;          // pinned: requests published
+    uint64_t *tail;                // device: next request to admit
+    ServeAdmitted *adm;            // device: [kServeMaxRequests]
+    uint32_t *adm_count;           // device
+    uint32_t req_cap;
+    uint32_t pad;
+    uint64_t patch_cap;
+};
+constexpr int kServeMaxRequests = 64;  // one admission wave: at most 64 requests per step
+
+// Admits, in sequence order, the published requests whose words fit the
+// table (FIFO: the first that does not fit waits for the next step): writes
+// the step's word loads, valid masks, load and patch counts (and no tasks
+// when nothing was admitted) into the device header, the admitted list, and
+// the report.  One wave.
+hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load *loads,
+                              ServeReport *report, int words, hipStream_t s);
+// Applies the admitted requests' patches, read from the pinned ring, with
+// their words shifted to where each request landed.
+hipError_t launch_serve_patch(const ServeRing &r, const StepHeader *dh, uint64_t *table,
+                              int words, hipStream_t s);
+
 // One non-empty bucket of a tagged DB during compaction.
 struct CompactRef {
     const uint32_t *lits;  // source rows (stored offsets)

@@ -680,6 +680,108 @@ __global__ __launch_bounds__(1024) void k_count_records(const uint32_t *alloc,
     }
 }
 
+// ---- serving: requests admitted on the device ---------------------------------
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(64) void k_serve_admit(ServeRing r, StepHeader *dh,
+                                                    gss_word_load *loads, ServeReport *report,
+                                                    int words) {
+    const int lane = (int)threadIdx.x;
+    __shared__ uint64_t s_head;
+    if (lane == 0) {
+        // published by the host with a release store after the entries
+        s_head = __hip_atomic_load(r.head, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // later reads see the published entries
+    const uint64_t tail = *r.tail;
+    const uint64_t head = s_head;
+    const uint64_t pend = head > tail ? head - tail : 0;
+    const int avail = (int)(pend < (uint64_t)kServeMaxRequests ? pend : kServeMaxRequests);
+    uint32_t solver = 0, k = 0, np = 0;
+    uint64_t poff = 0;
+    if (lane < avail) {
+        const volatile ServeReq *q = r.reqs + (tail + lane) % r.req_cap;
+        solver = q->solver;
+        k = q->k;
+        np = q->n_patches;
+        poff = q->patch_off;
+    }
+    const uint64_t nw = lane < avail ? (k + 63u) / 64u : 0;
+    const uint64_t end = wave_incl_scan(nw, lane);
+    // end grows with the lane (every request takes at least one word), so
+    // the requests that fit are a prefix
+    const bool fits = lane < avail && end <= (uint64_t)words;
+    const unsigned long long bal = __ballot(fits);
+    const int count = __popcll(bal);
+    const uint64_t pend_p = wave_incl_scan(fits ? (uint64_t)np : 0, lane);
+    const uint64_t used = __shfl(end, count > 0 ? count - 1 : 0);
+    const uint64_t total_words = count > 0 ? used : 0;
+    const uint64_t total_patches = __shfl(pend_p, count > 0 ? count - 1 : 0);
+    if (fits) {
+        const uint32_t w0 = (uint32_t)(end - nw);
+        ServeAdmitted a;
+        a.patch_off = poff;
+        a.patch_pfx = pend_p - np;
+        a.n_patches = np;
+        a.w0 = w0;
+        r.adm[lane] = a;
+        for (uint32_t j = 0; j < (uint32_t)nw; j++) {
+            gss_word_load ld;
+            ld.word = (uint16_t)(w0 + j);
+            ld.base = (uint16_t)solver;
+            loads[w0 + j] = ld;
+            const uint32_t last = k - 64u * j;  // slots of this word
+            dh->valid[w0 + j] = last >= 64u ? ~0ull : ((1ull << last) - 1ull);
+        }
+    }
+    // words no request took are not checked
+    if ((uint64_t)lane < (uint64_t)GSS_MAX_WORDS && (uint64_t)lane >= total_words)
+        dh->valid[lane] = 0ull;
+    if (lane == 0) {
+        *r.adm_count = (uint32_t)count;
+        dh->nloads = (int64_t)total_words;
+        dh->npatches = count > 0 ? (int64_t)total_patches : 0;
+        if (count == 0) dh->ntasks = 0;  // nothing to check
+        *r.tail = tail + (uint64_t)count;
+        report->first = tail;
+        report->count = (uint32_t)count;
+        report->words = (uint32_t)total_words;
+    }
+}
+
+__global__ void k_serve_patch(ServeRing r, const int64_t *n_p, uint64_t *__restrict__ table,
+                              int words) {
+    const int64_t n = *n_p;
+    const int cnt = (int)*r.adm_count;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = cnt;  // last admitted request whose patches start at or before i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)r.adm[mid].patch_pfx <= i) lo = mid;
+            else hi = mid;
+        }
+        const ServeAdmitted a = r.adm[lo];
+        const uint64_t j = (uint64_t)(i - (int64_t)a.patch_pfx);
+        const volatile gss_var_patch *pp = r.patches + (a.patch_off + j) % r.patch_cap;
+        const uint32_t var = pp->var;
+        const uint32_t word = pp->word + a.w0;
+        const uint64_t m = pp->mask, d = pp->is_def, t = pp->is_true;
+        const size_t base = (size_t)var * 2 * words + word;
+        table[base] = (table[base] & ~m) | (~(d & ~t) & m);
+        table[base + words] = (table[base + words] & ~m) | (~(d & t) & m);
+    }
+}
+
 // ---- tagged DB compaction --------------------------------------------------
 
 __device__ __forceinline__ int find_ref(const CompactRef *__restrict__ r, int nb, int64_t g) {
@@ -909,6 +1011,19 @@ hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, 
     hipLaunchKernelGGL(k_gather_chunks, dim3(2048), dim3(256), 0, s, alloc, fills, cap,
                        (const uint64_t *)offs, (const uint64_t *)hits, (uint64_t *)dense,
                        dense_cap, (uint32_t)rec_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load *loads,
+                              ServeReport *report, int words, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_admit, dim3(1), dim3(64), 0, s, r, dh, loads, report, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_patch(const ServeRing &r, const StepHeader *dh, uint64_t *table,
+                              int words, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_patch, dim3(1024), dim3(256), 0, s, r, &dh->npatches, table,
+                       words);
     return hipGetLastError();
 }
 

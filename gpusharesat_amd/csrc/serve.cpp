@@ -1,0 +1,525 @@
+// The serving loop of include/gss_serve.h: request and patch rings in
+// fine-grained pinned host memory, a serving thread that keeps batched steps
+// on the device (each admitting published requests on the device when it
+// starts), and per-request results.  Extends the reference's run loop
+// (GpuClauseSharerImpl.cu:83-94, GpuRunner.cu:211-261) from "the assignments
+// sent since the last run" to requests arriving at any time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine_internal.h"
+#include "errors.h"
+#include "gss_serve.h"
+
+using gss::kServeMaxRequests;
+using gss::ServeReport;
+using gss::ServeReq;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+int fail(int code, const std::string &msg) { return gss::record_error(code, msg.c_str()); }
+
+// A published request, until its step completes.
+struct Pending {
+    uint64_t seq;
+    int solver;
+    int k;
+    uint64_t patch_off;
+    uint32_t n_patches;
+    Clock::time_point t_submit;
+};
+
+// A launched step, until it completes.
+struct StepInfo {
+    int64_t index = 0;
+    int64_t clauses_after = 0;   // clauses in the DB once the step appended its batch
+    int64_t literals_after = 0;
+    int64_t bups_through = 0;    // base updates staged up to and including this step
+    uint64_t published_at_launch = 0;
+};
+
+struct Done {
+    gss_serve_result r{};
+    std::vector<gss_serve_hit> recs;
+};
+
+}  // namespace
+
+struct gss_server {
+    gss_engine *e = nullptr;
+    gss_server_config cfg{};
+    ServeReq *reqs = nullptr;        // pinned, fine-grained
+    gss_var_patch *patches = nullptr;
+    uint64_t *head = nullptr;        // requests published (read by the device)
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    uint64_t published = 0, patch_published = 0;
+    uint64_t completed = 0, patch_completed = 0;  // requests / patches whose step completed
+    std::deque<Pending> pending;                  // published, not completed, in order
+    int64_t pending_words = 0;                    // table words those requests take
+    std::vector<gss_base_update> bups;            // not yet staged
+    std::vector<uint64_t> seen;                   // [base][var / 64] bits, for deduplication
+    int64_t bups_submitted = 0;
+    std::vector<uint32_t> cl_lits;                // admitted, not yet staged
+    std::vector<int32_t> cl_lens;
+    int64_t clauses_admitted = 0, literals_admitted = 0, clauses_staged = 0, literals_staged = 0;
+    int64_t refused = 0;
+    std::vector<std::vector<int64_t>> ids;        // [length][row] -> clause id
+    std::deque<StepInfo> inflight;
+    int64_t steps = 0;
+    std::unordered_map<int64_t, Done> done;
+    std::deque<int64_t> ready;
+    int64_t records = 0, requests_done = 0;
+    double checks = 0.0, device_ms = 0.0;
+    bool stop = false;
+    int err_code = 0;
+    std::string err;
+    std::thread th;
+};
+
+namespace {
+
+bool can_launch(const gss_server *s) {
+    if (s->err_code || s->stop || (int)s->inflight.size() >= s->cfg.max_inflight) return false;
+    // requests published that no launched step can have seen yet, or more
+    // words of requests than the steps in flight can admit
+    return s->published > s->completed &&
+           (s->inflight.empty() || s->published > s->inflight.back().published_at_launch ||
+            s->pending_words > (int64_t)s->inflight.size() * s->cfg.words);
+}
+
+void set_failed(gss_server *s, int code, const char *what) {
+    s->err_code = code ? code : GSS_E_STATE;
+    s->err = std::string(what) + ": " + gss_last_error();
+    s->cv_done.notify_all();
+}
+
+// Stages the pending base updates and clauses and submits a serving step.
+void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
+    StepInfo si;
+    si.index = s->steps++;
+    // one update per (base, variable) in a step: the last one submitted wins
+    std::vector<gss_base_update> bu;
+    bu.swap(s->bups);
+    if (bu.size() > 1) {
+        // newest first: a (base, variable) already seen has a later update
+        const size_t vw = (size_t)(s->cfg.var_count + 63) / 64;
+        if (s->seen.empty()) s->seen.assign((size_t)s->cfg.solvers * vw, 0ull);
+        std::vector<gss_base_update> kept;
+        kept.reserve(bu.size());
+        for (size_t j = bu.size(); j-- > 0;) {
+            uint64_t &w = s->seen[(size_t)bu[j].base * vw + bu[j].var / 64];
+            const uint64_t bit = 1ull << (bu[j].var % 64);
+            if (w & bit) continue;
+            w |= bit;
+            kept.push_back(bu[j]);
+        }
+        for (const gss_base_update &u : kept) s->seen[(size_t)u.base * vw + u.var / 64] = 0ull;
+        bu.swap(kept);
+    }
+    si.bups_through = s->bups_submitted;
+    std::vector<uint32_t> lits;
+    std::vector<int32_t> lens;
+    lits.swap(s->cl_lits);
+    lens.swap(s->cl_lens);
+    const int64_t first_clause = s->clauses_staged;
+    s->clauses_staged += (int64_t)lens.size();
+    s->literals_staged += (int64_t)lits.size();
+    si.clauses_after = s->clauses_staged;
+    si.literals_after = s->literals_staged;
+    si.published_at_launch = s->published;
+    lk.unlock();
+    std::vector<uint32_t> rows(lens.size());
+    uint64_t valid[GSS_MAX_WORDS] = {};
+    gss_step_desc d{};
+    d.n_clauses = (int64_t)lens.size();
+    d.lits = lits.data();
+    d.lens = lens.data();
+    d.out_rows = rows.data();
+    d.n_base_ups = (int64_t)bu.size();
+    d.base_ups = bu.data();
+    d.valid = valid;
+    d.records = s->cfg.records;
+    const int rc = gss::step_submit_ex(s->e, &d, true);
+    lk.lock();
+    if (rc) {
+        set_failed(s, rc, "serving step");
+        return;
+    }
+    for (size_t i = 0; i < lens.size(); i++) {
+        const int len = lens[i];
+        if ((int)s->ids.size() <= len) s->ids.resize(len + 1);
+        std::vector<int64_t> &v = s->ids[len];
+        if (v.size() <= rows[i]) v.resize((size_t)rows[i] + 1, -1);
+        v[rows[i]] = first_clause + (int64_t)i;
+    }
+    s->inflight.push_back(si);
+}
+
+// Waits for the oldest step and hands each request it admitted its records.
+void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
+    lk.unlock();
+    int64_t n = 0;
+    float ms = 0.f;
+    int rc = gss_step_wait(s->e, &n, &ms);
+    ServeReport rep{};
+    if (!rc) rc = gss::step_serve_report(s->e, &rep);
+    const gss_hit *full = nullptr;
+    const gss_hit_id *idr = nullptr;
+    if (!rc && n > 0)
+        rc = s->cfg.records == GSS_RECORDS_IDS ? gss_step_ids_map(s->e, &idr, &n)
+                                               : gss_step_hits_map(s->e, &full, &n);
+    const Clock::time_point now = Clock::now();
+    lk.lock();
+    const StepInfo si = s->inflight.front();
+    s->inflight.pop_front();
+    if (rc) {
+        set_failed(s, rc, "serving step wait");
+        return;
+    }
+    if (rep.first != s->completed || rep.count > s->pending.size()) {
+        s->err_code = GSS_E_STATE;
+        s->err = "serving step admitted requests out of order";
+        s->cv_done.notify_all();
+        return;
+    }
+    const int cnt = (int)rep.count;
+    int owner[GSS_MAX_WORDS];
+    uint32_t w0[kServeMaxRequests];
+    std::fill(owner, owner + GSS_MAX_WORDS, -1);
+    std::vector<Done *> out(cnt, nullptr);
+    uint32_t w = 0;
+    double assigs = 0;
+    for (int i = 0; i < cnt; i++) {
+        const Pending &p = s->pending[i];
+        const uint32_t nw = (uint32_t)(p.k + 63) / 64;
+        w0[i] = w;
+        for (uint32_t j = 0; j < nw && w + j < (uint32_t)GSS_MAX_WORDS; j++) owner[w + j] = i;
+        w += nw;
+        s->pending_words -= nw;
+        assigs += p.k;
+        Done &dn = s->done[(int64_t)p.seq];
+        dn.r.request = (int64_t)p.seq;
+        dn.r.step = si.index;
+        dn.r.clauses = si.clauses_after;
+        dn.r.base_updates = si.bups_through;
+        dn.r.solver = p.solver;
+        dn.r.k = p.k;
+        dn.r.first_word = (int32_t)w0[i];
+        dn.r.latency_ms = std::chrono::duration<double, std::milli>(now - p.t_submit).count();
+        out[i] = &dn;
+    }
+    for (int64_t r = 0; r < n; r++) {
+        gss_serve_hit h{};
+        uint32_t len, row, word;
+        if (idr) {
+            len = idr[r].len;
+            row = idr[r].row;
+            word = idr[r].word;
+        } else {
+            len = full[r].len;
+            row = full[r].row;
+            word = full[r].word;
+            h.false_mask = full[r].false_mask;
+            h.unit_mask = full[r].unit_mask;
+        }
+        const int i = word < (uint32_t)GSS_MAX_WORDS ? owner[word] : -1;
+        if (i < 0) continue;  // cannot happen: only admitted words are valid
+        h.clause = (len < s->ids.size() && row < s->ids[len].size()) ? s->ids[len][row] : -1;
+        h.word = word - w0[i];
+        out[i]->recs.push_back(h);
+    }
+    for (int i = 0; i < cnt; i++) {
+        out[i]->r.n_records = (int64_t)out[i]->recs.size();
+        out[i]->r.records = out[i]->recs.data();
+        s->ready.push_back(out[i]->r.request);
+    }
+    if (cnt) {
+        const Pending &last = s->pending[cnt - 1];
+        s->patch_completed = last.patch_off + last.n_patches;
+    }
+    s->pending.erase(s->pending.begin(), s->pending.begin() + cnt);
+    s->completed += (uint64_t)cnt;
+    s->records += n;
+    s->requests_done += cnt;
+    s->device_ms += ms;
+    s->checks += (double)si.literals_after * assigs;
+    s->cv_done.notify_all();
+}
+
+void serve_loop(gss_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] {
+            return s->stop || s->err_code || can_launch(s) || !s->inflight.empty();
+        });
+        if (s->err_code) break;
+        if (can_launch(s)) {
+            launch(s, lk);
+            continue;
+        }
+        if (!s->inflight.empty()) {
+            complete_oldest(s, lk);
+            continue;
+        }
+        if (s->stop) break;
+    }
+    // a failed server still drains its steps, so the device is idle at exit
+    while (!s->inflight.empty()) {
+        lk.unlock();
+        int64_t n;
+        float ms;
+        (void)gss_step_wait(s->e, &n, &ms);
+        lk.lock();
+        s->inflight.pop_front();
+    }
+}
+
+int check(gss_server *s) { return s ? GSS_OK : fail(GSS_E_INVAL, "null server"); }
+
+void destroy(gss_server *s) {
+    if (s->th.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->th.join();
+    }
+    if (s->e) gss_engine_destroy(s->e);  // drains the engine stream first
+    if (s->reqs) (void)hipHostFree(s->reqs);
+    if (s->patches) (void)hipHostFree(s->patches);
+    if (s->head) (void)hipHostFree(s->head);
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gss_server_default_config(gss_server_config *c) {
+    if (!c) return fail(GSS_E_INVAL, "null config");
+    std::memset(c, 0, sizeof *c);
+    c->var_count = 0;
+    c->solvers = 1;
+    c->words = 64;
+    c->hbm_budget = 120e9;
+    c->max_inflight = 2;
+    c->records = GSS_RECORDS_FULL;
+    c->request_ring = 4096;
+    c->patch_ring = 8 << 20;
+    return GSS_OK;
+}
+
+int gss_server_create(int device, const gss_server_config *c, gss_server **out) {
+    if (!c || !out) return fail(GSS_E_INVAL, "null argument");
+    *out = nullptr;
+    if (c->var_count < 1 || c->solvers < 1 || c->solvers > 65535 || c->words < 1 ||
+        c->words > GSS_MAX_WORDS || (c->words & (c->words - 1)) || c->max_inflight < 1 ||
+        c->max_inflight > 2 || (c->records != GSS_RECORDS_FULL && c->records != GSS_RECORDS_IDS) ||
+        c->request_ring < 1 || c->patch_ring < 1 || !(c->hbm_budget > 0))
+        return fail(GSS_E_INVAL, "bad server configuration");
+    gss_server *s = new (std::nothrow) gss_server();
+    if (!s) return fail(GSS_E_NOMEM, "server allocation failed");
+    s->cfg = *c;
+    int rc;
+    if ((rc = gss_engine_create(device, &s->e)) || (rc = gss_assig_reset(s->e, c->var_count, c->words)) ||
+        (rc = gss_bases_reset(s->e, c->solvers))) {
+        const std::string m = gss_last_error();
+        destroy(s);
+        return fail(rc, m);
+    }
+    // rings the host writes while the device may read them: fine-grained
+    const unsigned fl = hipHostMallocCoherent;
+    if (hipHostMalloc((void **)&s->reqs, (size_t)c->request_ring * sizeof(ServeReq), fl) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&s->patches, (size_t)c->patch_ring * sizeof(gss_var_patch), fl) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&s->head, 64, fl) != hipSuccess) {
+        destroy(s);
+        return fail(GSS_E_NOMEM, "pinned ring allocation failed");
+    }
+    __atomic_store_n(s->head, (uint64_t)0, __ATOMIC_RELEASE);
+    if ((rc = gss::engine_serve_attach(s->e, s->reqs, (uint32_t)c->request_ring, s->patches,
+                                       (uint64_t)c->patch_ring, s->head))) {
+        const std::string m = gss_last_error();
+        destroy(s);
+        return fail(rc, m);
+    }
+    s->th = std::thread(serve_loop, s);
+    *out = s;
+    return GSS_OK;
+}
+
+int gss_server_destroy(gss_server *s) {
+    if (!s) return GSS_OK;
+    destroy(s);
+    return GSS_OK;
+}
+
+int gss_server_engine(gss_server *s, gss_engine **out) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!out) return fail(GSS_E_INVAL, "null out");
+    *out = s->e;
+    return GSS_OK;
+}
+
+int gss_server_update_base(gss_server *s, int64_t n, const gss_base_update *ups) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (n < 0 || (n && !ups)) return fail(GSS_E_INVAL, "bad base updates");
+    for (int64_t i = 0; i < n; i++)
+        if (ups[i].var >= (uint64_t)s->cfg.var_count || ups[i].base >= s->cfg.solvers)
+            return fail(GSS_E_INVAL, "base update " + std::to_string(i) + " out of range");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->err_code) return fail(s->err_code, s->err);
+    s->bups.insert(s->bups.end(), ups, ups + n);
+    s->bups_submitted += n;
+    return GSS_OK;
+}
+
+int gss_server_add_clauses(gss_server *s, int64_t n, const uint32_t *lits, const int32_t *lens,
+                           int64_t *first_id) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (n < 0 || (n && (!lits || !lens)) || !first_id) return fail(GSS_E_INVAL, "bad clauses");
+    *first_id = -1;
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
+            return fail(GSS_E_INVAL, "clause " + std::to_string(i) + ": bad length");
+        total += lens[i];
+    }
+    for (int64_t i = 0; i < total; i++)
+        if ((int64_t)(lits[i] >> 1) >= s->cfg.var_count)
+            return fail(GSS_E_INVAL, "literal " + std::to_string(lits[i]) + " past the variables");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->err_code) return fail(s->err_code, s->err);
+    if (4.0 * (double)(s->literals_admitted + total) > s->cfg.hbm_budget) {
+        s->refused += n;
+        return GSS_OK;  // over the budget: refused, *first_id = -1
+    }
+    *first_id = s->clauses_admitted;
+    s->cl_lits.insert(s->cl_lits.end(), lits, lits + total);
+    s->cl_lens.insert(s->cl_lens.end(), lens, lens + n);
+    s->clauses_admitted += n;
+    s->literals_admitted += total;
+    return GSS_OK;
+}
+
+int gss_server_submit(gss_server *s, int solver, int k, int64_t n_patches,
+                      const gss_var_patch *patches, int64_t *out_id) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!out_id || n_patches < 0 || (n_patches && !patches))
+        return fail(GSS_E_INVAL, "bad request");
+    *out_id = -1;
+    if (solver < 0 || solver >= s->cfg.solvers || k < 1 || k > 64 * s->cfg.words)
+        return fail(GSS_E_INVAL, "request solver or size out of range");
+    const uint32_t nw = (uint32_t)(k + 63) / 64;
+    for (int64_t i = 0; i < n_patches; i++)
+        if (patches[i].var >= (uint64_t)s->cfg.var_count || patches[i].word >= nw)
+            return fail(GSS_E_INVAL, "patch " + std::to_string(i) + " out of range");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->err_code) return fail(s->err_code, s->err);
+        if (s->stop) return fail(GSS_E_STATE, "server stopping");
+        if (s->published - s->completed >= (uint64_t)s->cfg.request_ring)
+            return fail(GSS_E_STATE, "request ring full");
+        if (s->patch_published + (uint64_t)n_patches - s->patch_completed >
+            (uint64_t)s->cfg.patch_ring)
+            return fail(GSS_E_STATE, "patch ring full");
+        const uint64_t pcap = (uint64_t)s->cfg.patch_ring;
+        for (int64_t i = 0; i < n_patches; i++)
+            s->patches[(s->patch_published + (uint64_t)i) % pcap] = patches[i];
+        const uint64_t seq = s->published;
+        ServeReq q{};
+        q.solver = (uint32_t)solver;
+        q.k = (uint32_t)k;
+        q.patch_off = s->patch_published;
+        q.n_patches = (uint32_t)n_patches;
+        q.seq = seq;
+        s->reqs[seq % (uint64_t)s->cfg.request_ring] = q;
+        s->pending.push_back(
+            Pending{seq, solver, k, s->patch_published, (uint32_t)n_patches, Clock::now()});
+        s->patch_published += (uint64_t)n_patches;
+        s->pending_words += nw;
+        s->published = seq + 1;
+        // the entries above are written before the device can see the new head
+        __atomic_store_n(s->head, s->published, __ATOMIC_RELEASE);
+        *out_id = (int64_t)seq;
+    }
+    s->cv_work.notify_one();
+    return GSS_OK;
+}
+
+int gss_server_poll(gss_server *s, int64_t max, int64_t *ids, int64_t *n, int64_t timeout_us) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!n || max < 0 || (max && !ids)) return fail(GSS_E_INVAL, "bad poll");
+    *n = 0;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->ready.empty() && timeout_us > 0)
+        s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us),
+                            [&] { return !s->ready.empty() || s->err_code; });
+    while (*n < max && !s->ready.empty()) {
+        ids[(*n)++] = s->ready.front();
+        s->ready.pop_front();
+    }
+    if (!*n && s->err_code) return fail(s->err_code, s->err);
+    return GSS_OK;
+}
+
+int gss_server_result(gss_server *s, int64_t id, gss_serve_result *out) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!out) return fail(GSS_E_INVAL, "null out");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->done.find(id);
+    if (it == s->done.end()) return fail(GSS_E_STATE, "request not completed (or released)");
+    *out = it->second.r;
+    return GSS_OK;
+}
+
+int gss_server_release(gss_server *s, int64_t id) {
+    int rc = check(s);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->done.erase(id);
+    return GSS_OK;
+}
+
+int gss_server_stats_get(gss_server *s, gss_server_stats *out) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!out) return fail(GSS_E_INVAL, "null out");
+    std::lock_guard<std::mutex> g(s->mu);
+    out->steps = s->steps;
+    out->requests_done = s->requests_done;
+    out->requests_pending = (int64_t)s->pending.size();
+    out->records = s->records;
+    out->clauses = s->clauses_admitted;
+    out->clauses_refused = s->refused;
+    out->literals = s->literals_admitted;
+    out->checks = s->checks;
+    out->device_ms = s->device_ms;
+    return GSS_OK;
+}
+
+}  // extern "C"

@@ -53,10 +53,22 @@ class Engine:
         _lib.call("gss_engine_create", device, ctypes.byref(self._h))
         self.words = 1
         self.vars = 0
+        self._dependents = []  # weak references to objects built on this engine
 
     # -- lifetime -------------------------------------------------------------
+    def add_dependent(self, obj) -> None:
+        """obj.close() runs before the engine is destroyed (whatever order the
+        garbage collector finds them in)."""
+        import weakref
+        self._dependents.append(weakref.ref(obj))
+
     def close(self) -> None:
         if self._h:
+            for ref in self._dependents:
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._dependents = []
             _lib.call("gss_engine_destroy", self._h)
             self._h = ctypes.c_void_p()
 

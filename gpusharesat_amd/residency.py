@@ -35,6 +35,7 @@ class Residency:
         self.max_tags = max_tags
         self._h = ctypes.c_void_p()
         _lib.call("gss_res_create", engine.handle, max_tags, decay, seed, ctypes.byref(self._h))
+        engine.add_dependent(self)  # the scorer goes before its engine
 
     def close(self) -> None:
         if self._h:
@@ -255,11 +256,21 @@ class SimResult:
 
 
 def simulate(cfg: SimConfig, policy: int, device: int = 0, train: bool = True,
-             snapshot: int = 8192, horizon: int = 5, log=None) -> SimResult:
+             snapshot: int = 8192, horizon: int = 5, log=None, after_reduce=None) -> SimResult:
     """Runs the residency workload under one policy (the learned one trains
-    online whatever the policy, so both runs do the same work)."""
-    gen = DriftingHotSets(cfg)
+    online whatever the policy, so both runs do the same work).
+    after_reduce(engine, round, evicted_tags), if given, runs after every
+    reduction (the tests check the kept DB there)."""
     e = Engine(device)
+    try:
+        return _simulate_on(e, cfg, policy, train, snapshot, horizon, log, after_reduce)
+    finally:
+        e.close()  # closes the scorer first, also when the run raised
+
+
+def _simulate_on(e: Engine, cfg: SimConfig, policy: int, train: bool, snapshot: int,
+                 horizon: int, log, after_reduce) -> SimResult:
+    gen = DriftingHotSets(cfg)
     e.enable_tags()
     e.reset_assignments(cfg.var_count, gen.words)
     max_tags = cfg.budget + cfg.arrivals // cfg.rounds + 1024
@@ -294,10 +305,11 @@ def simulate(cfg: SimConfig, policy: int, device: int = 0, train: bool = True,
             free[top:top + gone.size] = gone
             top += gone.size
             reductions += 1
+            if after_reduce is not None:
+                after_reduce(e, rnd, gone)
         if log and rnd % 25 == 0:
             log(f"policy {policy} round {rnd}: DB {e.totals()[0]}, hit pairs {pairs}, "
                 f"loss {loss:.4f}")
     out = SimResult(policy, pairs, checked, cfg.rounds, reductions, e.totals()[0], loss)
     res.close()
-    e.close()
     return out

@@ -1,16 +1,18 @@
-"""Serving loop over the batched step: requests (sets of assignments from a
-solver) arrive continuously and are admitted into the next step as table
-words free up, clause inserts are admitted under an HBM budget, two steps
-are in flight, and every request's records reach the host as soon as its
-step is done.
+"""Serving: requests (sets of assignments from a solver) arrive continuously
+and are checked against a device-resident clause DB that grows under an HBM
+budget.  The loop itself is native (include/gss_serve.h, GpuServer below):
+clients publish requests into a pinned ring, the library's serving thread
+keeps batched steps on the device, and each step admits the published
+requests that fit on the device when it starts.  This module drives it with
+the serving workload and measures it.
 
 A request of k assignments from solver s takes ceil(k / 64) consecutive
 table words.  The step loads those words from solver s's base (device-side
-packed planes, gss_bases_reset / gss_word_load), applies the base drift since
-s's previous request first (gss_base_update), and then the request's own
-per-slot differences as masked patches (gss_var_patch).  So the host sends
-only differences, never whole assignments, and no request depends on which
-words it lands in: its records are the same alone or in any mix.
+packed planes), after the base drift published before it, and then applies
+the request's own per-slot differences as masked patches (gss_var_patch).
+So the host sends only differences, never whole assignments, and no request
+depends on which words it lands in: its records are the same alone or in any
+mix.
 
 Workload (serving-32): 32 solvers over 2M variables, each assigning a
 fraction U[0.3, 0.7] of them; requests arrive as a Poisson process at 2000
@@ -28,7 +30,6 @@ from __future__ import annotations
 
 import ctypes
 import time
-from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -148,9 +149,11 @@ class ServeConfig:
     flips: int = 32                 # variables redrawn per assignment
     agree: float = 0.99             # an assigned variable takes the planted value
     lit_true: float = 0.9           # a clause literal is true under the planted assignment
-    pool: int = 48                  # distinct request payloads per size
+    planted: bool = True            # False: uniform polarities and random clause signs
+    pool: int = 48                  # distinct request payloads
     clause_pool: int = 2_000_000
-    max_inflight: int = 2           # steps on the device at once (1: each step starts once the previous has left it)
+    max_inflight: int = 2           # steps on the device at once
+    records: int = RECORDS_FULL     # RECORDS_IDS: clause and word only
     seed: int = 5
 
 
@@ -161,9 +164,6 @@ class Request:
     solver: int
     k: int
     payload: int
-    words: tuple = ()
-    t_done: float = 0.0
-    records: int = 0
 
 
 @dataclass
@@ -171,144 +171,203 @@ class ServeResult:
     requests: int
     steps: int
     elapsed: float
-    checks: int
+    checks: float
     latency_ms: np.ndarray
     clauses_end: int
     literals_end: int
     records: int
     max_queue: int
-    step_ms: list = field(default_factory=list)
-    arrive_s: np.ndarray = None  # arrival time of the request of each latency
+    device_ms: float = 0.0
+    arrive_s: np.ndarray = None     # arrival time of the request of each latency
+    kept: dict = field(default_factory=dict)  # request id -> (result, records), sampled
 
     @property
     def checks_per_s(self) -> float:
         return self.checks / self.elapsed
 
 
-class Server:
-    """The serving loop (host side) over one engine."""
+SERVE_HIT_DTYPE = np.dtype([("clause", "<i8"), ("word", "<u4"), ("pad", "<u4"),
+                            ("false_mask", "<u8"), ("unit_mask", "<u8")])
 
-    def __init__(self, cfg: ServeConfig, device: int = 0, log=None):
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("var_count", ctypes.c_int64), ("solvers", ctypes.c_int32),
+                ("words", ctypes.c_int32), ("hbm_budget", ctypes.c_double),
+                ("max_inflight", ctypes.c_int32), ("records", ctypes.c_int32),
+                ("request_ring", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("patch_ring", ctypes.c_int64)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_int64), ("step", ctypes.c_int64),
+                ("clauses", ctypes.c_int64), ("base_updates", ctypes.c_int64),
+                ("solver", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("first_word", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("latency_ms", ctypes.c_double), ("n_records", ctypes.c_int64),
+                ("records", ctypes.c_void_p)]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("requests_done", ctypes.c_int64),
+                ("requests_pending", ctypes.c_int64), ("records", ctypes.c_int64),
+                ("clauses", ctypes.c_int64), ("clauses_refused", ctypes.c_int64),
+                ("literals", ctypes.c_int64), ("checks", ctypes.c_double),
+                ("device_ms", ctypes.c_double)]
+
+
+class GpuServer:
+    """ctypes handle on the native serving loop (include/gss_serve.h)."""
+
+    def __init__(self, var_count: int, solvers: int, words: int = WORDS,
+                 hbm_budget: float = 120e9, max_inflight: int = 2, records: int = RECORDS_FULL,
+                 request_ring: int = 4096, patch_ring: int = 8 << 20, device: int = 0):
+        c = _ServerConfig()
+        _lib.call("gss_server_default_config", ctypes.byref(c))
+        c.var_count, c.solvers, c.words = var_count, solvers, words
+        c.hbm_budget, c.max_inflight, c.records = hbm_budget, max_inflight, records
+        c.request_ring, c.patch_ring = request_ring, patch_ring
+        self.words = words
+        self._h = ctypes.c_void_p()
+        _lib.call("gss_server_create", device, ctypes.byref(c), ctypes.byref(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.call("gss_server_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def engine_handle(self):
+        e = ctypes.c_void_p()
+        _lib.call("gss_server_engine", self._h, ctypes.byref(e))
+        return e
+
+    def set_tuning(self, settings: dict) -> None:
+        from .engine import TUNE_KEYS
+        e = self.engine_handle()
+        for name, value in settings.items():
+            _lib.call("gss_set_tuning", e, TUNE_KEYS[name], int(value))
+
+    def update_base(self, ups: np.ndarray) -> None:
+        ups = np.ascontiguousarray(ups, BASE_UP_DTYPE)
+        if ups.size:
+            _lib.call("gss_server_update_base", self._h, ups.size, _p(ups))
+
+    def add_clauses(self, lits: np.ndarray, lens: np.ndarray) -> int:
+        """First id of the admitted clauses, or -1 if the budget refused them."""
+        lits = np.ascontiguousarray(lits, np.uint32)
+        lens = np.ascontiguousarray(lens, np.int32)
+        first = ctypes.c_int64()
+        _lib.call("gss_server_add_clauses", self._h, lens.size, _p(lits), _p(lens),
+                  ctypes.byref(first))
+        return first.value
+
+    def submit(self, solver: int, k: int, patches: np.ndarray) -> int | None:
+        """Request id, or None if a ring is full."""
+        patches = np.ascontiguousarray(patches, PATCH_DTYPE)
+        rid = ctypes.c_int64()
+        rc = _lib.load().gss_server_submit(self._h, solver, k, patches.size, _p(patches),
+                                            ctypes.byref(rid))
+        if rc == -4 and b"full" in _lib.load().gss_last_error():
+            return None
+        _lib.check("gss_server_submit", rc)
+        return rid.value
+
+    def poll(self, max_ids: int = 1024, timeout_us: int = 0) -> list[int]:
+        ids = (ctypes.c_int64 * max_ids)()
+        n = ctypes.c_int64()
+        _lib.call("gss_server_poll", self._h, max_ids, ids, ctypes.byref(n), int(timeout_us))
+        return list(ids[:n.value])
+
+    def result(self, rid: int, records: bool = True) -> tuple[dict, np.ndarray | None]:
+        r = _Result()
+        _lib.call("gss_server_result", self._h, rid, ctypes.byref(r))
+        meta = {f: getattr(r, f) for f, _ in _Result._fields_ if f not in ("records", "pad")}
+        recs = None
+        if records:
+            if r.n_records:
+                buf = (ctypes.c_char * (r.n_records * SERVE_HIT_DTYPE.itemsize)).from_address(
+                    r.records)
+                recs = np.frombuffer(buf, SERVE_HIT_DTYPE).copy()
+            else:
+                recs = np.zeros(0, SERVE_HIT_DTYPE)
+        return meta, recs
+
+    def release(self, rid: int) -> None:
+        _lib.call("gss_server_release", self._h, rid)
+
+    def stats(self) -> dict:
+        st = _Stats()
+        _lib.call("gss_server_stats_get", self._h, ctypes.byref(st))
+        return {f: getattr(st, f) for f, _ in _Stats._fields_}
+
+
+class Server:
+    """The serving-32 workload driven through the native loop: arrivals on a
+    Poisson schedule (base drift, clause growth and the request published at
+    each arrival), completions polled between arrivals."""
+
+    def __init__(self, cfg: ServeConfig, device: int = 0, log=None, keep_history: bool = False):
         from . import workloads
         self.cfg = cfg
         self.log = log or (lambda m: None)
         rng = np.random.default_rng(cfg.seed)
         self.rng = rng
-        self.e = Engine(device)
-        self.e.reset_assignments(cfg.var_count, WORDS)
-        bases_reset(self.e, cfg.solvers)
+        self.srv = GpuServer(cfg.var_count, cfg.solvers, WORDS, cfg.hbm_budget,
+                             cfg.max_inflight, cfg.records, device=device)
+        self.device = device
         self.p_assigned = rng.uniform(cfg.p_min, cfg.p_max, size=cfg.solvers)
         self.sigma = rng.random(cfg.var_count) < 0.5  # the planted assignment
-        # payload pools: per solver-independent sets of 1024 slots; a request
-        # of k assignments takes the patches of its first ceil(k/64) words and
-        # the slots below k
-        # kept as uint64 [n, 4] rows sorted by word (first column = var |
-        # word << 32), with the first row of every word, so a request takes a
-        # prefix and shifts its words with one add
+        # payload pools: per solver-independent sets of 1024 slots, patches in
+        # relative words sorted by word, with the first patch of every word; a
+        # request of k assignments sends the patches of its first ceil(k/64)
+        # words (slots past k in its last word are not checked)
+        agree = cfg.agree if cfg.planted else 0.5
+        sigma = self.sigma if cfg.planted else None
         self.pool, self.pool_off = [], []
         for _ in range(cfg.pool):
-            pt = slot_patches(rng, cfg.var_count, 1024, cfg.flips, 0.5, self.sigma, cfg.agree)
+            pt = slot_patches(rng, cfg.var_count, 1024, cfg.flips, 0.5, sigma, agree)
             pt = pt[np.argsort(pt["word"], kind="stable")]
-            self.pool.append(pt.view(np.uint64).reshape(-1, 4).copy())
+            self.pool.append(pt)
             self.pool_off.append(np.searchsorted(pt["word"], np.arange(17)))
-        self.log(f"payload pool: {sum(p.shape[0] for p in self.pool)} patches")
+        self.log(f"payload pool: {sum(p.size for p in self.pool)} patches")
         lens = workloads.clause_lengths(rng, cfg.clause_pool, 0.1, 200)
         self.cl_lens = lens
         self.cl_off = np.zeros(lens.size + 1, np.int64)
         np.cumsum(lens, out=self.cl_off[1:])
         v = workloads.distinct_vars(rng, lens, cfg.var_count)
-        st = rng.random(v.size) < cfg.lit_true
-        # literal 2v + 1 is the negation: true under sigma iff sigma(v) is false
-        self.cl_lits = ((v << 1) | (st != self.sigma[v])).astype(np.uint32)
+        if cfg.planted:
+            st = rng.random(v.size) < cfg.lit_true
+            # literal 2v + 1 is the negation: true under sigma iff sigma(v) is false
+            self.cl_lits = ((v << 1) | (st != self.sigma[v])).astype(np.uint32)
+        else:
+            self.cl_lits = ((v << 1) | rng.integers(0, 2, size=v.size)).astype(np.uint32)
         self.cl_next = 0
+        self.keep_history = keep_history
+        self.base_history: list[np.ndarray] = []   # base updates in submission order
+        self.clause_ids: list[tuple[int, int, int]] = []  # (first id, pool lo, pool hi)
 
-    def initial_bases(self) -> None:
-        """Every solver's base: each variable assigned with the solver's
-        probability, polarity Bernoulli(0.5), sent in chunks of steps."""
-        c, rng = self.cfg, self.rng
-        for s in range(c.solvers):
-            vs = np.flatnonzero(rng.random(c.var_count) < self.p_assigned[s]).astype(np.uint32)
-            u = np.zeros(vs.size, BASE_UP_DTYPE)
-            u["var"] = vs
-            u["base"] = s
-            u["is_def"] = 1
-            u["is_true"] = (rng.random(vs.size) < c.agree) == self.sigma[vs]
-            submit(self.e, np.zeros(WORDS, np.uint64), base_ups=u)
-            self.e.step_wait()
-
-    def schedule(self) -> list[Request]:
-        c, rng = self.cfg, self.rng
-        n = rng.poisson(c.rate * c.duration)
-        t = np.sort(rng.random(n) * c.duration)
-        return [Request(i, float(t[i]), int(rng.integers(0, c.solvers)),
-                        int(rng.integers(c.set_min, c.set_max + 1)),
-                        int(rng.integers(0, c.pool))) for i in range(n)]
-
-    def _clauses(self, n: int):
-        c = self.cfg
-        lo = self.cl_next
-        hi = min(lo + n, c.clause_pool)
-        self.cl_next = hi if hi < c.clause_pool else 0
-        a, b = self.cl_off[lo], self.cl_off[hi]
-        return self.cl_lits[a:b], self.cl_lens[lo:hi]
-
-    def _drift(self, s: int) -> np.ndarray:
-        c, rng = self.cfg, self.rng
-        u = np.zeros(c.drift, BASE_UP_DTYPE)
-        u["var"] = rng.integers(0, c.var_count, size=c.drift)  # repeats removed in build()
-        u["base"] = s
-        d = rng.random(c.drift) < self.p_assigned[s]
-        u["is_def"] = d
-        u["is_true"] = d & ((rng.random(c.drift) < c.agree) == self.sigma[u["var"]])
-        return u
-
-    def build(self, reqs: list[Request], n_clauses: int):
-        """Step inputs for admitted requests (words assigned in order)."""
-        valid = np.zeros(WORDS, np.uint64)
-        loads, patches, bups = [], [], []
-        w0 = 0
-        for r in reqs:
-            nw = (r.k + 63) // 64
-            r.words = (w0, w0 + nw)
-            bups.append(self._drift(r.solver))
-            ld = np.zeros(nw, LOAD_DTYPE)
-            ld["word"] = np.arange(w0, w0 + nw)
-            ld["base"] = r.solver
-            loads.append(ld)
-            pool, off = self.pool[r.payload], self.pool_off[r.payload]
-            last = r.k - 64 * (nw - 1)  # slots used in the request's last word
-            full_end = off[nw - 1] if last < 64 else off[nw]
-            patches.append((pool[:full_end], w0))
-            if last < 64:  # the last word keeps only the request's slots
-                t = pool[off[nw - 1]:off[nw]].copy()
-                t[:, 1] &= np.uint64((1 << last) - 1)
-                patches.append((t[t[:, 1] != 0], w0))
-            valid[w0:w0 + nw - 1] = ONES
-            valid[w0 + nw - 1] = ONES if last == 64 else np.uint64((1 << last) - 1)
-            w0 += nw
-        lits, lens = self._clauses(n_clauses) if n_clauses else (None, None)
-        cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
-        bu = cat(bups, BASE_UP_DTYPE)
-        if bu.size:  # one update per (base, var) in a step: the last one wins
-            key = (bu["base"].astype(np.int64) << 32) | bu["var"]
-            _, last = np.unique(key[::-1], return_index=True)
-            bu = bu[bu.size - 1 - last]
-        # one pass: copy every request's rows into place and shift its words
-        out = np.empty((sum(a.shape[0] for a, _ in patches), 4), np.uint64)
-        at = 0
-        for a, w0 in patches:
-            np.add(a, np.array([w0 << 32, 0, 0, 0], np.uint64), out=out[at:at + a.shape[0]])
-            at += a.shape[0]
-        pt = out.view(PATCH_DTYPE).ravel()
-        return dict(valid=valid, lits=lits, lens=lens, patches=pt, base_ups=bu,
-                    loads=cat(loads, LOAD_DTYPE))
+    @property
+    def e(self):
+        return self.srv
 
     def tune(self, clauses: int = 2_000_000) -> dict:
         """Chooses the sweep tiles on a scratch engine holding `clauses` pooled
-        clauses and a 64-word table of the solvers' bases, and applies them
-        to the serving engine (whose DB starts empty)."""
+        clauses and a 64-word table, and applies them to the server's engine
+        (whose DB starts empty)."""
+        from .engine import Engine
         c = self.cfg
-        t = Engine(self.e.device)
+        t = Engine(self.device)
         try:
             n = min(clauses, c.clause_pool)
             t.append(self.cl_lits[:self.cl_off[n]], self.cl_lens[:n])
@@ -326,104 +385,132 @@ class Server:
             best, ms = t.autotune(np.full(WORDS, ONES))
         finally:
             t.close()
-        self.e.apply_tuning(best)
+        self.srv.set_tuning(best)
         self.log(f"tuned on {n} clauses: {best} ({ms:.3f} ms per sweep)")
         return best
 
-    def run(self) -> ServeResult:
-        c, e = self.cfg, self.e
+    def _update_base(self, u: np.ndarray) -> None:
+        self.srv.update_base(u)
+        if self.keep_history:
+            self.base_history.append(u.copy())
+
+    def initial_bases(self) -> None:
+        """Every solver's base: each variable assigned with the solver's
+        probability, planted polarity (uniform without the planted
+        assignment); published, then checked by a first empty-patch request."""
+        c, rng = self.cfg, self.rng
+        for s in range(c.solvers):
+            vs = np.flatnonzero(rng.random(c.var_count) < self.p_assigned[s]).astype(np.uint32)
+            u = np.zeros(vs.size, BASE_UP_DTYPE)
+            u["var"] = vs
+            u["base"] = s
+            u["is_def"] = 1
+            agree = c.agree if c.planted else 0.5
+            u["is_true"] = (rng.random(vs.size) < agree) == self.sigma[vs]
+            self._update_base(u)
+        rid = self.srv.submit(0, 64, np.zeros(0, PATCH_DTYPE))
+        while rid not in self.srv.poll(16, 1_000_000):
+            pass
+        self.srv.release(rid)
+
+    def schedule(self) -> list[Request]:
+        c, rng = self.cfg, self.rng
+        n = rng.poisson(c.rate * c.duration)
+        t = np.sort(rng.random(n) * c.duration)
+        return [Request(i, float(t[i]), int(rng.integers(0, c.solvers)),
+                        int(rng.integers(c.set_min, c.set_max + 1)),
+                        int(rng.integers(0, c.pool))) for i in range(n)]
+
+    def _clauses(self, n: int):
+        c = self.cfg
+        lo = self.cl_next
+        hi = min(lo + n, c.clause_pool)
+        self.cl_next = hi if hi < c.clause_pool else 0
+        a, b = self.cl_off[lo], self.cl_off[hi]
+        return self.cl_lits[a:b], self.cl_lens[lo:hi], lo, hi
+
+    def _drift(self, s: int) -> np.ndarray:
+        c, rng = self.cfg, self.rng
+        u = np.zeros(c.drift, BASE_UP_DTYPE)
+        u["var"] = rng.integers(0, c.var_count, size=c.drift)
+        u["base"] = s
+        d = rng.random(c.drift) < self.p_assigned[s]
+        u["is_def"] = d
+        agree = c.agree if c.planted else 0.5
+        u["is_true"] = d & ((rng.random(c.drift) < agree) == self.sigma[u["var"]])
+        return u
+
+    def request_patches(self, r: Request) -> np.ndarray:
+        nw = (r.k + 63) // 64
+        return self.pool[r.payload][:self.pool_off[r.payload][nw]]
+
+    def run(self, keep=None) -> ServeResult:
+        """keep(request) -> bool selects requests whose records are kept."""
+        c, srv = self.cfg, self.srv
         self.initial_bases()
         reqs = self.schedule()
         self.log(f"{len(reqs)} requests over {c.duration:.0f} s")
-        pending: deque[int] = deque()
-        inflight: deque = deque()
-        lat = []
-        arrive = []
+        by_id: dict[int, Request] = {}
+        t_sub: dict[int, float] = {}
+        lat, arrive = [], []
+        kept = {}
+        st0 = srv.stats()
         ai = 0
-        steps = 0
-        checks = 0
-        records = 0
         max_queue = 0
-        step_ms = []
-        lits_db = e.totals()[1]
-        start = time.perf_counter()
         last_cl = 0.0
         last_log = 0.0
+        backlog: list[Request] = []
+        start = time.perf_counter()
 
-        def arrivals(now):
-            nonlocal ai, max_queue
+        def collect(ids):
+            for rid in ids:
+                r = by_id.pop(rid)
+                want = keep is not None and keep(r)
+                meta, recs = srv.result(rid, records=want)
+                srv.release(rid)
+                lat.append((t_sub.pop(rid) - r.t_arrive) * 1e3 + meta["latency_ms"])
+                arrive.append(r.t_arrive)
+                if want:
+                    kept[rid] = (r, meta, recs)
+
+        while ai < len(reqs) or by_id or backlog:
+            now = time.perf_counter() - start
             while ai < len(reqs) and reqs[ai].t_arrive <= now:
-                pending.append(ai)
+                r = reqs[ai]
                 ai += 1
-            max_queue = max(max_queue, len(pending))
-
-        def admit(now):
-            """Submits one step with the pending requests whose words fit."""
-            nonlocal last_cl, lits_db, checks, steps, last_log
-            batch, used = [], 0
-            while pending:
-                r = reqs[pending[0]]
-                nw = (r.k + 63) // 64
-                if used + nw > WORDS:
-                    break
-                batch.append(r)
-                used += nw
-                pending.popleft()
-            n_cl = 0
-            if now <= c.duration:
-                n_cl = int(c.clause_rate * (now - last_cl))
-                if 4 * (lits_db + 11 * n_cl) > c.hbm_budget:
-                    n_cl = 0
-                last_cl = now
-            st = self.build(batch, n_cl)
-            submit(e, **st)
-            if st["lens"] is not None:
-                lits_db += int(st["lits"].size)
-            checks += lits_db * sum(r.k for r in batch)
-            inflight.append(([r.idx for r in batch], now))
-            steps += 1
+                self._update_base(self._drift(r.solver))
+                if now <= c.duration:
+                    n_cl = int(c.clause_rate * (now - last_cl))
+                    last_cl = now
+                    if n_cl:
+                        lits, lens, lo, hi = self._clauses(n_cl)
+                        first = srv.add_clauses(lits, lens)
+                        if first >= 0 and self.keep_history:
+                            self.clause_ids.append((first, lo, hi))
+                backlog.append(r)
+            while backlog:
+                r = backlog[0]
+                t = time.perf_counter() - start
+                rid = srv.submit(r.solver, r.k, self.request_patches(r))
+                if rid is None:
+                    break  # ring full: retry after completions
+                backlog.pop(0)
+                by_id[rid] = r
+                t_sub[rid] = t
+            max_queue = max(max_queue, len(by_id) + len(backlog))
+            nxt = reqs[ai].t_arrive - (time.perf_counter() - start) if ai < len(reqs) else 0.002
+            collect(srv.poll(1024, int(max(0.0, min(nxt, 0.002)) * 1e6)))
             if now - last_log > 5.0:
                 last_log = now
-                self.log(f"t={now:.1f}s steps {steps} served {len(lat)} queue {len(pending)} "
-                         f"DB {e.totals()[0]} clauses")
-
-        def finish():
-            """Waits for the oldest step; while its records travel to the host
-            the next step is admitted, then the records are read."""
-            nonlocal records
-            ids, t_sub = inflight.popleft()
-            _, ms = e.step_wait()
-            now = time.perf_counter() - start
-            arrivals(now)
-            if pending:
-                admit(now)
-            h = e.step_hits_view()
-            t = time.perf_counter() - start
-            records += h.size
-            step_ms.append(ms)
-            for i in ids:
-                r = reqs[i]
-                r.t_done = t
-                lat.append(t - r.t_arrive)
-                arrive.append(r.t_arrive)
-
-        while True:
-            now = time.perf_counter() - start
-            arrivals(now)
-            if ai >= len(reqs) and not pending and not inflight:
-                break
-            if len(inflight) >= c.max_inflight or (inflight and not pending):
-                finish()
-                continue
-            if not pending:
-                nxt = reqs[ai].t_arrive if ai < len(reqs) else now
-                time.sleep(max(0.0, min(nxt - now, 0.0005)))
-                continue
-            admit(now)
+                st = srv.stats()
+                self.log(f"t={now:.1f}s steps {st['steps']} served {len(lat)} queue "
+                         f"{len(by_id) + len(backlog)} DB {st['clauses']} clauses")
         elapsed = time.perf_counter() - start
-        cl, li = e.totals()
-        return ServeResult(len(lat), steps, elapsed, checks, np.array(lat) * 1e3, cl, li,
-                           records, max_queue, step_ms, np.array(arrive))
+        st = srv.stats()
+        return ServeResult(len(lat), st["steps"] - st0["steps"], elapsed,
+                           st["checks"] - st0["checks"], np.array(lat), st["clauses"],
+                           st["literals"], st["records"] - st0["records"], max_queue,
+                           st["device_ms"] - st0["device_ms"], np.array(arrive), kept)
 
     def close(self):
-        self.e.close()
+        self.srv.close()

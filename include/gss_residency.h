@@ -42,7 +42,8 @@ typedef struct gss_res gss_res;
 
 /* Creates the scorer over engine e (whose DB must be tagged) for tags in
  * [0, max_tags); decay is the per-added-clause activity decay (the
- * reference's 0.99999); seed initialises the MLP. */
+ * reference's 0.99999); seed initialises the MLP.  The scorer uses the
+ * engine until gss_res_destroy, which must come before gss_engine_destroy. */
 int gss_res_create(gss_engine *e, int64_t max_tags, float decay, uint64_t seed, gss_res **out);
 int gss_res_destroy(gss_res *r);
 

@@ -7,7 +7,7 @@ import re
 from gpusharesat_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = ["gss.h", "gss_sharer.h", "gss_residency.h", "gss_ceilings.h"]
+HEADERS = ["gss.h", "gss_sharer.h", "gss_residency.h", "gss_ceilings.h", "gss_serve.h"]
 
 
 def declared_symbols():

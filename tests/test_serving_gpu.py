@@ -1,11 +1,17 @@
-"""Serving steps (gss_step_submit with base planes, word loads and patches).
+"""Serving: the steps it is built on (gss_step_submit with base planes, word
+loads and patches) and the native serving loop (include/gss_serve.h).
 
 * Base updates, word loads and patches leave the table exactly as the
   explicit per-slot assignments say (read back with gss_assig_download).
 * A request's records are the same alone and in a mix (other requests
   around it, at another word offset), and equal the oracle's on the
-  request's explicit assignments (its solver's base plus its patches).
+  request's explicit assignments (its solver's base plus its patches):
+  through steps, and through the native loop's C ABI, where the requests
+  of a burst are admitted together on the device.
 * A short run of the serving loop completes every request.
+* serving-32 itself (32 solvers over 2M variables, Poisson arrivals, clause
+  growth) for two seconds: sampled requests' records equal the oracle's on
+  the DB and base their step had.
 """
 import numpy as np
 import pytest
@@ -178,3 +184,152 @@ def test_short_serving_run_completes_every_request():
     print(out.requests, out.steps, np.percentile(out.latency_ms, [50, 99]), flush=True)
     assert out.requests > 300 and out.steps > 0 and out.checks > 0
     assert (out.latency_ms > 0).all()
+
+
+def _explicit_request(bd, bt, patches, k):
+    """Planes [V, nw] of a request: its solver's base in every slot, then its
+    patches (words relative to the request)."""
+    nw = (k + 63) // 64
+    V = bd.size
+    d = np.repeat(np.where(bd, ALL, np.uint64(0))[:, None], nw, axis=1)
+    t = np.repeat(np.where(bt, ALL, np.uint64(0))[:, None], nw, axis=1)
+    if patches.size:
+        v = patches["var"].astype(np.int64)
+        w = patches["word"].astype(np.int64)
+        m = patches["mask"]
+        d[v, w] = (d[v, w] & ~m) | (patches["is_def"] & m)
+        t[v, w] = (t[v, w] & ~m) | (patches["is_true"] & patches["is_def"] & m)
+    assert V == d.shape[0]
+    return d, t & d
+
+
+def _oracle_request(off, lits, n_clauses, d, t, k):
+    return orc.sweep(off[:n_clauses + 1], lits[:off[n_clauses]], d, t, request_valid(k))
+
+
+def _as_oracle(recs):
+    out = np.empty(recs.size, orc.HIT_DTYPE)
+    out["clause"], out["word"] = recs["clause"], recs["word"]
+    out["false_mask"], out["unit_mask"] = recs["false_mask"], recs["unit_mask"]
+    return out[np.lexsort((out["word"], out["clause"]))]
+
+
+def _same(got, exp, what):
+    assert got.size == exp.size, (what, got.size, exp.size)
+    for f in ("clause", "word", "false_mask", "unit_mask"):
+        assert np.array_equal(got[f], exp[f]), (what, f)
+
+
+def test_native_loop_request_alone_equals_in_mix_and_oracle():
+    """Through the C ABI: request R alone, then copies of R inside a burst
+    of other requests (admitted together on the device, at other word
+    offsets): every copy's records equal R alone, and the oracle's on R's
+    explicit assignments."""
+    rng = np.random.default_rng(21)
+    V, W, N = 500, 16, 4000
+    lens = rng.integers(1, 13, size=N).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    off = np.zeros(N + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    bd, bt = random_bases(rng, 3, V)
+    with sv.GpuServer(V, 3, words=W) as srv:
+        assert srv.add_clauses(lits, lens) == 0
+        srv.update_base(base_updates(bd, bt))
+
+        def req(k):
+            p = sv.slot_patches(rng, V, k, 6, 0.6)
+            return p[p["word"] < (k + 63) // 64]
+
+        R = req(200)
+
+        def run(batch):
+            ids = [srv.submit(s, k, p) for s, k, p in batch]
+            assert None not in ids
+            got = {}
+            while len(got) < len(ids):
+                for rid in srv.poll(64, 2_000_000):
+                    got[rid] = srv.result(rid)
+                    srv.release(rid)
+            return [got[i] for i in ids]
+
+        (meta, alone), = run([(1, 200, R)])
+        assert meta["first_word"] == 0 and meta["clauses"] == N and alone.size > 0
+        d, t = _explicit_request(bd[1], bt[1], R, 200)
+        _same(_as_oracle(alone), _oracle_request(off, lits, N, d, t, 200), "R alone vs oracle")
+        burst = []
+        for _ in range(6):
+            burst.append((0, int(rng.integers(1, 300)), None))
+            burst.append((1, 200, R))
+            burst.append((2, int(rng.integers(1, 200)), None))
+        burst = [(s_, k, p if p is not None else req(k)) for s_, k, p in burst]
+        out = run(burst)
+        steps = {}
+        for (s_, k, _), (m, _) in zip(burst, out):
+            steps.setdefault(m["step"], []).append(m)
+        mixed = 0
+        for (s_, k, p), (m, recs) in zip(burst, out):
+            if p is R:
+                _same(_as_oracle(recs), _as_oracle(alone), f"R at word {m['first_word']}")
+                if m["first_word"] > 0 and len(steps[m["step"]]) > 1:
+                    mixed += 1
+        assert mixed >= 1, steps
+        st = srv.stats()
+        assert st["requests_done"] == 1 + len(burst) and st["requests_pending"] == 0
+
+
+def test_serving32_workload_sampled_requests_match_oracle():
+    """serving-32 through the native loop for two seconds (32 solvers over
+    2M variables, Poisson 2000 requests/s of U[64, 1024] assignments, clause
+    growth at 200k/s): at least 50 sampled requests, each checked against
+    the oracle on its explicit assignments (its solver's base after the base
+    updates its step had applied, plus its patches) and the DB its step had."""
+    cfg = sv.ServeConfig(duration=2.0, records=sv.RECORDS_FULL)
+    srv = sv.Server(cfg, log=print, keep_history=True)
+    big = []
+
+    def keep(r):
+        if r.k <= 128:
+            return True
+        if 448 < r.k <= 512 and len(big) < 3:
+            big.append(r.idx)
+            return True
+        return False
+
+    try:
+        out = srv.run(keep=keep)
+    finally:
+        srv.close()
+    print(out.requests, out.steps, np.percentile(out.latency_ms, [50, 99]), len(out.kept),
+          flush=True)
+    assert out.requests > 3000 and len(out.kept) >= 50
+    # the DB: clause id i is pool clause i (no budget refusals, no wrap)
+    for first, lo, hi in srv.clause_ids:
+        assert first == lo
+    off, lits = srv.cl_off, srv.cl_lits
+    # replay the base updates in submission order, requests sorted by that count
+    hist = srv.base_history
+    cum = np.cumsum([0] + [h.size for h in hist])
+    bd = np.zeros((cfg.solvers, cfg.var_count), bool)
+    bt = np.zeros((cfg.solvers, cfg.var_count), bool)
+    applied = 0
+    items = sorted(out.kept.values(), key=lambda x: x[1]["base_updates"])[:60]
+    checked = 0
+    for r, meta, recs in items:
+        need = meta["base_updates"]
+        while applied < len(hist) and cum[applied + 1] <= need:
+            u = hist[applied]
+            # a variable drawn twice in one batch keeps its last update
+            key = u["base"].astype(np.int64) * cfg.var_count + u["var"]
+            _, last = np.unique(key[::-1], return_index=True)
+            u = u[u.size - 1 - last]
+            bd[u["base"], u["var"]] = u["is_def"].astype(bool)
+            bt[u["base"], u["var"]] = (u["is_def"] & u["is_true"]).astype(bool)
+            applied += 1
+        assert cum[applied] == need, "a step applied part of one update batch"
+        d, t = _explicit_request(bd[r.solver], bt[r.solver], srv.request_patches(r), r.k)
+        exp = _oracle_request(off, lits, meta["clauses"], d, t, r.k)
+        print(f"request {meta['request']} (solver {r.solver}, k {r.k}, step {meta['step']}, "
+              f"{meta['clauses']} clauses): {recs.size} records", flush=True)
+        _same(_as_oracle(recs), exp, f"request {meta['request']}")
+        checked += 1
+    assert checked >= 50
