@@ -77,6 +77,7 @@ SIGNATURES: dict[str, list] = {
     "gss_server_submit": [c_p, c_int, c_int, c_i64, c_p, c_p],
     "gss_server_poll": [c_p, c_i64, c_p, c_p, c_i64],
     "gss_server_result": [c_p, c_i64, c_p],
+    "gss_server_records": [c_p, c_i64, c_p, c_p],
     "gss_server_release": [c_p, c_i64],
     "gss_server_stats_get": [c_p, c_p],
     "gss_step_submit": [c_p, c_p],

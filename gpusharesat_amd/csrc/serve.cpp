@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -52,9 +53,20 @@ struct StepInfo {
     uint64_t published_at_launch = 0;
 };
 
+// One step's records grouped by request, as the step recorded them (rw
+// 8-byte words each; the first word is row | len << 32 | word << 48 with the
+// word made relative to the request).
+struct StepRecords {
+    std::vector<uint64_t> words;
+    int rw = 3;
+};
+
 struct Done {
     gss_serve_result r{};
-    std::vector<gss_serve_hit> recs;
+    std::shared_ptr<const StepRecords> step;
+    size_t first = 0;                 // its first record in step->words
+    std::vector<gss_serve_hit> recs;  // translated on request (gss_server_records)
+    bool translated = false;
 };
 
 }  // namespace
@@ -101,6 +113,19 @@ bool can_launch(const gss_server *s) {
     return s->published > s->completed &&
            (s->inflight.empty() || s->published > s->inflight.back().published_at_launch ||
             s->pending_words > (int64_t)s->inflight.size() * s->cfg.words);
+}
+
+// can_launch once the oldest step (which admitted `done` requests) is off
+// the device: it no longer counts as in flight.
+bool can_launch_after(const gss_server *s, int done) {
+    if (s->err_code || s->stop || (int)s->inflight.size() - 1 >= s->cfg.max_inflight) return false;
+    const uint64_t completed = s->completed + (uint64_t)done;
+    if (s->published <= completed) return false;
+    if (s->inflight.size() == 1) return true;  // only the finished step was in flight
+    int64_t words_done = 0;
+    for (int i = 0; i < done; i++) words_done += (s->pending[i].k + 63) / 64;
+    return s->published > s->inflight.back().published_at_launch ||
+           s->pending_words - words_done > (int64_t)(s->inflight.size() - 1) * s->cfg.words;
 }
 
 void set_failed(gss_server *s, int code, const char *what) {
@@ -171,7 +196,10 @@ void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
     s->inflight.push_back(si);
 }
 
-// Waits for the oldest step and hands each request it admitted its records.
+// Waits for the oldest step and hands each request it admitted its records:
+// two sequential passes over the step's records in pinned memory (count per
+// request, then copy grouped by request), outside the lock; clause ids are
+// looked up only when a client reads a request's records.
 void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     lk.unlock();
     int64_t n = 0;
@@ -179,12 +207,74 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     int rc = gss_step_wait(s->e, &n, &ms);
     ServeReport rep{};
     if (!rc) rc = gss::step_serve_report(s->e, &rep);
-    const gss_hit *full = nullptr;
-    const gss_hit_id *idr = nullptr;
-    if (!rc && n > 0)
-        rc = s->cfg.records == GSS_RECORDS_IDS ? gss_step_ids_map(s->e, &idr, &n)
-                                               : gss_step_hits_map(s->e, &full, &n);
+    const uint64_t *raw = nullptr;
+    const int rw = s->cfg.records == GSS_RECORDS_IDS ? 1 : 3;
+    if (!rc && n > 0) {
+        if (rw == 1) {
+            const gss_hit_id *p = nullptr;
+            rc = gss_step_ids_map(s->e, &p, &n);
+            raw = (const uint64_t *)p;
+        } else {
+            const gss_hit *p = nullptr;
+            rc = gss_step_hits_map(s->e, &p, &n);
+            raw = (const uint64_t *)p;
+        }
+    }
     const Clock::time_point now = Clock::now();
+    // the requests it admitted are the oldest pending ones (only this thread
+    // removes them, so they stay put while the lock is released)
+    const int cnt = rc ? 0 : (int)rep.count;
+    std::vector<int> ks(cnt);
+    bool order_ok;
+    {
+        lk.lock();
+        order_ok = rep.first == s->completed && (size_t)cnt <= s->pending.size();
+        for (int i = 0; order_ok && i < cnt; i++) ks[i] = s->pending[i].k;
+        // keep the device busy: the next step goes out before this one's
+        // records are grouped (they stay in the slot's pinned memory until
+        // that slot is waited for again, two steps on)
+        if (!rc && order_ok && (int)s->inflight.size() <= s->cfg.max_inflight &&
+            can_launch_after(s, cnt))
+            launch(s, lk);
+        lk.unlock();
+    }
+    int owner[GSS_MAX_WORDS];
+    uint32_t w0[kServeMaxRequests];
+    std::fill(owner, owner + GSS_MAX_WORDS, -1);
+    uint32_t w = 0;
+    double assigs = 0;
+    for (int i = 0; order_ok && i < cnt; i++) {
+        const uint32_t nw = (uint32_t)(ks[i] + 63) / 64;
+        w0[i] = w;
+        for (uint32_t j = 0; j < nw && w + j < (uint32_t)GSS_MAX_WORDS; j++) owner[w + j] = i;
+        w += nw;
+        assigs += ks[i];
+    }
+    auto step = std::make_shared<StepRecords>();
+    step->rw = rw;
+    std::vector<size_t> first(cnt + 1, 0);
+    if (order_ok && n > 0) {
+        std::vector<size_t> count(cnt, 0);
+        for (int64_t r = 0; r < n; r++) {
+            const uint32_t word = (uint32_t)(raw[r * rw] >> 48);
+            const int i = word < (uint32_t)GSS_MAX_WORDS ? owner[word] : -1;
+            if (i >= 0) count[i]++;  // other words are never valid
+        }
+        for (int i = 0; i < cnt; i++) first[i + 1] = first[i] + count[i];
+        step->words.resize(first[cnt] * rw);
+        std::vector<size_t> at(first.begin(), first.end() - 1);
+        for (int64_t r = 0; r < n; r++) {
+            const uint64_t id = raw[r * rw];
+            const uint32_t word = (uint32_t)(id >> 48);
+            const int i = word < (uint32_t)GSS_MAX_WORDS ? owner[word] : -1;
+            if (i < 0) continue;
+            uint64_t *dst = step->words.data() + at[i]++ * rw;
+            dst[0] = (id & 0x0000FFFFFFFFFFFFull) | ((uint64_t)(word - w0[i]) << 48);
+            for (int q = 1; q < rw; q++) dst[q] = raw[r * rw + q];
+        }
+    } else {
+        std::fill(first.begin(), first.end(), 0);
+    }
     lk.lock();
     const StepInfo si = s->inflight.front();
     s->inflight.pop_front();
@@ -192,27 +282,14 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
         set_failed(s, rc, "serving step wait");
         return;
     }
-    if (rep.first != s->completed || rep.count > s->pending.size()) {
+    if (!order_ok) {
         s->err_code = GSS_E_STATE;
         s->err = "serving step admitted requests out of order";
         s->cv_done.notify_all();
         return;
     }
-    const int cnt = (int)rep.count;
-    int owner[GSS_MAX_WORDS];
-    uint32_t w0[kServeMaxRequests];
-    std::fill(owner, owner + GSS_MAX_WORDS, -1);
-    std::vector<Done *> out(cnt, nullptr);
-    uint32_t w = 0;
-    double assigs = 0;
     for (int i = 0; i < cnt; i++) {
         const Pending &p = s->pending[i];
-        const uint32_t nw = (uint32_t)(p.k + 63) / 64;
-        w0[i] = w;
-        for (uint32_t j = 0; j < nw && w + j < (uint32_t)GSS_MAX_WORDS; j++) owner[w + j] = i;
-        w += nw;
-        s->pending_words -= nw;
-        assigs += p.k;
         Done &dn = s->done[(int64_t)p.seq];
         dn.r.request = (int64_t)p.seq;
         dn.r.step = si.index;
@@ -222,32 +299,12 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
         dn.r.k = p.k;
         dn.r.first_word = (int32_t)w0[i];
         dn.r.latency_ms = std::chrono::duration<double, std::milli>(now - p.t_submit).count();
-        out[i] = &dn;
-    }
-    for (int64_t r = 0; r < n; r++) {
-        gss_serve_hit h{};
-        uint32_t len, row, word;
-        if (idr) {
-            len = idr[r].len;
-            row = idr[r].row;
-            word = idr[r].word;
-        } else {
-            len = full[r].len;
-            row = full[r].row;
-            word = full[r].word;
-            h.false_mask = full[r].false_mask;
-            h.unit_mask = full[r].unit_mask;
-        }
-        const int i = word < (uint32_t)GSS_MAX_WORDS ? owner[word] : -1;
-        if (i < 0) continue;  // cannot happen: only admitted words are valid
-        h.clause = (len < s->ids.size() && row < s->ids[len].size()) ? s->ids[len][row] : -1;
-        h.word = word - w0[i];
-        out[i]->recs.push_back(h);
-    }
-    for (int i = 0; i < cnt; i++) {
-        out[i]->r.n_records = (int64_t)out[i]->recs.size();
-        out[i]->r.records = out[i]->recs.data();
-        s->ready.push_back(out[i]->r.request);
+        dn.r.n_records = (int64_t)(first[i + 1] - first[i]);
+        dn.r.records = nullptr;
+        dn.step = step;
+        dn.first = first[i];
+        s->pending_words -= (p.k + 63) / 64;
+        s->ready.push_back(dn.r.request);
     }
     if (cnt) {
         const Pending &last = s->pending[cnt - 1];
@@ -494,6 +551,37 @@ int gss_server_result(gss_server *s, int64_t id, gss_serve_result *out) {
     auto it = s->done.find(id);
     if (it == s->done.end()) return fail(GSS_E_STATE, "request not completed (or released)");
     *out = it->second.r;
+    return GSS_OK;
+}
+
+int gss_server_records(gss_server *s, int64_t id, const gss_serve_hit **out, int64_t *n) {
+    int rc = check(s);
+    if (rc) return rc;
+    if (!out || !n) return fail(GSS_E_INVAL, "null out");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->done.find(id);
+    if (it == s->done.end()) return fail(GSS_E_STATE, "request not completed (or released)");
+    Done &d = it->second;
+    if (!d.translated) {
+        const int rw = d.step->rw;
+        d.recs.resize((size_t)d.r.n_records);
+        for (int64_t r = 0; r < d.r.n_records; r++) {
+            const uint64_t *src = d.step->words.data() + (d.first + (size_t)r) * rw;
+            const uint32_t row = (uint32_t)src[0];
+            const uint32_t len = (uint32_t)(src[0] >> 32) & 0xFFFFu;
+            gss_serve_hit &h = d.recs[r];
+            h.clause = (len < s->ids.size() && row < s->ids[len].size()) ? s->ids[len][row] : -1;
+            h.word = (uint32_t)(src[0] >> 48);
+            h.pad = 0;
+            h.false_mask = rw == 3 ? src[1] : 0;
+            h.unit_mask = rw == 3 ? src[2] : 0;
+        }
+        d.translated = true;
+        d.step.reset();  // the step's buffer goes once every request has its records
+        d.r.records = d.recs.data();
+    }
+    *out = d.recs.data();
+    *n = d.r.n_records;
     return GSS_OK;
 }
 

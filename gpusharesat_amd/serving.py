@@ -295,9 +295,10 @@ class GpuServer:
         meta = {f: getattr(r, f) for f, _ in _Result._fields_ if f not in ("records", "pad")}
         recs = None
         if records:
-            if r.n_records:
-                buf = (ctypes.c_char * (r.n_records * SERVE_HIT_DTYPE.itemsize)).from_address(
-                    r.records)
+            p, n = ctypes.c_void_p(), ctypes.c_int64()
+            _lib.call("gss_server_records", self._h, rid, ctypes.byref(p), ctypes.byref(n))
+            if n.value:
+                buf = (ctypes.c_char * (n.value * SERVE_HIT_DTYPE.itemsize)).from_address(p.value)
                 recs = np.frombuffer(buf, SERVE_HIT_DTYPE).copy()
             else:
                 recs = np.zeros(0, SERVE_HIT_DTYPE)

@@ -72,9 +72,9 @@ typedef struct gss_serve_result {
     int32_t k;
     int32_t first_word;       /* table word the request's slots started at in its step */
     int32_t pad;
-    double latency_ms;        /* submit -> records handed to this result */
+    double latency_ms;        /* submit -> its records grouped for it on the host */
     int64_t n_records;
-    const gss_serve_hit *records; /* valid until gss_server_release */
+    const gss_serve_hit *records; /* NULL until gss_server_records has been called */
 } gss_serve_result;
 
 typedef struct gss_server_stats {
@@ -116,6 +116,9 @@ int gss_server_submit(gss_server *s, int solver, int k, int64_t n_patches,
  * for one if none is ready. */
 int gss_server_poll(gss_server *s, int64_t max, int64_t *ids, int64_t *n, int64_t timeout_us);
 int gss_server_result(gss_server *s, int64_t id, gss_serve_result *out);
+/* The request's records with clause ids (looked up on this call, once), valid
+ * until gss_server_release. */
+int gss_server_records(gss_server *s, int64_t id, const gss_serve_hit **out, int64_t *n);
 int gss_server_release(gss_server *s, int64_t id);
 int gss_server_stats_get(gss_server *s, gss_server_stats *out);
 
