@@ -53,6 +53,7 @@ SIGNATURES: dict[str, list] = {
     "gss_assig_roll_forward": [c_p, c_i64, c_p, c_p],
     "gss_assig_roll_forward_applied": [c_p, c_p],
     "gss_engine_mem_info": [c_p, c_p, c_p],
+    "gss_engine_set_page_locked_limit": [c_p, c_i64],
     "gss_sweep_async": [c_p, c_p],
     "gss_sweep_wait": [c_p, c_p],
     "gss_sweep": [c_p, c_p, c_p],

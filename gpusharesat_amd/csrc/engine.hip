@@ -110,25 +110,47 @@ struct Bucket {
     int64_t cap = 0;  // rows
 };
 
+// Host memory the device copies from or to (or, for staging that kernels
+// read directly, reads itself).  Page-locked; a buffer given a page-locked
+// limit (one that is only a copy endpoint) falls back to pageable memory when
+// an allocation reaches the limit or page-locking fails, as the reference
+// does with maxPageLockedMemory (gpuShareLib/CorrespArr.cu:212-235).
 template <typename T>
 struct Pinned {
     T *p = nullptr;
     size_t cap = 0;
+    bool paged = false;
+    const int64_t *page_limit = nullptr;  // bytes (< 0: none); null = page-locked only
     int reserve(size_t n) {
         if (n <= cap) return GSS_OK;
         size_t nc = std::max(n, cap * 2);
-        if (p) (void)hipHostFree(p);
+        release();
+        const size_t bytes = nc * sizeof(T);
+        const bool lock = !page_limit || *page_limit < 0 || (int64_t)bytes < *page_limit;
+        if (lock && hipHostMalloc((void **)&p, bytes, hipHostMallocDefault) == hipSuccess) {
+            paged = false;
+            cap = nc;
+            return GSS_OK;
+        }
         p = nullptr;
-        cap = 0;
-        if (hipHostMalloc((void **)&p, nc * sizeof(T), hipHostMallocDefault) != hipSuccess)
-            return set_err(GSS_E_NOMEM, "pinned host allocation of %zu bytes failed",
-                           nc * sizeof(T));
+        if (!page_limit)
+            return set_err(GSS_E_NOMEM, "pinned host allocation of %zu bytes failed", bytes);
+        p = (T *)std::malloc(bytes);
+        if (!p) return set_err(GSS_E_NOMEM, "host allocation of %zu bytes failed", bytes);
+        paged = true;
         cap = nc;
         return GSS_OK;
     }
-    ~Pinned() {
-        if (p) (void)hipHostFree(p);
+    void release() {
+        if (p) {
+            if (paged) std::free(p);
+            else (void)hipHostFree(p);
+        }
+        p = nullptr;
+        cap = 0;
+        paged = false;
     }
+    ~Pinned() { release(); }
 };
 
 template <typename T>
@@ -291,6 +313,9 @@ struct gss_engine {
     std::vector<uint64_t> valid_up;
     Pinned<uint32_t> h_count;  // [0] = chunks claimed, [1], [2] = records (low, high word)
     uint64_t *gather_scratch = nullptr;  // this device's address of `scratch`
+    // host buffers that are only copy endpoints of at least this many bytes
+    // are pageable (< 0: no limit); see Pinned
+    int64_t page_limit = -1;
     DevBuf<gss_hit> d_hits;
     DevBuf<uint32_t> d_fills;
     uint32_t *d_alloc = nullptr;
@@ -512,6 +537,10 @@ int gss_engine_create(int device, gss_engine **out) {
         return fail(set_err(GSS_E_NOMEM, "device allocation failed"));
     if ((rc = e->h_valid.reserve(GSS_MAX_WORDS)) || (rc = e->h_lastpos.reserve(GSS_MAX_WORDS)))
         return fail(rc);
+    // copy endpoints only: staging of the plain path, the steps' records
+    e->h_stage.page_limit = &e->page_limit;
+    e->h_ups.page_limit = &e->page_limit;
+    for (StepSlot &sl : e->slots) sl.h_hits.page_limit = &e->page_limit;
     if ((rc = gss_assig_reset(e, 0, 1))) return fail(rc);
     *out = e;
     return GSS_OK;
@@ -585,6 +614,13 @@ int gss_engine_sync(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return GSS_OK;
+}
+
+int gss_engine_set_page_locked_limit(gss_engine *e, int64_t bytes) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    e->page_limit = bytes < 0 ? -1 : bytes;
     return GSS_OK;
 }
 

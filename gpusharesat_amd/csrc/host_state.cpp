@@ -244,6 +244,57 @@ void ClauseMirror::drop_bucket(int len) {
     lits_[len].clear();
 }
 
+void ClauseMirror::compact_by_tags(const uint8_t *evicted, std::vector<uint32_t> &freed) {
+    for (size_t len = 1; len < meta_.size(); len++) {
+        std::vector<ClauseMeta> &meta = meta_[len];
+        std::vector<uint32_t> &lits = lits_[len];
+        size_t to = 0;
+        for (size_t r = 0; r < meta.size(); r++) {
+            const ClauseMeta &m = meta[r];
+            if (m.tag != kNoTag && evicted[m.tag]) {
+                freed.push_back(m.tag);
+                continue;
+            }
+            if (to != r) {
+                meta[to] = m;
+                std::memmove(&lits[to * len], &lits[r * len], len * sizeof(uint32_t));
+            }
+            to++;
+        }
+        const long removed = (long)(meta.size() - to);
+        count_ -= removed;
+        len_sum_ -= removed * (long)len;
+        meta.resize(to);
+        lits.resize(to * len);
+    }
+}
+
+bool ClauseMirror::tag_tail(int len, int64_t count, std::vector<uint32_t> &free) {
+    if ((int64_t)free.size() < count || len >= (int)meta_.size() ||
+        count > (int64_t)meta_[len].size())
+        return false;
+    std::vector<ClauseMeta> &meta = meta_[len];
+    for (size_t r = meta.size() - (size_t)count; r < meta.size(); r++) {
+        if (meta[r].tag == kNoTag) {
+            meta[r].tag = free.back();
+            free.pop_back();
+        }
+    }
+    return true;
+}
+
+void ClauseMirror::drop_tail(int len, int64_t count, std::vector<uint32_t> &freed) {
+    if (len >= (int)meta_.size()) return;
+    std::vector<ClauseMeta> &meta = meta_[len];
+    count = std::min<int64_t>(count, (int64_t)meta.size());
+    for (size_t r = meta.size() - (size_t)count; r < meta.size(); r++)
+        if (meta[r].tag != kNoTag) freed.push_back(meta[r].tag);
+    meta.resize(meta.size() - (size_t)count);
+    lits_[len].resize(meta.size() * len);
+    count_ -= (long)count;
+    len_sum_ -= (long)count * len;
+}
+
 /* ---- ReportQueue (gpuShareLib/Reported.cu:97-204) ------------------------- */
 
 ClauseBatch *ReportQueue::get_new() {

@@ -745,6 +745,10 @@ int gss_res_add(gss_res *r, int64_t n, const uint32_t *lits, const int32_t *lens
         if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
             return set_err(GSS_E_INVAL, "a tag appears twice in the batch");
     }
+    // every allocation first: a batch the device cannot take changes nothing
+    if ((rc = r->tags_in.reserve(n)) || (rc = r->lens_in.reserve(n)) ||
+        (rc = r->act_in.reserve(n)))
+        return rc;
     if ((rc = gss_db_append_tagged(r->e, n, lits, lens, tags, nullptr))) return rc;
     // initial activities: the increment after each clause's own decay step
     std::vector<float> act0((size_t)n);
@@ -756,9 +760,6 @@ int gss_res_add(gss_res *r, int64_t n, const uint32_t *lits, const int32_t *lens
         if (inc > kRescale) rescale = true;
     }
     hipStream_t st = engine_stream(r->e);
-    if ((rc = r->tags_in.reserve(n)) || (rc = r->lens_in.reserve(n)) ||
-        (rc = r->act_in.reserve(n)))
-        return rc;
     RES_HIP(hipMemcpyAsync(r->tags_in.p, tags, n * 4, hipMemcpyHostToDevice, st));
     RES_HIP(hipMemcpyAsync(r->lens_in.p, lens, n * 4, hipMemcpyHostToDevice, st));
     RES_HIP(hipMemcpyAsync(r->act_in.p, act0.data(), n * 4, hipMemcpyHostToDevice, st));

@@ -92,6 +92,29 @@ EngineClauseSharer::EngineClauseSharer(GpuClauseSharerOptions opts,
         gss_engine_mem_info(engine_, &f, &t);
         opts_.maxPageLockedMemory = (int)std::min<size_t>(t / 3, (size_t)INT32_MAX);
     }
+    // host copies of at least this size are pageable (CorrespArr.cu:212-235)
+    check_rc(gss_engine_set_page_locked_limit(engine_, opts_.maxPageLockedMemory),
+             "gss_engine_set_page_locked_limit");
+    // reduction policy: the reference's activity median on the host mirror
+    // (default), the same rule on the device, or the learned score
+    if (const char *p = std::getenv("GPUSHARESAT_REDUCE_POLICY")) {
+        const std::string v = p;
+        if (v == "device") policy_ = Policy::Device;
+        else if (v == "learned") policy_ = Policy::Learned;
+        else if (!v.empty() && v != "host")
+            throw std::invalid_argument("GPUSHARESAT_REDUCE_POLICY must be host, device or learned");
+    }
+    if (policy_ != Policy::Host) {
+        max_tags_ = 1 << 22;
+        if (const char *t = std::getenv("GPUSHARESAT_RESIDENCY_MAX_TAGS"))
+            max_tags_ = std::max(1ll, std::atoll(t));
+        check_rc(gss_db_enable_tags(engine_), "gss_db_enable_tags");
+        check_rc(gss_res_create(engine_, max_tags_, (float)opts_.clauseActivityDecay, 1, &res_),
+                 "gss_res_create");
+        free_tags_.resize((size_t)max_tags_);
+        for (int64_t i = 0; i < max_tags_; i++) free_tags_[i] = (uint32_t)(max_tags_ - 1 - i);
+        evicted_.assign((size_t)max_tags_, 0);
+    }
     log(1, "c GPU clause sharing on MI355X engine, " + std::to_string(limits_.slots) +
                " assignment slots per solver");
     setCpuSolverCount(1);
@@ -100,6 +123,7 @@ EngineClauseSharer::EngineClauseSharer(GpuClauseSharerOptions opts,
 EngineClauseSharer::~EngineClauseSharer() {
     if (engine_) {
         gss_engine_sync(engine_);
+        if (res_) gss_res_destroy(res_);  // the scorer goes before its engine
         gss_engine_destroy(engine_);
     }
 }
@@ -162,20 +186,25 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();
     if (!b.lens.empty()) {
         clause_vars_ = std::max<int64_t>(clause_vars_, (b.max_lit >> 1) + 1);
-        // room for the batch first, before any window is taken: out of
-        // device memory, the DB is halved and re-uploaded
-        // (GpuRunner.cu:243-260), but only after the run in flight has been
-        // delivered: its hits name clauses by row, and the reduction moves
-        // rows (GpuRunner.cu:251-253).  The device then holds none of this
-        // batch, which the mirror already has.
-        const int rc = gss_db_reserve(engine_, (int64_t)b.lens.size(), b.lens.data());
-        if (rc == GSS_E_NOMEM) {
-            ran_out_of_memory_ = true;
-            reduce_pending_ = true;
-            dirty_.insert(b.lens.begin(), b.lens.end());
-            return false;
+        if (policy_ != Policy::Host) {
+            // device policies: appended now, with tags (gss_res_add)
+            if (!add_on_device(b)) return false;
+        } else {
+            // room for the batch first, before any window is taken: out of
+            // device memory, the DB is halved and re-uploaded
+            // (GpuRunner.cu:243-260), but only after the run in flight has
+            // been delivered: its hits name clauses by row, and the
+            // reduction moves rows (GpuRunner.cu:251-253).  The device then
+            // holds none of this batch, which the mirror already has.
+            const int rc = gss_db_reserve(engine_, (int64_t)b.lens.size(), b.lens.data());
+            if (rc == GSS_E_NOMEM) {
+                ran_out_of_memory_ = true;
+                reduce_pending_ = true;
+                dirty_.insert(b.lens.begin(), b.lens.end());
+                return false;
+            }
+            check_rc(rc, "gss_db_reserve");
         }
-        check_rc(rc, "gss_db_reserve");
     }
     if (clauses_.clause_count() == 0) return false;  // GpuRunner.cu:281-286
     ensure_table();
@@ -198,6 +227,19 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     }
     // table vars may have grown through the windows while we held no lock
     ensure_table();
+    if (policy_ != Policy::Host) {
+        // a plain run: apply, sweep (if some slot is valid); the roll-forward
+        // and the records' accounting follow the wait (collect)
+        run.plain = true;
+        check_rc(gss_assig_apply(engine_, (int64_t)run.updates.size(), run.updates.data()),
+                 "gss_assig_apply");
+        run.swept = false;
+        for (uint64_t v : valid) run.swept = run.swept || v != 0;
+        if (run.swept) check_rc(gss_sweep_async(engine_, valid.data()), "gss_sweep_async");
+        run.clause_count = clauses_.clause_count();
+        run.active = true;
+        return true;
+    }
     std::vector<uint32_t> rows(b.lens.size());
     gss_step_desc d{};
     d.n_clauses = (int64_t)b.lens.size();
@@ -219,6 +261,32 @@ bool EngineClauseSharer::start_run(InFlight &run) {
 }
 
 void EngineClauseSharer::collect(InFlight &run) {
+    if (run.plain) {
+        int64_t n = 0;
+        if (run.swept) check_rc(gss_sweep_wait(engine_, &n), "gss_sweep_wait");
+        check_rc(gss_assig_roll_forward_applied(engine_, run.last_pos.data()),
+                 "gss_assig_roll_forward_applied");
+        id_buf_.clear();
+        if (run.swept) {
+            // one activity bump per record, hit history (GpuRunner.cu:375-378)
+            int64_t pairs = 0;
+            check_rc(gss_res_account(res_, round_, &pairs), "gss_res_account");
+            hit_buf_.resize((size_t)n);
+            if (n) check_rc(gss_hits_copy(engine_, hit_buf_.data(), n, &n), "gss_hits_copy");
+            id_buf_.resize((size_t)n);
+            for (int64_t i = 0; i < n; i++)
+                id_buf_[i] = gss_hit_id{hit_buf_[i].row, hit_buf_[i].len, hit_buf_[i].word};
+            if (opts_.quickProf) {
+                float ms = 0;
+                gss_sweep_kernel_ms(engine_, &ms);
+                global_stats_[timeSpentTestingClauses] += (unsigned long)(ms * 1000.0f);
+            }
+        }
+        run.records = n;
+        if (policy_ == Policy::Learned) train_policy();
+        round_++;
+        return;
+    }
     int64_t n = 0;
     float ms = 0.f;
     check_rc(gss_step_wait(engine_, &n, &ms), "gss_step_wait");
@@ -226,10 +294,80 @@ void EngineClauseSharer::collect(InFlight &run) {
     if (opts_.quickProf) global_stats_[timeSpentTestingClauses] += (unsigned long)(ms * 1000.0f);
 }
 
+// Learned policy: every 16 runs a snapshot of up to 8192 resident clauses'
+// features is taken, and 5 runs later the MLP trains on it (label: reported
+// since), as the residency workload does per round.
+void EngineClauseSharer::train_policy() {
+    if (!failed_lens_.empty()) return;  // the mirror holds clauses the device has not
+    const int32_t phase = round_ % 16;
+    if (phase == 11 && clauses_.clause_count() > 0) {
+        std::vector<uint32_t> tags;
+        const long total = clauses_.clause_count();
+        const long want = std::min<long>(8192, total);
+        uint64_t x = 0x9E3779B97F4A7C15ull ^ (uint64_t)round_;
+        std::vector<int64_t> start(clauses_.max_len() + 2, 0);
+        for (int len = 1; len <= clauses_.max_len(); len++)
+            start[len + 1] = start[len] + clauses_.rows(len);
+        for (long i = 0; i < want; i++) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            const int64_t g = (int64_t)(x % (uint64_t)total);
+            const int len = (int)(std::upper_bound(start.begin(), start.end(), g) - start.begin()) - 1;
+            const uint32_t t = clauses_.tag(len, (uint32_t)(g - start[len]));
+            if (t != kNoTag) tags.push_back(t);
+        }
+        std::sort(tags.begin(), tags.end());
+        tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+        if (!tags.empty())
+            check_rc(gss_res_snapshot(res_, round_, (int64_t)tags.size(), tags.data()),
+                     "gss_res_snapshot");
+    } else if (phase == 0 && round_ >= 16) {
+        float loss = 0.f;
+        int64_t samples = 0;
+        check_rc(gss_res_train(res_, round_, 5, 2, 2048, 0.02f, &loss, &samples), "gss_res_train");
+    }
+}
+
+// Appends a committed batch to the device under fresh tags (device policies).
+// When the device cannot take it (out of memory, or of tags) the batch stays
+// the tail of its buckets in the mirror, is remembered, and the DB is reduced
+// once the run in flight is delivered (reduce_db_on_device adds it then).
+bool EngineClauseSharer::add_on_device(const ClauseMirror::Batch &b) {
+    std::vector<int64_t> per(clauses_.max_len() + 1, 0);
+    for (int32_t len : b.lens) per[len]++;
+    bool tagged = (int64_t)free_tags_.size() >= (int64_t)b.lens.size();
+    for (int len = 1; tagged && len <= clauses_.max_len(); len++)
+        if (per[len]) clauses_.tag_tail(len, per[len], free_tags_);
+    int rc = GSS_E_NOMEM;
+    if (tagged) {
+        std::vector<int64_t> seen(clauses_.max_len() + 1, 0);
+        std::vector<uint32_t> tags(b.lens.size());
+        for (size_t i = 0; i < b.lens.size(); i++) {
+            const int len = b.lens[i];
+            tags[i] = clauses_.tag(len, (uint32_t)(clauses_.rows(len) - per[len] + seen[len]++));
+        }
+        rc = gss_res_add(res_, (int64_t)b.lens.size(), b.lits.data(), b.lens.data(), tags.data(),
+                         round_);
+    }
+    if (rc == GSS_OK) return true;
+    if (rc != GSS_E_NOMEM) check_rc(rc, "gss_res_add");
+    ran_out_of_memory_ = true;
+    reduce_pending_ = true;
+    failed_lits_.insert(failed_lits_.end(), b.lits.begin(), b.lits.end());
+    failed_lens_.insert(failed_lens_.end(), b.lens.begin(), b.lens.end());
+    return false;
+}
+
 void EngineClauseSharer::deliver(InFlight &run) {
     const gss_hit_id *ids = nullptr;
     int64_t n = 0;
-    if (run.records) check_rc(gss_step_ids_map(engine_, &ids, &n), "gss_step_ids_map");
+    if (run.plain) {
+        ids = id_buf_.data();
+        n = (int64_t)id_buf_.size();
+    } else if (run.records) {
+        check_rc(gss_step_ids_map(engine_, &ids, &n), "gss_step_ids_map");
+    }
     global_stats_[gpuRuns]++;
     long assigs = 0, solvers_with = 0;
     for (const auto &t : run.taken) {
@@ -297,6 +435,14 @@ void EngineClauseSharer::reduce_db_now() {
     added_at_last_reduce_ = clauses_.added();
     global_stats_[gpuReduceDbs]++;
     log(2, "c Reducing gpu clause db, clause count is " + std::to_string(clauses_.clause_count()));
+    if (policy_ != Policy::Host) {
+        reduce_db_on_device();
+        global_stats_[gpuClauses] = (unsigned long)clauses_.clause_count();
+        global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();
+        log(2, "c Done reducing gpu clause db, clause count is " +
+                   std::to_string(clauses_.clause_count()));
+        return;
+    }
     // Reduce, then upload every bucket whose device rows differ from the
     // mirror's: the reduced ones first (they shrink and give memory back),
     // then those a failed append left behind (the mirror already holds that
@@ -336,6 +482,41 @@ void EngineClauseSharer::reduce_db_now() {
     global_stats_[gpuClauseLengthSum] = (unsigned long)clauses_.length_sum();
     log(2, "c Done reducing gpu clause db, clause count is " +
                std::to_string(clauses_.clause_count()));
+}
+
+// Device policies: the device keeps about half of the clauses of length >= 3
+// (activity median, or the learned score's), compacting in place; the mirror
+// drops the same rows.  A batch the device could not take is then added again
+// (its tags assigned now if they were missing); if it still does not fit the
+// reduction repeats, and when nothing more goes the batch is dropped from the
+// mirror, so device and mirror always agree.
+void EngineClauseSharer::reduce_db_on_device() {
+    const int pol = policy_ == Policy::Learned ? GSS_RES_POLICY_LEARNED : GSS_RES_POLICY_ACTIVITY;
+    for (;;) {
+        int64_t kept = 0;
+        std::fill(evicted_.begin(), evicted_.end(), 0);
+        check_rc(gss_res_reduce(res_, pol, round_, evicted_.data(), &kept), "gss_res_reduce");
+        std::vector<uint32_t> freed;
+        clauses_.compact_by_tags(evicted_.data(), freed);
+        free_tags_.insert(free_tags_.end(), freed.begin(), freed.end());
+        if (failed_lens_.empty()) return;
+        ClauseMirror::Batch b;
+        b.lits.swap(failed_lits_);
+        b.lens.swap(failed_lens_);
+        if (add_on_device(b)) return;
+        if (!freed.empty()) continue;  // reduced something: try again
+        // nothing left to reduce: the batch leaves the mirror
+        std::vector<int64_t> per(clauses_.max_len() + 1, 0);
+        for (int32_t len : failed_lens_) per[len]++;
+        std::vector<uint32_t> dropped;
+        for (int len = 1; len <= clauses_.max_len(); len++)
+            if (per[len]) clauses_.drop_tail(len, per[len], dropped);
+        free_tags_.insert(free_tags_.end(), dropped.begin(), dropped.end());
+        failed_lits_.clear();
+        failed_lens_.clear();
+        log(1, "c GPU memory short with nothing left to reduce: a batch of clauses was dropped");
+        return;
+    }
 }
 
 void EngineClauseSharer::reduceDb() {

@@ -27,6 +27,7 @@
 
 #include "gpuShareLib/GpuClauseSharer.h"
 #include "gss.h"
+#include "gss_residency.h"
 
 namespace GpuShare {
 
@@ -78,10 +79,13 @@ private:
     std::vector<gss_var_update> updates_;
 };
 
+constexpr uint32_t kNoTag = 0xFFFFFFFFu;
+
 struct ClauseMeta {
     long id;
     int lbd;
     float activity;
+    uint32_t tag = kNoTag;  // slot in the device residency state (device policies)
 };
 
 class ClauseMirror {
@@ -104,6 +108,15 @@ public:
     std::vector<int> reduce(int max_lbd);
     // removes every clause of bucket len
     void drop_bucket(int len);
+    // Device policies: keeps the rows whose tag is not flagged in evicted
+    // (rows without a tag are kept), in order, as the device compaction
+    // does; the tags of the removed rows go to freed.
+    void compact_by_tags(const uint8_t *evicted, std::vector<uint32_t> &freed);
+    // Tags for the last count rows of bucket len (in order) from free.
+    bool tag_tail(int len, int64_t count, std::vector<uint32_t> &free);
+    // Removes the last count rows of bucket len; their tags go to freed.
+    void drop_tail(int len, int64_t count, std::vector<uint32_t> &freed);
+    uint32_t tag(int len, uint32_t row) const { return meta_[len][row].tag; }
     float approx_nth_activity(int min_lbd, int max_lbd, long n) const;
 
     long clause_count() const { return count_; }
@@ -232,10 +245,15 @@ private:
         std::vector<int8_t> last_pos;
         long clause_count = 0;
         int64_t records = 0;  // hit records of the waited step
+        bool plain = false;   // a plain sweep (device policies) instead of a step
+        bool swept = false;   // plain: a sweep was launched (some slot was valid)
     };
 
     void run_once(bool can_start);
     void reduce_db_now();
+    void reduce_db_on_device();
+    bool add_on_device(const ClauseMirror::Batch &b);
+    void train_policy();
     bool start_run(InFlight &run);
     void collect(InFlight &run);
     void deliver(InFlight &run);
@@ -267,6 +285,24 @@ private:
     bool ran_out_of_memory_ = false;
     bool reduce_pending_ = false;  // out of memory: reduce once the run in flight is delivered
     std::set<int> dirty_;          // buckets whose device rows differ from the mirror's
+
+    // Reduction policy (GPUSHARESAT_REDUCE_POLICY): the reference's activity
+    // median on the host mirror (default), or on the device over a tagged DB,
+    // or the learned score (gss_residency.h).  The device policies run each
+    // gpuRun as a plain sweep whose records feed the per-clause state.
+    enum class Policy { Host, Device, Learned };
+    Policy policy_ = Policy::Host;
+    gss_res *res_ = nullptr;
+    int64_t max_tags_ = 0;
+    std::vector<uint32_t> free_tags_;
+    std::vector<uint8_t> evicted_;
+    int32_t round_ = 0;
+    std::vector<gss_hit> hit_buf_;
+    std::vector<gss_hit_id> id_buf_;
+    // a batch the device could not take (no memory, or no free tags): it is
+    // the tail of its buckets in the mirror, its clause order is kept here
+    std::vector<uint32_t> failed_lits_;
+    std::vector<int32_t> failed_lens_;
 };
 
 }  // namespace GpuShare

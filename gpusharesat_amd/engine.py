@@ -96,6 +96,10 @@ class Engine:
     def sync(self) -> None:
         _lib.call("gss_engine_sync", self._h)
 
+    def set_page_locked_limit(self, nbytes: int) -> None:
+        """Copy-endpoint host buffers of at least nbytes become pageable."""
+        _lib.call("gss_engine_set_page_locked_limit", self._h, nbytes)
+
     # -- clause DB ------------------------------------------------------------
     def append(self, lits: np.ndarray, lens: np.ndarray) -> np.ndarray:
         """Appends clauses (CSR-free: lens[i] literals each, concatenated).

@@ -121,6 +121,13 @@ int gss_engine_destroy(gss_engine *e);
 int gss_engine_cu_count(gss_engine *e, int *out);
 /* Blocks until all work queued on the engine's stream is done. */
 int gss_engine_sync(gss_engine *e);
+/* Host buffers that are only copy endpoints (the plain path's staging, the
+ * steps' records on the host) are allocated pageable instead of page-locked
+ * when one allocation reaches `bytes`, or when page-locking fails; bytes < 0
+ * (the default): no limit.  Replaces maxPageLockedMemory
+ * (GpuClauseSharer.h:47, gpuShareLib/CorrespArr.cu:212-235).  Staging that the
+ * step's kernels read in place stays page-locked. */
+int gss_engine_set_page_locked_limit(gss_engine *e, int64_t bytes);
 /* Free and total device memory (replaces cudaMemGetInfo in
  * GpuClauseSharerImpl::getGpuMemInfo, GpuClauseSharerImpl.cu:205-207). */
 int gss_engine_mem_info(gss_engine *e, size_t *free_bytes, size_t *total_bytes);

@@ -495,20 +495,22 @@ def test_out_of_device_memory_halves_the_db(tmp_path, monkeypatch):
         assert got == exp
 
 
-def _db_matches_cnf(s, tmp_path, V, rng):
+def _db_matches_cnf(s, tmp_path, V, rng, solver=0):
     """The device holds exactly the clauses of the written CNF: one
-    assignment's reports equal the CNF clauses it makes falsified or unit."""
+    assignment's reports equal the CNF clauses it makes falsified or unit
+    (for a solver with no reports from earlier assignments still held back,
+    Reported.cu:97-158)."""
     from collections import Counter
     p = tmp_path / "db.cnf"
     s.writeClausesInCnf(str(p))
     lines = p.read_text().splitlines()
     db = [[mkLit(abs(int(x)) - 1, int(x) < 0) for x in line.split()[:-1]] for line in lines[1:]]
     assig = {int(v): bool(rng.integers(2)) for v in rng.choice(V, size=V // 2, replace=False)}
-    s.unsetSolverValues(0, [mkLit(v) for v in range(V)])
-    assert s.trySetSolverValues(0, [mkLit(v, not val) for v, val in assig.items()])
-    assert s.trySendAssignment(0) >= 0
+    s.unsetSolverValues(solver, [mkLit(v) for v in range(V)])
+    assert s.trySetSolverValues(solver, [mkLit(v, not val) for v, val in assig.items()])
+    assert s.trySendAssignment(solver) >= 0
     s.execute()
-    got = Counter(tuple(sorted(l)) for l, _ in s.popAll(0))
+    got = Counter(tuple(sorted(l)) for l, _ in s.popAll(solver))
     exp = Counter(tuple(sorted(c)) for c in db if _useful(c, assig))
     assert got == exp
     return len(db)
@@ -558,6 +560,21 @@ def test_out_of_memory_with_nothing_left_to_reduce(tmp_path, monkeypatch):
         s.gpuRun()
         assert s.getGlobalStat("gpuClauses") == 150
         assert _db_matches_cnf(s, tmp_path, V, rng) == 150
+
+
+def test_pageable_host_memory_option():
+    """maxPageLockedMemory = 1 byte (GpuClauseSharer.h:47): every host copy
+    endpoint is pageable (CorrespArr.cu:212-235); reports are unchanged."""
+    with sharer(4, 2, max_page_locked_memory=1) as s:
+        a = s.addClause(-1, [F(0), mkLit(1)])
+        b = s.addClause(-1, [mkLit(0), F(1), mkLit(2)])
+        assert s.trySetSolverValues(0, [T(0)])
+        assert s.trySetSolverValues(1, [F(0), T(1)])
+        s.trySendAssignment(0)
+        s.trySendAssignment(1)
+        s.execute()
+        assert s.popAll(0) == [([F(0), mkLit(1)], a)]
+        assert s.popAll(1) == [([mkLit(0), F(1), mkLit(2)], b)]
 
 
 def test_rejected_clauses_and_bad_ids():
@@ -729,3 +746,88 @@ def test_solver_threads_and_gpu_thread_concurrently():
         for sid in range(S):
             for lits in got[sid]:
                 assert any(_useful(lits, a) for a in sent[sid]), (sid, lits)
+
+
+def test_device_activity_policy_keeps_active_clauses(monkeypatch):
+    """GPUSHARESAT_REDUCE_POLICY=device: reduceDb runs the reference's rule
+    (keep activity >= the approximate median, Clauses.cu:391-525) on the
+    device over a tagged DB (gss_res_reduce); the reported clause stays."""
+    monkeypatch.setenv("GPUSHARESAT_REDUCE_POLICY", "device")
+    with sharer(5, 1) as s:
+        s.addClause(-1, [F(1), mkLit(3), mkLit(4)])
+        used = s.addClause(-1, [F(0), mkLit(2), mkLit(4)])
+        assert s.trySetSolverValues(0, [T(0), F(4)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert [c for _, c in s.popAll(0)] == [used]
+        s.reduceDb()
+        assert s.getGlobalStat("gpuClauses") == 1
+        assert s.getGlobalStat("gpuClauseLengthSum") == 3
+        s.unsetSolverValues(0, [T(0), F(4)])
+        assert s.trySetSolverValues(0, [T(0), F(2), F(4)])
+        s.trySendAssignment(0)
+        s.execute()
+        assert [c for _, c in s.popAll(0)] == [used]
+
+
+@pytest.mark.parametrize("policy", ["device", "learned"])
+def test_device_policies_reduce_and_agree_with_the_mirror(tmp_path, monkeypatch, policy):
+    """The device policies through reduceDb: about half of the clauses of
+    length >= 3 go, every clause of length <= 2 stays (Clauses.cu:255), and
+    the device holds exactly the mirror's clauses (reports of a fresh
+    assignment = the clauses of the written CNF it makes falsified or unit).
+    The learned policy trains on the run's own records every 16 runs."""
+    monkeypatch.setenv("GPUSHARESAT_REDUCE_POLICY", policy)
+    rng = np.random.default_rng(31)
+    V = 300
+    with sharer(V, 3) as s:  # solvers 0 and 1 send; solver 2 checks the DB at the end
+        shorts = 0
+        for _ in range(3000):
+            n = int(rng.integers(1, 9))
+            shorts += n <= 2
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2)))
+                             for v in rng.choice(V, size=n, replace=False)])
+        for r in range(40):  # runs with assignments: records feed the policy state
+            for so in range(2):
+                s.unsetSolverValues(so, [mkLit(v) for v in range(V)])
+                assig = rng.choice(V, size=V // 2, replace=False)
+                assert s.trySetSolverValues(so, [mkLit(int(v), bool(rng.integers(2)))
+                                                 for v in assig])
+                s.trySendAssignment(so)
+            s.gpuRun()
+            s.popAll(0)
+            s.popAll(1)
+        before = s.getGlobalStat("gpuClauses")
+        s.reduceDb()
+        after = s.getGlobalStat("gpuClauses")
+        longs = before - shorts
+        assert shorts + 0.3 * longs <= after <= shorts + 0.7 * longs, (before, after, shorts)
+        p = tmp_path / "db.cnf"
+        s.writeClausesInCnf(str(p))
+        lens = [len(line.split()) - 1 for line in p.read_text().splitlines()[1:]]
+        assert sum(1 for n in lens if n <= 2) == shorts
+        assert _db_matches_cnf(s, tmp_path, V, rng, solver=2) == after
+
+
+def test_device_policy_out_of_memory_reduces_and_retries(tmp_path, monkeypatch):
+    """Device policy with a DB byte cap: the append that does not fit makes
+    the device reduce and the batch is added again, until it fits; device
+    and mirror agree afterwards."""
+    monkeypatch.setenv("GPUSHARESAT_REDUCE_POLICY", "device")
+    monkeypatch.setenv("GPUSHARESAT_DB_LIMIT_BYTES", str(160 << 10))
+    rng = np.random.default_rng(33)
+    V = 400
+    with sharer(V, 1) as s:
+        for _ in range(4000):
+            vs = rng.choice(V, size=int(rng.integers(3, 8)), replace=False)
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+        s.gpuRun()
+        assert not s.hasRunOutOfGpuMemoryOnce()
+        for _ in range(4000):
+            vs = rng.choice(V, size=int(rng.integers(3, 8)), replace=False)
+            s.addClause(-1, [mkLit(int(v), bool(rng.integers(2))) for v in vs])
+        s.gpuRun()
+        assert s.hasRunOutOfGpuMemoryOnce()
+        kept = s.getGlobalStat("gpuClauses")
+        assert 4000 < kept < 8000  # the new batch and what was kept of the first
+        assert _db_matches_cnf(s, tmp_path, V, rng) == kept

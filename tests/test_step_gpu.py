@@ -87,11 +87,14 @@ class Plain:
 
 
 def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
-              grow_at=None, eager=True, tuning=()):
+              grow_at=None, eager=True, tuning=(), page_limit=None):
     rng = np.random.default_rng(seed)
     a, b = engine_factory(), engine_factory()
     for key, val in tuning:
         a.set_tuning(key, val)
+    if page_limit is not None:  # host copy endpoints pageable past this size
+        a.set_page_locked_limit(page_limit)
+        b.set_page_locked_limit(page_limit)
     a.reset_assignments(V, W)
     b.reset_assignments(V, W)
     d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
@@ -210,6 +213,14 @@ def test_records_read_after_the_slot_is_enqueued_again(engine_factory, W):
     da, ta = a.download()
     db, tb = b.download()
     assert np.array_equal(da, db) and np.array_equal(ta, tb)
+
+
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_steps_with_pageable_host_buffers(engine_factory, pipelined):
+    # maxPageLockedMemory of 0 bytes: the records' host copies and the plain
+    # path's staging are pageable (CorrespArr.cu:212-235); results unchanged
+    run_steps(engine_factory, seed=380 + pipelined, V=1500, W=16, nsteps=6, n_new=500,
+              maxlen=20, n_ups=600, pipelined=pipelined, page_limit=0)
 
 
 def test_table_regrowth_recaptures(engine_factory):
