@@ -176,7 +176,8 @@ def main() -> int:
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--records", default="full", choices=["full", "ids"],
-                    help="serving-32: records with masks (full) or clause and word only (ids)")
+                    help="batched-16, serving-32: records with masks (full) or clause and word "
+                         "only (ids, what a solver receives)")
     ap.add_argument("--signs", default="planted", choices=["planted", "random"],
                     help="serving-32: clauses and assignments near a planted assignment, or "
                          "random signs and polarities")
@@ -283,17 +284,18 @@ def main() -> int:
                 "sweeps": len(lat), "bytes_per_record": size,
                 "what": "submit -> every hit record of the sweep in pinned host memory"}
 
-    latency = latency_of(sv.RECORDS_IDS)
-    latency_full = latency_of(sv.RECORDS_FULL)
+    latency = latency_of(sv.RECORDS_IDS) if args.lat_steps > 0 else None
+    latency_full = latency_of(sv.RECORDS_FULL) if args.lat_steps > 0 else None
     if latency:
         log(f"report latency (8-byte records) p50 {latency['p50']:.2f} ms, p99 "
             f"{latency['p99']:.2f} ms; 24-byte records p50 {latency_full['p50']:.2f} ms, p99 "
             f"{latency_full['p99']:.2f} ms ({hits * 8 / 1e6:.0f} / {hits * 24 / 1e6:.0f} MB)")
     # the ids the solvers receive are the first word of the full records
-    full = e.step_hits_view()
-    if first is not None and not np.array_equal(
-            np.sort(first.view(np.uint64)), np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0])):
-        raise RuntimeError("id records differ from the full records' first words")
+    if first is not None:
+        full = e.step_hits_view()
+        if not np.array_equal(np.sort(first.view(np.uint64)),
+                              np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0])):
+            raise RuntimeError("id records differ from the full records' first words")
     # The device's own ceiling for this access pattern: random 128-byte line
     # gathers (independent, 4 in flight per lane) from a table of the same size.
     kernel_name = "k_sweep<16,1,4,true>"
@@ -451,11 +453,15 @@ def run_batched(args) -> int:
     t_start = time.perf_counter()
     todo = list(range(args.warmup, steps_total))
 
+    from gpusharesat_amd import serving as sv
+    rec = sv.RECORDS_IDS if args.records == "ids" else sv.RECORDS_FULL
+
     def enqueue_next():
         nonlocal lits_db, checks
         st = plan[todo.pop(0)]
         pending.append(time.perf_counter())
-        e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
+        sv.submit(e, st.valid, lits=st.lits, lens=st.lens, ups=st.ups, last_pos=st.last_pos,
+                  records=rec)
         lits_db += int(st.lits.size)
         checks += lits_db * st.assignments
 
@@ -470,7 +476,8 @@ def run_batched(args) -> int:
         _, ms = e.step_wait()
         if todo:
             enqueue_next()
-        hits_total += e.step_hits_view().size  # records on the host (pinned, no copy)
+        # records on the host (pinned, no copy)
+        hits_total += (e.step_ids_view() if rec == sv.RECORDS_IDS else e.step_hits_view()).size
         done_t.append(time.perf_counter())
         lat.append(done_t[-1] - pending.pop(0))
         step_ms.append(ms)
@@ -499,7 +506,7 @@ def run_batched(args) -> int:
             "workload": "batched-16: 1M vars, 16 streams x U[16,256] assignments (64 words), "
                         "Poisson(16x2800) Pareto(1.5) clauses per step, DB 1M -> ~10M",
             "clauses_end": int(clauses_end), "literals_end": int(lits_db),
-            "hit_records": int(hits_total), "tuned": tuned,
+            "hit_records": int(hits_total), "tuned": tuned, "records": args.records,
             "verified_steps": verified, "steps_in_flight": args.inflight,
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
