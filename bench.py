@@ -456,12 +456,16 @@ def run_batched(args) -> int:
     from gpusharesat_amd import serving as sv
     rec = sv.RECORDS_IDS if args.records == "ids" else sv.RECORDS_FULL
 
+    submit_s = []
+
     def enqueue_next():
         nonlocal lits_db, checks
         st = plan[todo.pop(0)]
-        pending.append(time.perf_counter())
+        t = time.perf_counter()
+        pending.append(t)
         sv.submit(e, st.valid, lits=st.lits, lens=st.lens, ups=st.ups, last_pos=st.last_pos,
                   records=rec)
+        submit_s.append(time.perf_counter() - t)
         lits_db += int(st.lits.size)
         checks += lits_db * st.assignments
 
@@ -514,6 +518,8 @@ def run_batched(args) -> int:
                        "p999": float(np.percentile(lat_ms, 99.9)),
                        "max": float(lat_ms.max())},
         "graph_ms": {"mean": float(np.mean(step_ms)), "max": float(np.max(step_ms))},
+        "host_submit_ms": {"mean": float(np.mean(submit_s)) * 1e3,
+                           "max": float(np.max(submit_s)) * 1e3},
     }
     if n_plain:
         plain_rate = sum(plain_checks) / plain_s

@@ -249,6 +249,10 @@ struct StepSlot {
     bool busy = false;
     bool serve = false;    // the slot's last step admitted requests from the serving ring
     Pinned<ServeReport> h_report;  // what that step admitted (serving steps)
+    // serving steps: records made dense grouped by assignment word, and where
+    // each word's records start (h_group[w], [64] = all, [65] = 1 if grouped)
+    DevBuf<uint32_t> d_wcount;     // 64 counts + 64 cursors
+    Pinned<uint64_t> h_group;
     bool results = false;  // h_out / d_hits hold the results of the last waited step
     int rec_words = 3;     // record form of the slot's last submitted step (3: gss_hit,
                            // 1: gss_hit_id)
@@ -571,6 +575,7 @@ int gss_engine_destroy(gss_engine *e) {
         sl.d_fills.release();
         sl.d_dense.release();
         sl.d_offs.release();
+        sl.d_wcount.release();
         sl.d_cseg.release();
     }
     for (Bucket &b : e->buckets) {
@@ -1596,9 +1601,15 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     // records made dense on the device right after the sweep, so that reading
     // them on the host is one DMA copy (no compute units taken from the next
     // step's sweep)
+    // (serving steps: grouped by assignment word, so each request's records
+    // are one range)
     if (he == hipSuccess && sl.d_dense.cap > 0)
-        he = launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_offs.p,
-                                    sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap, sl.rec_words, cs);
+        he = serve ? launch_group_by_word(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_hits.p,
+                                          sl.rec_words, sl.d_wcount.p, sl.d_dense.p,
+                                          sl.d_dense.cap, sl.h_group.p, cs)
+                   : launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_offs.p,
+                                            sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap,
+                                            sl.rec_words, cs);
     hipGraph_t g = nullptr;
     hipError_t he2 = hipStreamEndCapture(cs, &g);
     if (he != hipSuccess) {
@@ -1675,6 +1686,18 @@ int gss::engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_c
     return GSS_OK;
 }
 
+int gss::step_word_groups(gss_engine *e, uint64_t *offs, bool *grouped) {
+    if (!e || !offs || !grouped) return set_err(GSS_E_INVAL, "null argument");
+    if (e->last_step_slot < 0) return set_err(GSS_E_STATE, "no waited step");
+    const StepSlot &sl = e->slots[e->last_step_slot];
+    // grouped only if the graph's dense buffer held them (else the records
+    // reached the host in chunk order)
+    *grouped = sl.serve && sl.h_group.p && sl.rec_total > 0 &&
+               (uint64_t)sl.rec_total <= sl.step_dense && sl.h_group.p[GSS_MAX_WORDS + 1] == 1;
+    if (*grouped) std::memcpy(offs, sl.h_group.p, (GSS_MAX_WORDS + 1) * sizeof(uint64_t));
+    return GSS_OK;
+}
+
 int gss::step_serve_report(gss_engine *e, ServeReport *out) {
     if (!e || !out) return set_err(GSS_E_INVAL, "null argument");
     if (e->last_step_slot < 0) return set_err(GSS_E_STATE, "no waited step");
@@ -1715,12 +1738,19 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     // ---- validation (nothing changes before it passes; the updates are
     // checked while they are copied into this idle slot's pinned staging)
     if ((rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1)))) return rc;
-    for (int64_t i = 0; i < n_ups; i++) {
-        const gss_var_update u = ups[i];
-        if (u.var >= (uint64_t)e->vars || u.word >= (uint32_t)W)
-            return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
-                           (long long)i, u.var, u.word);
-        sl.h_ups.p[i] = u;
+    if (n_ups) {
+        // bulk copy, then a branch-free range check (the error path finds
+        // the culprit)
+        std::memcpy(sl.h_ups.p, ups, (size_t)n_ups * sizeof(gss_var_update));
+        const uint32_t vmax = (uint32_t)std::min<int64_t>(e->vars, 0xFFFFFFFFll);
+        uint32_t bad = 0;
+        for (int64_t i = 0; i < n_ups; i++)
+            bad |= (uint32_t)(ups[i].var >= vmax) | (uint32_t)(ups[i].word >= (uint32_t)W);
+        if (bad)
+            for (int64_t i = 0; i < n_ups; i++)
+                if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
+                    return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
+                                   (long long)i, ups[i].var, ups[i].word);
     }
     for (int64_t i = 0; i < n_patches; i++)
         if (desc->patches[i].var >= (uint64_t)e->vars || desc->patches[i].word >= (uint32_t)W)
@@ -1757,11 +1787,8 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     // literals are staged in the caller's order while their maximum is taken
     if ((rc = sl.h_lits.reserve(std::max<int64_t>(total, 1)))) return rc;
     uint32_t mx = 0;
-    for (int64_t i = 0; i < total; i++) {
-        const uint32_t x = lits[i];
-        mx = std::max(mx, x);
-        sl.h_lits.p[i] = x;
-    }
+    if (total) std::memcpy(sl.h_lits.p, lits, (size_t)total * sizeof(uint32_t));
+    for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
     if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
     const int64_t need_lit = std::max<int64_t>(n ? (int64_t)mx : -1, e->max_lit);
     if (need_lit >= 2 * e->vars)
@@ -1819,7 +1846,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     // this step, frees of replaced buffers included, runs there.
     hipStream_t es = e->stream;
     if (sl.copy_queued) HIP_TRY(hipStreamWaitEvent(es, sl.ev_copied, 0));
-    if (serve && (rc = sl.h_report.reserve(1))) return rc;
+    if (serve && ((rc = sl.h_report.reserve(1)) || (rc = sl.h_group.reserve(GSS_MAX_WORDS + 2))))
+        return rc;
+    if (serve && (rc = sl.d_wcount.reserve_async(2 * GSS_MAX_WORDS, e->stream))) return rc;
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
         (rc = sl.d_cls.reserve_async(sl.h_cls.cap, es)) ||
         (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
@@ -1863,7 +1892,8 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (const void *)(intptr_t)e->tune_unroll, (const void *)(intptr_t)blocks,
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
         (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words,
-        (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr};
+        (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr,
+        sl.h_group.p, sl.d_wcount.p};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
         if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve))) return rc;

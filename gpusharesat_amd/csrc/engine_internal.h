@@ -34,6 +34,10 @@ int engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,
 int step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve);
 // What the last waited serving step admitted.
 int step_serve_report(gss_engine *e, ServeReport *out);
+// For the last waited serving step: whether its records reached the host
+// grouped by assignment word, and then offs[w] = first record of word w,
+// offs[64] = all (65 entries).
+int step_word_groups(gss_engine *e, uint64_t *offs, bool *grouped);
 // Makes the engine's device current (GSS_OK or an error code).
 int engine_activate(gss_engine *e);
 

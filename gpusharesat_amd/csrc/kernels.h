@@ -191,6 +191,13 @@ hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, ui
 hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                                   uint64_t *offs, const void *hits, void *dense,
                                   uint64_t dense_cap, int rec_words, hipStream_t s);
+// Dense copy of a sweep's records grouped by assignment word, ascending:
+// counts = 2 * 64 uint32 of device scratch (cleared here), group = 66 uint64:
+// group[w] = first record of word w, group[64] = all, group[65] = 1 if they
+// fitted in dense_cap records (else nothing moved).
+hipError_t launch_group_by_word(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                const void *hits, int rec_words, uint32_t *counts,
+                                void *dense, uint64_t dense_cap, uint64_t *group, hipStream_t s);
 // Solver base planes: bases[(2 b) * vw + i] / bases[(2 b + 1) * vw + i] hold
 // bit (v mod 64) of word i = v / 64 of base b's is_def / is_true.
 hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uint64_t *bases,

@@ -680,6 +680,84 @@ __global__ __launch_bounds__(1024) void k_count_records(const uint32_t *alloc,
     }
 }
 
+// ---- records grouped by assignment word (serving steps) ----------------------
+
+// Records per word over the claimed chunks (word = the top 16 bits of a
+// record's first word); counts[64] must start at zero.
+__global__ __launch_bounds__(256) void k_word_count(const uint32_t *alloc,
+                                                    const uint32_t *__restrict__ fills,
+                                                    uint32_t cap, const uint64_t *__restrict__ hits,
+                                                    uint32_t rec_words, uint32_t *counts) {
+    __shared__ uint32_t h[GSS_MAX_WORDS];
+    if (threadIdx.x < GSS_MAX_WORDS) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const uint32_t f = fills[c];
+        const uint64_t *src = hits + (uint64_t)c * kHitChunk * rec_words;
+        for (uint32_t i = threadIdx.x; i < f; i += blockDim.x)
+            atomicAdd(&h[(uint32_t)(src[(uint64_t)i * rec_words] >> 48) & (GSS_MAX_WORDS - 1)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < GSS_MAX_WORDS && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+
+// Moves the records into `dense` grouped by word, ascending (within a word in
+// no set order): a block takes one chunk at a time, counts its records per
+// word in LDS, reserves each word's share with one atomic on that word's
+// cursor, then places them.  group[w] = first record of word w, group[64] =
+// all records, group[65] = 1 when they fitted in dense_cap (else nothing is
+// moved and the host groups the records itself).  cursor[64] starts at zero.
+__global__ __launch_bounds__(256) void k_word_scatter(const uint32_t *alloc,
+                                                      const uint32_t *__restrict__ fills,
+                                                      uint32_t cap,
+                                                      const uint64_t *__restrict__ hits,
+                                                      uint32_t rec_words,
+                                                      const uint32_t *__restrict__ counts,
+                                                      uint32_t *cursor, uint64_t *__restrict__ dense,
+                                                      uint64_t dense_cap, uint64_t *group) {
+    __shared__ uint64_t base[GSS_MAX_WORDS + 1];
+    __shared__ uint32_t lc[GSS_MAX_WORDS], lbase[GSS_MAX_WORDS];
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (int w = 0; w < GSS_MAX_WORDS; w++) {
+            base[w] = acc;
+            acc += counts[w];
+        }
+        base[GSS_MAX_WORDS] = acc;
+    }
+    __syncthreads();
+    const bool fits = base[GSS_MAX_WORDS] <= dense_cap;
+    if (blockIdx.x == 0 && threadIdx.x <= GSS_MAX_WORDS + 1)
+        group[threadIdx.x] = threadIdx.x <= GSS_MAX_WORDS ? base[threadIdx.x] : (fits ? 1ull : 0ull);
+    if (!fits) return;  // the same for every thread of every block
+    const uint32_t used = *alloc;
+    const uint32_t n = used < cap ? used : cap;
+    for (uint32_t c = blockIdx.x; c < n; c += gridDim.x) {
+        if (threadIdx.x < GSS_MAX_WORDS) lc[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t f = fills[c];
+        const uint64_t *src = hits + (uint64_t)c * kHitChunk * rec_words;
+        for (uint32_t i = threadIdx.x; i < f; i += blockDim.x)
+            atomicAdd(&lc[(uint32_t)(src[(uint64_t)i * rec_words] >> 48) & (GSS_MAX_WORDS - 1)], 1u);
+        __syncthreads();
+        if (threadIdx.x < GSS_MAX_WORDS) {
+            const uint32_t k = lc[threadIdx.x];
+            lbase[threadIdx.x] = k ? atomicAdd(&cursor[threadIdx.x], k) : 0u;
+            lc[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < f; i += blockDim.x) {
+            const uint64_t *r = src + (uint64_t)i * rec_words;
+            const uint32_t w = (uint32_t)(r[0] >> 48) & (GSS_MAX_WORDS - 1);
+            const uint64_t pos = base[w] + lbase[w] + atomicAdd(&lc[w], 1u);
+            for (uint32_t q = 0; q < rec_words; q++) dense[pos * rec_words + q] = r[q];
+        }
+        __syncthreads();
+    }
+}
+
 // ---- serving: requests admitted on the device ---------------------------------
 
 __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, int lane) {
@@ -1011,6 +1089,22 @@ hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, 
     hipLaunchKernelGGL(k_gather_chunks, dim3(2048), dim3(256), 0, s, alloc, fills, cap,
                        (const uint64_t *)offs, (const uint64_t *)hits, (uint64_t *)dense,
                        dense_cap, (uint32_t)rec_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_by_word(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
+                                const void *hits, int rec_words, uint32_t *counts,
+                                void *dense, uint64_t dense_cap, uint64_t *group, hipStream_t s) {
+    if (rec_words != 1 && rec_words != 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_u32, dim3(1), dim3(256), 0, s, counts, 2 * GSS_MAX_WORDS);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_word_count, dim3(1024), dim3(256), 0, s, alloc, fills, cap,
+                       (const uint64_t *)hits, (uint32_t)rec_words, counts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_word_scatter, dim3(1024), dim3(256), 0, s, alloc, fills, cap,
+                       (const uint64_t *)hits, (uint32_t)rec_words, counts,
+                       counts + GSS_MAX_WORDS, (uint64_t *)dense, dense_cap, group);
     return hipGetLastError();
 }
 

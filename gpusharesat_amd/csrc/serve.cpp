@@ -59,6 +59,7 @@ struct StepInfo {
 struct StepRecords {
     std::vector<uint64_t> words;
     int rw = 3;
+    bool absolute = false;  // words not yet made relative (subtract first_word)
 };
 
 struct Done {
@@ -220,6 +221,9 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
             raw = (const uint64_t *)p;
         }
     }
+    uint64_t goff[GSS_MAX_WORDS + 1];
+    bool grouped = false;
+    if (!rc && n > 0) rc = gss::step_word_groups(s->e, goff, &grouped);
     const Clock::time_point now = Clock::now();
     // the requests it admitted are the oldest pending ones (only this thread
     // removes them, so they stay put while the lock is released)
@@ -253,7 +257,18 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     auto step = std::make_shared<StepRecords>();
     step->rw = rw;
     std::vector<size_t> first(cnt + 1, 0);
-    if (order_ok && n > 0) {
+    std::vector<size_t> last(cnt, 0);
+    if (order_ok && n > 0 && grouped) {
+        // grouped by word on the device: each request's records are one
+        // range; a single copy keeps them past the slot's reuse
+        step->words.assign(raw, raw + (size_t)n * rw);
+        step->absolute = true;
+        for (int i = 0; i < cnt; i++) {
+            const uint32_t nw = (uint32_t)(ks[i] + 63) / 64;
+            first[i] = goff[w0[i]];
+            last[i] = goff[std::min<uint32_t>(w0[i] + nw, GSS_MAX_WORDS)];
+        }
+    } else if (order_ok && n > 0) {
         std::vector<size_t> count(cnt, 0);
         for (int64_t r = 0; r < n; r++) {
             const uint32_t word = (uint32_t)(raw[r * rw] >> 48);
@@ -261,6 +276,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
             if (i >= 0) count[i]++;  // other words are never valid
         }
         for (int i = 0; i < cnt; i++) first[i + 1] = first[i] + count[i];
+        for (int i = 0; i < cnt; i++) last[i] = first[i + 1];
         step->words.resize(first[cnt] * rw);
         std::vector<size_t> at(first.begin(), first.end() - 1);
         for (int64_t r = 0; r < n; r++) {
@@ -299,7 +315,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
         dn.r.k = p.k;
         dn.r.first_word = (int32_t)w0[i];
         dn.r.latency_ms = std::chrono::duration<double, std::milli>(now - p.t_submit).count();
-        dn.r.n_records = (int64_t)(first[i + 1] - first[i]);
+        dn.r.n_records = (int64_t)(last[i] - first[i]);
         dn.r.records = nullptr;
         dn.step = step;
         dn.first = first[i];
@@ -571,7 +587,7 @@ int gss_server_records(gss_server *s, int64_t id, const gss_serve_hit **out, int
             const uint32_t len = (uint32_t)(src[0] >> 32) & 0xFFFFu;
             gss_serve_hit &h = d.recs[r];
             h.clause = (len < s->ids.size() && row < s->ids[len].size()) ? s->ids[len][row] : -1;
-            h.word = (uint32_t)(src[0] >> 48);
+            h.word = (uint32_t)(src[0] >> 48) - (d.step->absolute ? (uint32_t)d.r.first_word : 0u);
             h.pad = 0;
             h.false_mask = rw == 3 ? src[1] : 0;
             h.unit_mask = rw == 3 ? src[2] : 0;
