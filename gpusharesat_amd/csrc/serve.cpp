@@ -56,10 +56,23 @@ struct StepInfo {
 // One step's records grouped by request, as the step recorded them (rw
 // 8-byte words each; the first word is row | len << 32 | word << 48 with the
 // word made relative to the request).
+//
+// Grouped on the device, a step's records are lent straight from the engine's
+// pinned buffer (valid until that step's slot is waited for again, two steps
+// on); before that wait, the records of a step whose results a client still
+// holds unread are copied (materialize).
 struct StepRecords {
     std::vector<uint64_t> words;
+    const uint64_t *lent = nullptr;  // engine pinned memory, while valid
+    int64_t n = 0;                   // records
     int rw = 3;
-    bool absolute = false;  // words not yet made relative (subtract first_word)
+    bool absolute = false;           // words not yet made relative (subtract first_word)
+    const uint64_t *data() const { return lent ? lent : words.data(); }
+    void materialize() {
+        if (!lent) return;
+        words.assign(lent, lent + (size_t)n * rw);
+        lent = nullptr;
+    }
 };
 
 struct Done {
@@ -96,6 +109,7 @@ struct gss_server {
     std::deque<StepInfo> inflight;
     int64_t steps = 0;
     std::unordered_map<int64_t, Done> done;
+    std::deque<std::pair<int64_t, std::weak_ptr<StepRecords>>> lent;  // (step, records)
     std::deque<int64_t> ready;
     int64_t records = 0, requests_done = 0;
     double checks = 0.0, device_ms = 0.0;
@@ -202,6 +216,13 @@ void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
 // request, then copy grouped by request), outside the lock; clause ids are
 // looked up only when a client reads a request's records.
 void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
+    // the step about to be waited for reuses the slot of the step two before
+    // it: records lent from there that a client still holds are copied now
+    const int64_t next = s->inflight.front().index;
+    while (!s->lent.empty() && s->lent.front().first <= next - 2) {
+        if (auto sp = s->lent.front().second.lock()) sp->materialize();
+        s->lent.pop_front();
+    }
     lk.unlock();
     int64_t n = 0;
     float ms = 0.f;
@@ -260,8 +281,9 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     std::vector<size_t> last(cnt, 0);
     if (order_ok && n > 0 && grouped) {
         // grouped by word on the device: each request's records are one
-        // range; a single copy keeps them past the slot's reuse
-        step->words.assign(raw, raw + (size_t)n * rw);
+        // range of the engine's pinned buffer, lent until the slot's reuse
+        step->lent = raw;
+        step->n = n;
         step->absolute = true;
         for (int i = 0; i < cnt; i++) {
             const uint32_t nw = (uint32_t)(ks[i] + 63) / 64;
@@ -322,6 +344,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
         s->pending_words -= (p.k + 63) / 64;
         s->ready.push_back(dn.r.request);
     }
+    if (step->lent) s->lent.emplace_back(si.index, step);
     if (cnt) {
         const Pending &last = s->pending[cnt - 1];
         s->patch_completed = last.patch_off + last.n_patches;
@@ -352,7 +375,11 @@ void serve_loop(gss_server *s) {
         }
         if (s->stop) break;
     }
-    // a failed server still drains its steps, so the device is idle at exit
+    // a failed server still drains its steps, so the device is idle at exit;
+    // the records lent from the engine are copied before any slot is reused
+    for (auto &l : s->lent)
+        if (auto sp = l.second.lock()) sp->materialize();
+    s->lent.clear();
     while (!s->inflight.empty()) {
         lk.unlock();
         int64_t n;
@@ -582,7 +609,7 @@ int gss_server_records(gss_server *s, int64_t id, const gss_serve_hit **out, int
         const int rw = d.step->rw;
         d.recs.resize((size_t)d.r.n_records);
         for (int64_t r = 0; r < d.r.n_records; r++) {
-            const uint64_t *src = d.step->words.data() + (d.first + (size_t)r) * rw;
+            const uint64_t *src = d.step->data() + (d.first + (size_t)r) * rw;
             const uint32_t row = (uint32_t)src[0];
             const uint32_t len = (uint32_t)(src[0] >> 32) & 0xFFFFu;
             gss_serve_hit &h = d.recs[r];
