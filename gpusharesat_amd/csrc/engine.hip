@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -32,6 +33,7 @@
 #include "gss.h"
 #include "engine_internal.h"
 #include "kernels.h"
+#include "stage_pool.h"
 
 using namespace gss;
 
@@ -56,6 +58,8 @@ uint64_t *g_scratch_ptr[kMaxDevices] = {};
 // Clauses of at most this many literals read both rows of each literal in one
 // pass (see kernels.hip).
 constexpr int kDefaultFuseLen = 4;
+// steps staging at least this many bytes split their copies over the stage pool
+constexpr size_t kStagePoolMinBytes = 1u << 20;
 
 // Clause literals are stored on the device as row offsets, literal << 7 (the
 // byte offset of the literal's row in a 16-word table); see kernels.hip.
@@ -248,6 +252,10 @@ struct StepSlot {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool busy = false;
     bool serve = false;    // the slot's last step admitted requests from the serving ring
+    // the slot's graph holds the base-plane nodes (base updates, word loads)
+    // and the patch node once a step of the slot needed them (then always:
+    // a caller alternating between step kinds does not re-capture each time)
+    bool g_bases = false, g_patches = false;
     Pinned<ServeReport> h_report;  // what that step admitted (serving steps)
     // serving steps: records made dense grouped by assignment word, and where
     // each word's records start (h_group[w], [64] = all, [65] = 1 if grouped)
@@ -376,6 +384,10 @@ struct gss_engine {
     int64_t last_step_hits = 0;
     uint32_t last_step_chunks = 0;
     float last_step_ms = 0.f;
+    // host threads for the staging copies of large steps
+    // (GPUSHARESAT_STAGE_THREADS, default 4; 0 or 1 = the calling thread only)
+    int stage_threads = 4;
+    std::unique_ptr<gss::StagePool> stage_pool;
 };
 
 namespace {
@@ -515,6 +527,8 @@ int gss_engine_create(int device, gss_engine **out) {
     e->device = device;
     if (const char *lim = std::getenv("GPUSHARESAT_DB_LIMIT_BYTES"))
         e->db_limit = std::max<long long>(0, std::atoll(lim));
+    if (const char *st = std::getenv("GPUSHARESAT_STAGE_THREADS"))
+        e->stage_threads = std::min(64, std::max(0, std::atoi(st)));
     int rc = GSS_OK;
     auto fail = [&](int code) {
         gss_engine_destroy(e);
@@ -1577,17 +1591,17 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     if (he == hipSuccess) he = launch_stage_in(st, cs);
     if (he == hipSuccess)
         he = launch_scatter_clauses(sl.d_cls.p, &dh->nclauses, sl.d_lits.p, kLitShift, cs);
-    if (he == hipSuccess)
+    if (he == hipSuccess && sl.g_bases)
         he = launch_base_apply_dyn(sl.d_bups.p, &dh->nbups, e->bases.p, e->bases_vw, cs);
     // serving: the published requests that fit are admitted here, on the
     // device, when the step starts (their loads, valid slots and counts)
     if (he == hipSuccess && serve)
         he = launch_serve_admit(e->serve, dh, sl.d_loads.p, sl.h_report.p, e->words, cs);
-    if (he == hipSuccess)
+    if (he == hipSuccess && sl.g_bases)
         he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
                                    e->table.p, e->words, e->vars, cs);
     if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
-    if (he == hipSuccess)
+    if (he == hipSuccess && sl.g_patches)
         he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, cs)
                    : launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words,
                                             cs);
@@ -1735,22 +1749,60 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     if (sl.busy) return set_err(GSS_E_STATE, "two steps in flight: wait for the oldest first");
     const int W = e->words;
     if (n > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many clauses in one step");
-    // ---- validation (nothing changes before it passes; the updates are
-    // checked while they are copied into this idle slot's pinned staging)
-    if ((rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1)))) return rc;
-    if (n_ups) {
-        // bulk copy, then a branch-free range check (the error path finds
-        // the culprit)
-        std::memcpy(sl.h_ups.p, ups, (size_t)n_ups * sizeof(gss_var_update));
-        const uint32_t vmax = (uint32_t)std::min<int64_t>(e->vars, 0xFFFFFFFFll);
+    // ---- validation (nothing changes before it passes).  The updates and
+    // the literals are copied into this idle slot's pinned staging while they
+    // are checked; a large step splits those copies over the stage pool.
+    int maxlen = 0;
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
+            return set_err(GSS_E_INVAL, "clause %lld: length %d outside [1, %d]", (long long)i,
+                           lens[i], GSS_MAX_CLAUSE_LEN);
+        maxlen = std::max(maxlen, (int)lens[i]);
+        total += lens[i];
+    }
+    if (total > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "too many literals in one step");
+    if ((rc = sl.h_ups.reserve(std::max<int64_t>(n_ups, 1))) ||
+        (rc = sl.h_lits.reserve(std::max<int64_t>(total, 1))))
+        return rc;
+    gss::StagePool *pool = nullptr;
+    const size_t stage_bytes = (size_t)n_ups * sizeof(gss_var_update) + (size_t)total * 4 +
+                               (size_t)n * sizeof(ClauseSeg);
+    if (e->stage_threads > 1 && stage_bytes >= kStagePoolMinBytes) {
+        if (!e->stage_pool) {
+            try {
+                e->stage_pool.reset(new gss::StagePool(e->stage_threads));
+            } catch (...) {
+                e->stage_threads = 0;  // no threads to be had: stage on this thread
+            }
+        }
+        pool = e->stage_pool.get();
+    }
+    const int parts = pool ? pool->size() : 1;
+    const uint32_t vmax = (uint32_t)std::min<int64_t>(e->vars, 0xFFFFFFFFll);
+    std::vector<uint32_t> part_bad(parts, 0), part_max(parts, 0);
+    // part k: updates [n_ups k / parts, ...) and literals [total k / parts, ...)
+    auto copy_part = [&](int k) {
+        const int64_t u0 = n_ups * k / parts, u1 = n_ups * (k + 1) / parts;
         uint32_t bad = 0;
-        for (int64_t i = 0; i < n_ups; i++)
-            bad |= (uint32_t)(ups[i].var >= vmax) | (uint32_t)(ups[i].word >= (uint32_t)W);
-        if (bad)
-            for (int64_t i = 0; i < n_ups; i++)
-                if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
-                    return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
-                                   (long long)i, ups[i].var, ups[i].word);
+        for (int64_t i = u0; i < u1; i++) {
+            const gss_var_update u = ups[i];
+            bad |= (uint32_t)(u.var >= vmax) | (uint32_t)(u.word >= (uint32_t)W);
+            sl.h_ups.p[i] = u;
+        }
+        const int64_t l0 = total * k / parts, l1 = total * (k + 1) / parts;
+        if (l1 > l0) std::memcpy(sl.h_lits.p + l0, lits + l0, (size_t)(l1 - l0) * sizeof(uint32_t));
+        uint32_t mx = 0;
+        for (int64_t i = l0; i < l1; i++) mx = std::max(mx, lits[i]);
+        part_bad[k] = bad;
+        part_max[k] = mx;
+    };
+    gss::StageJoin join;  // every return below waits for the copies first
+    if (pool) {
+        pool->start(copy_part);
+        join.pool = pool;
+    } else {
+        copy_part(0);
     }
     for (int64_t i = 0; i < n_patches; i++)
         if (desc->patches[i].var >= (uint64_t)e->vars || desc->patches[i].word >= (uint32_t)W)
@@ -1774,29 +1826,52 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         if (valid[w]) active_words++;
     }
     if (serve) active_words = W;  // any word may be admitted on the device
-    int maxlen = 0;
-    int64_t total = 0;
-    for (int64_t i = 0; i < n; i++) {
-        if (lens[i] < 1 || lens[i] > GSS_MAX_CLAUSE_LEN)
-            return set_err(GSS_E_INVAL, "clause %lld: length %d outside [1, %d]", (long long)i,
-                           lens[i], GSS_MAX_CLAUSE_LEN);
-        maxlen = std::max(maxlen, (int)lens[i]);
-        total += lens[i];
+    // clause records: part k of cparts writes clauses [n k / cparts, ...); its
+    // first row in each bucket (after the step's earlier parts) and its first
+    // literal are counted here, while the copies run
+    const int cparts = (pool && maxlen <= 4096 && n >= 4096) ? parts : 1;
+    const size_t stride = (size_t)maxlen + 1;
+    std::vector<int64_t> per(stride, 0), part_row(cparts * stride, 0);
+    std::vector<uint32_t> part_pos(cparts, 0);
+    {
+        uint32_t pos = 0;
+        for (int k = 0; k < cparts; k++) {
+            int64_t *h = part_row.data() + k * stride;
+            part_pos[k] = pos;
+            const int64_t c0 = n * k / cparts, c1 = n * (k + 1) / cparts;
+            for (int64_t i = c0; i < c1; i++) {
+                h[lens[i]]++;
+                pos += (uint32_t)lens[i];
+            }
+        }
+        for (size_t len = 1; len < stride; len++) {
+            int64_t acc = 0;
+            for (int k = 0; k < cparts; k++) {
+                int64_t &h = part_row[k * stride + len];
+                const int64_t c = h;
+                h = acc;
+                acc += c;
+            }
+            per[len] = acc;
+        }
     }
-    if (total > 0xFFFFFFFFll) return set_err(GSS_E_INVAL, "too many literals in one step");
-    // literals are staged in the caller's order while their maximum is taken
-    if ((rc = sl.h_lits.reserve(std::max<int64_t>(total, 1)))) return rc;
-    uint32_t mx = 0;
-    if (total) std::memcpy(sl.h_lits.p, lits, (size_t)total * sizeof(uint32_t));
-    for (int64_t i = 0; i < total; i++) mx = std::max(mx, lits[i]);
+    if (pool) pool->wait();
+    uint32_t bad = 0, mx = 0;
+    for (int k = 0; k < parts; k++) {
+        bad |= part_bad[k];
+        mx = std::max(mx, part_max[k]);
+    }
+    if (bad)
+        for (int64_t i = 0; i < n_ups; i++)
+            if (ups[i].var >= (uint64_t)e->vars || ups[i].word >= (uint32_t)W)
+                return set_err(GSS_E_INVAL, "update %lld: var %u / word %u out of range",
+                               (long long)i, ups[i].var, ups[i].word);
     if (mx > kMaxLit) return set_err(GSS_E_INVAL, "literal %u exceeds %u", mx, kMaxLit);
     const int64_t need_lit = std::max<int64_t>(n ? (int64_t)mx : -1, e->max_lit);
     if (need_lit >= 2 * e->vars)
         return set_err(GSS_E_STATE, "clause literal %lld needs variable %lld but the table has "
                                     "%lld variables; grow the table before the step",
                        (long long)need_lit, (long long)(need_lit >> 1), (long long)e->vars);
-    std::vector<int64_t> per(maxlen + 1, 0);
-    for (int64_t i = 0; i < n; i++) per[lens[i]]++;
     // ---- capacity: buckets, staging, device copies, hit buffer
     if ((int)e->buckets.size() <= maxlen) e->buckets.resize(maxlen + 1);  // empty buckets
     bool grow = false;
@@ -1881,6 +1956,8 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     if (!sl.t1) HIP_TRY(hipEventCreate(&sl.t1));
     // ---- graph over exactly these buffers and settings
     const int fuse = fuse_of(e);
+    sl.g_bases |= serve || n_bups > 0 || n_loads > 0;
+    sl.g_patches |= serve || n_patches > 0;
     std::vector<const void *> key = {
         sl.h_hdr.p, sl.h_desc.p, sl.h_cls.p, sl.h_lits.p, sl.h_ups.p, sl.h_out.p,
         sl.h_patches.p, sl.h_bups.p, sl.h_loads.p,
@@ -1893,7 +1970,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
         (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words,
         (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr,
-        sl.h_group.p, sl.d_wcount.p};
+        sl.h_group.p, sl.d_wcount.p, (const void *)(intptr_t)(sl.g_bases + 2 * sl.g_patches)};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
         if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve))) return rc;
@@ -1904,19 +1981,26 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     // ---- stage this step (the slot is idle) and commit it to the DB: one
     // record per clause (the device copies it to its bucket row)
     {
-        std::vector<int64_t> fill(maxlen + 1, 0);
-        uint32_t pos = 0;
-        for (int64_t i = 0; i < n; i++) {
-            const int len = lens[i];
-            const Bucket &b = e->buckets[len];
-            const int64_t row = b.count + fill[len]++;
-            ClauseSeg &c = sl.h_cls.p[i];
-            c.dst = b.d + (size_t)row * len;
-            c.src = pos;
-            c.len = (uint32_t)len;
-            if (out_rows) out_rows[i] = (uint32_t)row;
-            pos += (uint32_t)len;
-        }
+        auto fill_part = [&](int k) {
+            int64_t *fill = part_row.data() + k * stride;
+            uint32_t pos = part_pos[k];
+            const int64_t c0 = n * k / cparts, c1 = n * (k + 1) / cparts;
+            for (int64_t i = c0; i < c1; i++) {
+                const int len = lens[i];
+                const Bucket &b = e->buckets[len];
+                const int64_t row = b.count + fill[len]++;
+                ClauseSeg &c = sl.h_cls.p[i];
+                c.dst = b.d + (size_t)row * len;
+                c.src = pos;
+                c.len = (uint32_t)len;
+                if (out_rows) out_rows[i] = (uint32_t)row;
+                pos += (uint32_t)len;
+            }
+        };
+        if (cparts > 1)
+            pool->run(fill_part);
+        else
+            fill_part(0);
         for (int len = 1; len <= maxlen; len++) e->buckets[len].count += per[len];
     }
     e->total_clauses += n;

@@ -266,6 +266,56 @@ def test_sweep_only_step_and_errors(engine_factory):
     assert a.totals() == b.totals()
 
 
+@pytest.mark.parametrize("threads", ["1", "3", "4"])
+def test_large_steps_split_over_stage_threads(engine_factory, monkeypatch, threads):
+    """Steps staging more than 1 MB (50k updates, up to 20k clauses) split
+    their copies and clause records over the engine's stage threads
+    (GPUSHARESAT_STAGE_THREADS, read when the engine is created); the records,
+    rows and table must be those of the plain sequence and the oracle for any
+    thread count, uneven part sizes included."""
+    monkeypatch.setenv("GPUSHARESAT_STAGE_THREADS", threads)
+    run_steps(engine_factory, seed=1300 + int(threads), V=20000, W=16, nsteps=4,
+              n_new=20000, maxlen=30, n_ups=50000, pipelined=True)
+
+
+def test_large_step_errors_leave_the_engine_unchanged(engine_factory, monkeypatch):
+    """A large step whose last update or last literal is out of range is
+    refused while its copies run on the stage threads; nothing of it reaches
+    the engine, and the next step matches the plain sequence."""
+    monkeypatch.setenv("GPUSHARESAT_STAGE_THREADS", "4")
+    rng = np.random.default_rng(1400)
+    V, W = 20000, 16
+    a, b = engine_factory(), engine_factory()
+    for e in (a, b):
+        e.reset_assignments(V, W)
+    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    a.upload(d0, t0)
+    b.upload(d0, t0)
+    lits, lens, ups, valid, last_pos = make_step(rng, V, W, 20000, 30, 50000)
+    before = a.totals()
+    bad_ups = ups.copy()
+    bad_ups["var"][-1] = V
+    with pytest.raises(GssError, match="out of range"):
+        a.step_enqueue(lits, lens, bad_ups, valid, last_pos)
+    bad_lits = lits.copy()
+    bad_lits[-1] = 2 * V
+    with pytest.raises(GssError, match="grow the table"):
+        a.step_enqueue(bad_lits, lens, ups, valid, last_pos)
+    assert a.totals() == before
+    assert np.array_equal(a.download()[0], d0)
+    plain = Plain(b)
+    exp = plain.step(lits, lens, ups, valid, last_pos)
+    rows = a.step_enqueue(lits, lens, ups, valid, last_pos)
+    n, _ = a.step_wait()
+    got = to_records(a.step_hits(), lens, rows)
+    assert n == got.size
+    assert_same(got, exp, "large step after refused ones")
+    da, ta = a.download()
+    db, tb = b.download()
+    assert np.array_equal(da, db) and np.array_equal(ta, tb)
+
+
 def test_id_records_are_the_full_records_first_words(engine_factory):
     """A step submitted with GSS_RECORDS_IDS records, for every (clause, word)
     with a report, exactly the first 8-byte word of the gss_hit the full form
