@@ -175,6 +175,8 @@ def main() -> int:
                              "perftest"])
     ap.add_argument("--duration", type=float, default=60.0, help="serving-32: seconds")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
+    ap.add_argument("--preload", type=int, default=0,
+                    help="serving-32: clauses in the DB before the run (default none)")
     ap.add_argument("--records", default="full", choices=["full", "ids"],
                     help="batched-16, serving-32: records with masks (full) or clause and word "
                          "only (ids, what a solver receives)")
@@ -553,6 +555,10 @@ def run_serving(args) -> int:
             tuned = args.tuning
         else:
             tuned = None if args.no_autotune else srv.tune()
+        preloaded = 0
+        if args.preload:
+            preloaded = srv.preload(args.preload)
+            log(f"preloaded {preloaded} clauses")
         out = srv.run()
     finally:
         srv.close()
@@ -587,12 +593,20 @@ def run_serving(args) -> int:
                    "served_per_s": out.requests / out.elapsed,
                    "clauses_end": out.clauses_end, "literals_end": out.literals_end,
                    "records_total": out.records, "max_queue": out.max_queue, "tuned": tuned,
-                   "steps_in_flight": cfg.max_inflight, "loop": "native (gss_server)"},
+                   "steps_in_flight": cfg.max_inflight, "loop": "native (gss_server)",
+                   "prefilter": os.environ.get("GPUSHARESAT_SERVE_PREFILTER", "1") != "0",
+                   "prefilter_vote": int(os.environ.get("GPUSHARESAT_SERVE_VOTE", "2")),
+                   "preloaded_clauses": preloaded},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
                        "max": float(lat.max()),
                        "what": "arrival -> the request's records handed to it on the host"},
         "latency_by_arrival_10s": windows,
         "device_ms_per_step": out.device_ms / max(1, out.steps),
+        "device_ms_per_step_by_window": [
+            {"from_s": round(a[0], 1), "to_s": round(b[0], 1), "clauses_end": int(b[3]),
+             "steps": int(b[1] - a[1]),
+             "device_ms": (b[2] - a[2]) / (b[1] - a[1]) if b[1] > a[1] else None}
+            for a, b in zip(out.samples, out.samples[1:])],
     }), flush=True)
     return 0
 

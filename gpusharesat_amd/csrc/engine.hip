@@ -339,7 +339,7 @@ struct gss_engine {
     float last_ms = 0.f;
     double total_ms = 0.0;
     int64_t nsweeps = 0;
-    int occ[7][9][2] = {};  // [log2 words][unroll][fused] resident blocks per CU
+    int occ[7][9][2][2] = {};  // [log2 words][unroll][fused][prefilter] resident blocks per CU
 
     // tuning
     int tune_waves_per_cu = 0;
@@ -375,6 +375,13 @@ struct gss_engine {
     DevBuf<uint64_t> serve_tail;
     DevBuf<ServeAdmitted> serve_adm;
     DevBuf<uint32_t> serve_cnt;
+    // serving steps: aggregate rows ahead of the exact check (SweepArgs::agg);
+    // GPUSHARESAT_SERVE_PREFILTER=0 leaves them out (same records)
+    bool serve_prefilter = true;
+    int serve_vote = 2;  // aggregate entries per vote (GPUSHARESAT_SERVE_VOTE: 1, 2, 4 or 8)
+    DevBuf<uint8_t> serve_agg;         // 2 * vars + 2 bytes, padded to 4
+    DevBuf<uint64_t> serve_agg_words;  // [8]
+    DevBuf<gss_var_patch> serve_dpatches;  // the step's admitted patches (ring capacity)
 
     // batched steps (gss_step_*)
     StepSlot slots[2];
@@ -405,13 +412,14 @@ int fuse_of(const gss_engine *e) {
 }
 
 // Resident 256-thread blocks per CU of the sweep variant the engine runs.
-int resident_blocks(gss_engine *e) {
+int resident_blocks(gss_engine *e, bool agg = false) {
     int l = 0;
     while ((1 << l) < e->words) l++;
     const int ui = std::min(e->tune_unroll, 8);
     const int fi = fuse_of(e) > 0;
-    if (!e->occ[l][ui][fi]) e->occ[l][ui][fi] = sweep_occupancy(e->words, e->tune_unroll, fi);
-    return e->occ[l][ui][fi];
+    int &o = e->occ[l][ui][fi][agg];
+    if (!o) o = sweep_occupancy(e->words, e->tune_unroll, fi, agg);
+    return o;
 }
 
 int log2_exact(int w) {
@@ -529,6 +537,12 @@ int gss_engine_create(int device, gss_engine **out) {
         e->db_limit = std::max<long long>(0, std::atoll(lim));
     if (const char *st = std::getenv("GPUSHARESAT_STAGE_THREADS"))
         e->stage_threads = std::min(64, std::max(0, std::atoi(st)));
+    if (const char *pf = std::getenv("GPUSHARESAT_SERVE_PREFILTER"))
+        e->serve_prefilter = std::atoi(pf) != 0;
+    if (const char *sv = std::getenv("GPUSHARESAT_SERVE_VOTE")) {
+        const int v = std::atoi(sv);
+        if (v == 1 || v == 2 || v == 4 || v == 8) e->serve_vote = v;
+    }
     int rc = GSS_OK;
     auto fail = [&](int code) {
         gss_engine_destroy(e);
@@ -610,6 +624,9 @@ int gss_engine_destroy(gss_engine *e) {
     e->serve_tail.release();
     e->serve_adm.release();
     e->serve_cnt.release();
+    e->serve_agg.release();
+    e->serve_agg_words.release();
+    e->serve_dpatches.release();
     if (e->d_valid) (void)hipFree(e->d_valid);
     if (e->d_alloc) (void)hipFree(e->d_alloc);
     if (e->d_lastpos) (void)hipFree(e->d_lastpos);
@@ -1492,8 +1509,8 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
 // Persistent grid of a step: enough 4-wave blocks for the tasks, rounded up
 // to a power of two (so a growing DB re-captures the graph rarely), at most
 // the resident maximum.
-static int step_blocks(gss_engine *e, int64_t tasks) {
-    int per_cu = resident_blocks(e);
+static int step_blocks(gss_engine *e, int64_t tasks, bool agg) {
+    int per_cu = resident_blocks(e, agg);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
     const int64_t full = (int64_t)e->cu * per_cu;
     int64_t b = 1;
@@ -1587,6 +1604,10 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     a.fuse_len = fuse;
     a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
     a.rec_words = sl.rec_words;
+    uint8_t *agg = serve && e->serve_prefilter ? e->serve_agg.p : nullptr;
+    a.agg = agg;
+    a.agg_words = agg ? e->serve_agg_words.p : nullptr;
+    a.agg_vote = e->serve_vote;
     hipError_t he = hipSuccess;
     if (he == hipSuccess) he = launch_stage_in(st, cs);
     if (he == hipSuccess)
@@ -1597,12 +1618,16 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     // device, when the step starts (their loads, valid slots and counts)
     if (he == hipSuccess && serve)
         he = launch_serve_admit(e->serve, dh, sl.d_loads.p, sl.h_report.p, e->words, cs);
+    if (he == hipSuccess && serve) he = launch_serve_stage_patches(e->serve, dh, cs);
     if (he == hipSuccess && sl.g_bases)
         he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
                                    e->table.p, e->words, e->vars, cs);
+    if (he == hipSuccess && agg)
+        he = launch_serve_agg(e->serve, dh, sl.d_loads.p, e->bases.p, e->bases_vw, e->vars, agg,
+                              e->serve_agg_words.p, cs);
     if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
     if (he == hipSuccess && sl.g_patches)
-        he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, cs)
+        he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, agg, cs)
                    : launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words,
                                             cs);
     if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
@@ -1683,7 +1708,7 @@ int gss::engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_c
         return set_err(GSS_E_INVAL, "bad serving ring");
     if (!e->inflight.empty()) return set_err(GSS_E_STATE, "steps in flight");
     if ((rc = e->serve_tail.reserve(1)) || (rc = e->serve_adm.reserve(kServeMaxRequests)) ||
-        (rc = e->serve_cnt.reserve(1)))
+        (rc = e->serve_cnt.reserve(1)) || (rc = e->serve_dpatches.reserve(patch_cap)))
         return rc;
     HIP_TRY(launch_fill_u64(e->serve_tail.p, 1, 0ull, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -1694,6 +1719,7 @@ int gss::engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_c
     r.tail = e->serve_tail.p;
     r.adm = e->serve_adm.p;
     r.adm_count = e->serve_cnt.p;
+    r.dpatches = e->serve_dpatches.p;
     r.req_cap = req_cap;
     r.patch_cap = patch_cap;
     e->serve_on = true;
@@ -1726,8 +1752,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     if (rc) return rc;
     if (!desc) return set_err(GSS_E_INVAL, "null step");
     if (serve && !e->serve_on) return set_err(GSS_E_STATE, "no serving ring attached");
-    if (serve && (desc->n_loads || desc->n_patches))
-        return set_err(GSS_E_INVAL, "a serving step takes its loads and patches from the ring");
+    if (serve && (desc->n_loads || desc->n_patches || desc->n_ups))
+        return set_err(GSS_E_INVAL, "a serving step takes its loads and patches from the ring "
+                                    "and no updates");
     const int64_t n = desc->n_clauses, n_ups = desc->n_ups, n_patches = desc->n_patches,
                   n_bups = desc->n_base_ups, n_loads = desc->n_loads;
     const uint32_t *lits = desc->lits;
@@ -1903,7 +1930,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         tasks += (c + tc - 1) / tc;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
-    const int blocks = step_blocks(e, tasks);
+    const int blocks = step_blocks(e, tasks, serve && e->serve_prefilter);
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
     const uint64_t ub = (uint64_t)(e->total_clauses + n) * (uint64_t)active_words;
@@ -1924,6 +1951,10 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     if (serve && ((rc = sl.h_report.reserve(1)) || (rc = sl.h_group.reserve(GSS_MAX_WORDS + 2))))
         return rc;
     if (serve && (rc = sl.d_wcount.reserve_async(2 * GSS_MAX_WORDS, e->stream))) return rc;
+    const bool prefilter = serve && e->serve_prefilter;
+    if (prefilter && ((rc = e->serve_agg.reserve_async((size_t)((2 * e->vars + 2 + 3) / 4 * 4), es)) ||
+                      (rc = e->serve_agg_words.reserve_async(8, es))))
+        return rc;
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
         (rc = sl.d_cls.reserve_async(sl.h_cls.cap, es)) ||
         (rc = sl.d_ups.reserve_async(sl.h_ups.cap, es)) ||
@@ -1970,7 +2001,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (const void *)(intptr_t)tc, (const void *)(intptr_t)fuse,
         (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words,
         (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr,
-        sl.h_group.p, sl.d_wcount.p, (const void *)(intptr_t)(sl.g_bases + 2 * sl.g_patches)};
+        sl.h_group.p, sl.d_wcount.p, (const void *)(intptr_t)(sl.g_bases + 2 * sl.g_patches),
+        prefilter ? (const void *)e->serve_agg.p : nullptr,
+        prefilter ? (const void *)e->serve_agg_words.p : nullptr};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
         if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve))) return rc;

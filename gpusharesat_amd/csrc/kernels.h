@@ -37,6 +37,17 @@ struct SweepArgs {
     uint32_t sentinel;       // stored row offset of the sentinel literal
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
     int rec_words;           // 8-byte words per record: 3 (gss_hit) or 1 (gss_hit_id)
+    // Serving steps (null otherwise): bit b of agg[l] is set only if literal
+    // l is non-false in every valid slot of every request of group b (the
+    // step's admitted requests r with r % 8 = b); 2 * vars + 2 bytes (padded
+    // to 4), agg[2 * vars] = 0 for the sentinel.  agg_words[b] = the table
+    // words of group b's requests.  In a group where two literal occurrences
+    // of a clause are so marked no slot can report, so the exact check reads
+    // table rows only for the clause's other words.  4 MB at 2M variables: the
+    // votes are served from L2.
+    const uint8_t *agg;
+    const uint64_t *agg_words;  // [8]
+    int agg_vote;               // aggregate entries read together per vote: 1, 2, 4 or 8
 };
 
 // Staged regions of a batched step, copied host -> device by k_stage_in.
@@ -120,6 +131,8 @@ struct ServeRing {
     uint64_t *tail;                // device: next request to admit
     ServeAdmitted *adm;            // device: [kServeMaxRequests]
     uint32_t *adm_count;           // device
+    gss_var_patch *dpatches;       // device: the step's admitted patches, in admission order
+                                   // (patch_cap entries)
     uint32_t req_cap;
     uint32_t pad;
     uint64_t patch_cap;
@@ -133,10 +146,20 @@ constexpr int kServeMaxRequests = 64;  // one admission wave: at most 64 request
 // the report.  One wave.
 hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load *loads,
                               ServeReport *report, int words, hipStream_t s);
-// Applies the admitted requests' patches, read from the pinned ring, with
-// their words shifted to where each request landed.
+// Copies the admitted requests' patches from the pinned ring to r.dpatches
+// (consecutive lanes read consecutive 4-byte words: one pass over PCIe).
+hipError_t launch_serve_stage_patches(const ServeRing &r, const StepHeader *dh, hipStream_t s);
+// Builds the aggregate entries of a serving step (see SweepArgs::agg) and
+// agg_words from the solver bases its admitted requests were loaded from:
+// one pass over the variables, after the admission and the base updates.
+hipError_t launch_serve_agg(const ServeRing &r, const StepHeader *dh, const gss_word_load *loads,
+                            const uint64_t *bases, int64_t vw, int64_t vars, uint8_t *agg,
+                            uint64_t *agg_words, hipStream_t s);
+// Applies the admitted requests' patches (staged in r.dpatches) with their
+// words shifted to where each request landed; with agg, clears the group bit
+// of every literal a patch makes false in a valid slot.
 hipError_t launch_serve_patch(const ServeRing &r, const StepHeader *dh, uint64_t *table,
-                              int words, hipStream_t s);
+                              int words, uint8_t *agg, hipStream_t s);
 
 // One non-empty bucket of a tagged DB during compaction.
 struct CompactRef {
@@ -154,8 +177,8 @@ struct CompactRef {
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
 // Max resident 256-thread blocks per CU for the sweep kernel variant (with or
-// without the fused schedule).
-int sweep_occupancy(int words, int unroll, bool fused);
+// without the fused schedule, with or without the serving prefilter).
+int sweep_occupancy(int words, int unroll, bool fused, bool agg = false);
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, int shift,

@@ -333,6 +333,33 @@ __device__ __forceinline__ void check_clause(const uint32_t *__restrict__ cl, in
     for (int k = 0; k < K; k++) um[k] = cand[k] & cs.nt[k];
 }
 
+// Serving steps, one clause per lane: the request groups in which fewer
+// than two literal occurrences of the clause (its row of len stored
+// offsets) are non-false in every valid slot (SweepArgs::agg), i.e. the only
+// groups where it can report; 0 once none is left.  `vote` literals and then
+// their entries are read together per vote (1, 2, 4 or 8), so a wave keeps
+// up to 64 * vote loads in flight.
+__device__ __forceinline__ uint32_t lane_agg(const uint32_t *__restrict__ row, int len,
+                                             const uint8_t *__restrict__ agg, int vote,
+                                             uint32_t gdead) {
+    constexpr int kMaxVote = 8;
+    uint32_t a1 = 0, a2 = gdead;  // groups with >= 1 / >= 2 such occurrences (empty: dead)
+    for (int j = 0; j < len; j += vote) {
+        uint32_t l[kMaxVote], x[kMaxVote];
+#pragma unroll
+        for (int u = 0; u < kMaxVote; u++) l[u] = (u < vote && j + u < len) ? row[j + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < kMaxVote; u++) x[u] = (u < vote && j + u < len) ? agg[l[u] >> 7] : 0u;
+#pragma unroll
+        for (int u = 0; u < kMaxVote; u++) {
+            a2 |= a1 & x[u];
+            a1 |= x[u];
+        }
+        if ((a2 & 0xFFu) == 0xFFu) return 0;
+    }
+    return ~a2 & 0xFFu;
+}
+
 __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int nb, int task) {
     int lo = 0, hi = nb;
     while (hi - lo > 1) {
@@ -343,7 +370,7 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
     return lo;
 }
 
-template <int W, int S, int U, bool FUSE>
+template <int W, int S, int U, bool FUSE, bool AGG>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     constexpr int K = W > 16 ? W / 16 : 1;  // table words per lane
     constexpr int LW = W / K;               // lanes across one row
@@ -360,6 +387,10 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     uint64_t valid[K];
 #pragma unroll
     for (int k = 0; k < K; k++) valid[k] = a.valid[lw * K + k];
+    uint32_t gdead = 0xFFu;  // serving steps: request groups without a word
+    if constexpr (AGG)
+        for (int b = 0; b < 8; b++)
+            if (a.agg_words[b]) gdead &= ~(1u << b);
     // wave id and count are wave-uniform: keep the task walk scalar
     const int gw = uniform((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nw = (int)((gridDim.x * blockDim.x) >> 6);
@@ -371,6 +402,42 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     }
     int chunk = -1;
     int fill = 0;
+    // Appends this pass's records (all lanes call it; `active` marks the
+    // groups that checked a clause).
+    auto emit = [&](bool active, int64_t row, int len, const uint64_t (&fm)[K],
+                    const uint64_t (&um)[K]) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const bool hit = active && sub == 0 && (fm[k] | um[k]) != 0;
+            const unsigned long long bal = __ballot(hit);
+            if (bal) {
+                const int n = __popcll(bal);
+                if (chunk < 0 || fill + n > kHitChunk) {
+                    if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
+                        a.fills[chunk] = (uint32_t)fill;
+                    uint32_t c = 0;
+                    if (lane == 0) c = atomicAdd(a.alloc, 1u);
+                    chunk = uniform((int)__shfl(c, 0));
+                    fill = 0;
+                }
+                if (hit && (uint32_t)chunk < a.chunk_cap) {
+                    // Records are streamed out and never read by the kernel:
+                    // non-temporal stores keep them from displacing table
+                    // rows in L2 and the memory-side cache.
+                    const uint64_t id = (uint64_t)(uint32_t)row | ((uint64_t)len << 32) |
+                                        ((uint64_t)(lw * K + k) << 48);
+                    const size_t r = (size_t)chunk * kHitChunk + fill + prefix_count(bal);
+                    uint64_t *dst = (uint64_t *)a.hits + r * (size_t)a.rec_words;
+                    __builtin_nontemporal_store(id, dst);
+                    if (a.rec_words == 3) {
+                        __builtin_nontemporal_store(fm[k], dst + 1);
+                        __builtin_nontemporal_store(um[k], dst + 2);
+                    }
+                }
+                fill += n;
+            }
+        }
+    };
     for (int task = gw; task < ntasks; task += nw) {
         const int bi = uniform(find_bucket(a.buckets, nbuckets, task));
         const BucketDesc bd = a.buckets[bi];
@@ -379,54 +446,63 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
         const int64_t rem = (int64_t)uniform((int)bd.count) - row0;
         const int nrows = rem < a.task_clauses ? (int)rem : a.task_clauses;
         const uint32_t *rows_base = bd.lits + (size_t)row0 * len;
-        // literal q of the group's first clause chunk, prefetched one pass ahead
-        uint32_t next_chunk =
-            (g < nrows && q < len) ? rows_base[(size_t)g * len + q] : a.sentinel;
-        for (int it = 0; it < nrows; it += CPW) {
-            const bool active = it + g < nrows;
-            const int64_t row = row0 + it + g;
-            const uint32_t chunk0 = next_chunk;
-            const int nxt = it + CPW + g;
-            next_chunk = (nxt < nrows && q < len) ? rows_base[(size_t)nxt * len + q] : a.sentinel;
-            uint64_t fm[K], um[K];
+        if constexpr (AGG) {
+            // Serving steps: each lane screens one clause against the
+            // aggregate entries; the groups then check the survivors exactly,
+            // CPW at a time, on the words of their surviving request groups.
+            for (int it = 0; it < nrows; it += 64) {
+                const int c = it + lane;
+                const uint32_t grp =
+                    c < nrows ? lane_agg(rows_base + (size_t)c * len, len, a.agg, a.agg_vote, gdead)
+                              : 0u;
+                unsigned long long left = __ballot(grp != 0);
+                while (left) {
+                    unsigned long long m = left;
+                    for (int k = 0; k < g && m; k++) m &= m - 1;  // group g: the g-th survivor
+                    const int src = m ? (int)__builtin_ctzll(m) : 0;
+                    const bool active = m != 0;
+                    for (int k = 0; k < CPW && left; k++) left &= left - 1;
+                    const uint32_t gr = (uint32_t)__shfl((int)grp, src);  // the survivor's lane
+                    const int64_t row = row0 + it + src;
+                    uint64_t fm[K], um[K];
 #pragma unroll
-            for (int k = 0; k < K; k++) fm[k] = um[k] = 0;
-            if (active) {
-                check_clause<W, S, U, K, FUSE>(bd.lits + (size_t)row * len, len, chunk0,
-                                               a.sentinel,
-                                         a.table, q, gmask, gbase, w8, sub, valid, a.fuse_len,
-                                         fm, um);
-            }
+                    for (int k = 0; k < K; k++) fm[k] = um[k] = 0;
+                    if (active) {
+                        uint64_t alive = 0;
+                        for (int b = 0; b < 8; b++)
+                            if ((gr >> b) & 1u) alive |= a.agg_words[b];
+                        uint64_t vk[K];
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const bool hit = active && sub == 0 && (fm[k] | um[k]) != 0;
-                const unsigned long long bal = __ballot(hit);
-                if (bal) {
-                    const int n = __popcll(bal);
-                    if (chunk < 0 || fill + n > kHitChunk) {
-                        if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
-                            a.fills[chunk] = (uint32_t)fill;
-                        uint32_t c = 0;
-                        if (lane == 0) c = atomicAdd(a.alloc, 1u);
-                        chunk = uniform((int)__shfl(c, 0));
-                        fill = 0;
+                        for (int k = 0; k < K; k++)
+                            vk[k] = ((alive >> (lw * K + k)) & 1ull) ? valid[k] : 0ull;
+                        const uint32_t *cl = bd.lits + (size_t)row * len;
+                        const uint32_t chunk0 = q < len ? cl[q] : a.sentinel;
+                        check_clause<W, S, U, K, FUSE>(cl, len, chunk0, a.sentinel, a.table, q,
+                                                       gmask, gbase, w8, sub, vk, a.fuse_len, fm,
+                                                       um);
                     }
-                    if (hit && (uint32_t)chunk < a.chunk_cap) {
-                        // Records are streamed out and never read by the kernel:
-                        // non-temporal stores keep them from displacing table
-                        // rows in L2 and the memory-side cache.
-                        const uint64_t id = (uint64_t)(uint32_t)row | ((uint64_t)len << 32) |
-                                            ((uint64_t)(lw * K + k) << 48);
-                        const size_t r = (size_t)chunk * kHitChunk + fill + prefix_count(bal);
-                        uint64_t *dst = (uint64_t *)a.hits + r * (size_t)a.rec_words;
-                        __builtin_nontemporal_store(id, dst);
-                        if (a.rec_words == 3) {
-                            __builtin_nontemporal_store(fm[k], dst + 1);
-                            __builtin_nontemporal_store(um[k], dst + 2);
-                        }
-                    }
-                    fill += n;
+                    emit(active, row, len, fm, um);
                 }
+            }
+        } else {
+            // literal q of the group's first clause chunk, prefetched one pass ahead
+            uint32_t next_chunk =
+                (g < nrows && q < len) ? rows_base[(size_t)g * len + q] : a.sentinel;
+            for (int it = 0; it < nrows; it += CPW) {
+                const bool active = it + g < nrows;
+                const int64_t row = row0 + it + g;
+                const uint32_t chunk0 = next_chunk;
+                const int nxt = it + CPW + g;
+                next_chunk =
+                    (nxt < nrows && q < len) ? rows_base[(size_t)nxt * len + q] : a.sentinel;
+                uint64_t fm[K], um[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) fm[k] = um[k] = 0;
+                if (active)
+                    check_clause<W, S, U, K, FUSE>(bd.lits + (size_t)row * len, len, chunk0,
+                                                   a.sentinel, a.table, q, gmask, gbase, w8, sub,
+                                                   valid, a.fuse_len, fm, um);
+                emit(active, row, len, fm, um);
             }
         }
     }
@@ -836,27 +912,101 @@ __global__ __launch_bounds__(64) void k_serve_admit(ServeRing r, StepHeader *dh,
     }
 }
 
-__global__ void k_serve_patch(ServeRing r, const int64_t *n_p, uint64_t *__restrict__ table,
-                              int words) {
-    const int64_t n = *n_p;
+// The admitted request (index into r.adm) of the step's i-th patch.
+__device__ __forceinline__ int admitted_of(const ServeRing &r, int cnt, int64_t i) {
+    int lo = 0, hi = cnt;  // last admitted request whose patches start at or before i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t)r.adm[mid].patch_pfx <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_serve_stage_patches(ServeRing r, const StepHeader *__restrict__ dh) {
+    constexpr int kWords = (int)(sizeof(gss_var_patch) / 4);
+    const int64_t n = dh->npatches * kWords;
+    const int cnt = (int)*r.adm_count;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = k / kWords;
+        const int f = (int)(k - i * kWords);
+        const ServeAdmitted a = r.adm[admitted_of(r, cnt, i)];
+        const uint64_t j = (uint64_t)(i - (int64_t)a.patch_pfx);
+        const uint32_t *src = (const uint32_t *)(r.patches + (a.patch_off + j) % r.patch_cap);
+        ((uint32_t *)(r.dpatches + i))[f] = src[f];
+    }
+}
+
+// Aggregate entries of a serving step: bit b of agg[2v] (agg[2v + 1]) is set
+// if literal v (~v) is not false in the bases of all of group b's requests
+// (r % 8 = b), or if the group is empty.  A request's words are loaded whole
+// from its solver's base, so until its patches (k_serve_patch clears what
+// they make false) the base value holds in all its slots.
+__global__ void k_serve_agg(ServeRing r, const StepHeader *__restrict__ dh,
+                            const gss_word_load *__restrict__ loads,
+                            const uint64_t *__restrict__ bases, int64_t vw, int64_t vars,
+                            uint8_t *__restrict__ agg, uint64_t *__restrict__ agg_words) {
+    __shared__ uint32_t s_base[kServeMaxRequests];
+    const int cnt = (int)*r.adm_count;
+    if ((int)threadIdx.x < cnt) s_base[threadIdx.x] = loads[r.adm[threadIdx.x].w0].base;
+    if (blockIdx.x == 0 && threadIdx.x < 8) {
+        const int64_t nl = dh->nloads;
+        uint64_t m = 0;
+        for (int q = (int)threadIdx.x; q < cnt; q += 8) {
+            const uint32_t w0 = r.adm[q].w0;
+            const uint32_t w1 = q + 1 < cnt ? r.adm[q + 1].w0 : (uint32_t)nl;
+            for (uint32_t w = w0; w < w1 && w < 64u; w++) m |= 1ull << w;
+        }
+        agg_words[threadIdx.x] = m;
+    }
+    __syncthreads();
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v <= vars;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t pos = 0xFFu, neg = 0xFFu;
+        if (v < vars) {
+            const uint64_t bit = 1ull << (v & 63);
+            const int64_t i = v >> 6;
+            for (int q = 0; q < cnt; q++) {
+                const uint64_t *def = bases + (size_t)(2 * s_base[q]) * vw;
+                const bool d = (def[i] & bit) != 0;
+                const bool t = (def[vw + i] & bit) != 0;
+                const uint32_t gb = 1u << (q & 7);
+                if (d && !t) pos &= ~gb;  // v is false in this base
+                if (d && t) neg &= ~gb;   // ~v is false in this base
+            }
+        } else {  // sentinel pair: the padding entry contributes nothing
+            pos = 0u;
+        }
+        ((uint16_t *)agg)[v] = (uint16_t)(pos | (neg << 8));
+    }
+}
+
+__global__ void k_serve_patch(ServeRing r, const StepHeader *__restrict__ dh,
+                              uint64_t *__restrict__ table, int words,
+                              uint32_t *__restrict__ agg32) {
+    const int64_t n = dh->npatches;
     const int cnt = (int)*r.adm_count;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int lo = 0, hi = cnt;  // last admitted request whose patches start at or before i
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if ((int64_t)r.adm[mid].patch_pfx <= i) lo = mid;
-            else hi = mid;
-        }
-        const ServeAdmitted a = r.adm[lo];
-        const uint64_t j = (uint64_t)(i - (int64_t)a.patch_pfx);
-        const volatile gss_var_patch *pp = r.patches + (a.patch_off + j) % r.patch_cap;
-        const uint32_t var = pp->var;
-        const uint32_t word = pp->word + a.w0;
-        const uint64_t m = pp->mask, d = pp->is_def, t = pp->is_true;
+        const int q = admitted_of(r, cnt, i);
+        const uint32_t w0 = r.adm[q].w0;
+        const gss_var_patch pp = r.dpatches[i];
+        const uint32_t var = pp.var;
+        const uint32_t word = pp.word + w0;
+        const uint64_t m = pp.mask, d = pp.is_def, t = pp.is_true;
         const size_t base = (size_t)var * 2 * words + word;
         table[base] = (table[base] & ~m) | (~(d & ~t) & m);
         table[base + words] = (table[base + words] & ~m) | (~(d & t) & m);
+        if (agg32 && word < 64u) {
+            const uint64_t vm = m & dh->valid[word];
+            const uint32_t gb = 1u << (q & 7);
+            // entries 2 var and 2 var + 1 are bytes 2 (var & 1) and +1 of
+            // 4-byte word var / 2
+            const uint32_t sh = (var & 1u) * 16u;
+            if (d & ~t & vm) atomicAnd(&agg32[var >> 1], ~(gb << sh));
+            if (d & t & vm) atomicAnd(&agg32[var >> 1], ~(gb << (sh + 8u)));
+        }
     }
 }
 
@@ -914,26 +1064,38 @@ __global__ void k_compact_move(const CompactRef *__restrict__ refs, int nb, int6
 
 template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
-    if (a.fuse_len > 0)
-        hipLaunchKernelGGL((k_sweep<W, S, U, true>), dim3(blocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_sweep<W, S, U, false>), dim3(blocks), dim3(256), 0, s, a);
+    const dim3 g(blocks), b(256);
+    if (a.agg) {
+        if (a.fuse_len > 0) hipLaunchKernelGGL((k_sweep<W, S, U, true, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_sweep<W, S, U, false, true>), g, b, 0, s, a);
+    } else {
+        if (a.fuse_len > 0) hipLaunchKernelGGL((k_sweep<W, S, U, true, false>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_sweep<W, S, U, false, false>), g, b, 0, s, a);
+    }
     return hipGetLastError();
 }
 
 template <int W, int S, int U>
-int occupancy_t(bool fused) {
+int occupancy_t(bool fused, bool agg) {
     int nb = 0;
-    const hipError_t e =
-        fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, true>, 256, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, false>, 256,
-                                                             0);
+    hipError_t e;
+    if (agg)
+        e = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, true, true>,
+                                                                 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, k_sweep<W, S, U, false, true>, 256, 0);
+    else
+        e = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, k_sweep<W, S, U, true, false>, 256, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, k_sweep<W, S, U, false, false>, 256, 0);
     if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 
 // Kernel variants: (W, S, U), each with and without the fused schedule
-// (fuse_len <= 0 selects the one without).  `unroll` selects U for W >= 16
+// (fuse_len <= 0 selects the one without), and with and without the
+// aggregate prefilter of serving steps (SweepArgs::agg set).  `unroll` selects U for W >= 16
 // (0 = default).
 #define GSS_SWEEP_VARIANTS(X)                  \
     X(1, 16, 1)                                \
@@ -984,10 +1146,10 @@ hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, h
     return hipErrorInvalidValue;
 }
 
-int sweep_occupancy(int words, int unroll, bool fused) {
+int sweep_occupancy(int words, int unroll, bool fused, bool agg) {
     const Variant v = pick(words, unroll);
 #define X(W_, S_, U_) \
-    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>(fused);
+    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>(fused, agg);
     GSS_SWEEP_VARIANTS(X)
 #undef X
     return 1;
@@ -1114,10 +1276,23 @@ hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load 
     return hipGetLastError();
 }
 
+hipError_t launch_serve_stage_patches(const ServeRing &r, const StepHeader *dh, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_stage_patches, dim3(1024), dim3(256), 0, s, r, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_agg(const ServeRing &r, const StepHeader *dh, const gss_word_load *loads,
+                            const uint64_t *bases, int64_t vw, int64_t vars, uint8_t *agg,
+                            uint64_t *agg_words, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_agg, dim3(2048), dim3(256), 0, s, r, dh, loads, bases, vw, vars,
+                       agg, agg_words);
+    return hipGetLastError();
+}
+
 hipError_t launch_serve_patch(const ServeRing &r, const StepHeader *dh, uint64_t *table,
-                              int words, hipStream_t s) {
-    hipLaunchKernelGGL(k_serve_patch, dim3(1024), dim3(256), 0, s, r, &dh->npatches, table,
-                       words);
+                              int words, uint8_t *agg, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_patch, dim3(1024), dim3(256), 0, s, r, dh, table, words,
+                       (uint32_t *)agg);
     return hipGetLastError();
 }
 

@@ -180,6 +180,8 @@ class ServeResult:
     device_ms: float = 0.0
     arrive_s: np.ndarray = None     # arrival time of the request of each latency
     kept: dict = field(default_factory=dict)  # request id -> (result, records), sampled
+    # (seconds, steps, device ms, clauses) of the server, about every 5 s
+    samples: list = field(default_factory=list)
 
     @property
     def checks_per_s(self) -> float:
@@ -441,6 +443,34 @@ class Server:
         u["is_true"] = d & ((rng.random(c.drift) < agree) == self.sigma[u["var"]])
         return u
 
+    def preload(self, n: int, chunk: int = 1_000_000) -> int:
+        """Adds n clauses of the workload's clause stream before the run (in
+        chunks, each appended by the step of one one-assignment request);
+        returns the clauses in the DB."""
+        left = n
+        while left > 0:
+            k = min(left, chunk)
+            added = 0
+            while added < k:
+                lits, lens, lo, hi = self._clauses(k - added)
+                first = self.srv.add_clauses(lits, lens)
+                if first < 0:  # the budget is full
+                    return self.srv.stats()["clauses"]
+                if self.keep_history:
+                    self.clause_ids.append((first, lo, hi))
+                added += hi - lo
+            left -= k
+            rid = self.srv.submit(0, 1, np.zeros(0, PATCH_DTYPE))
+            if rid is None:
+                raise RuntimeError("the request ring is full")
+            done = False
+            while not done:
+                for i in self.srv.poll(16, 1_000_000):
+                    self.srv.result(i, records=False)
+                    self.srv.release(i)
+                    done |= i == rid
+        return self.srv.stats()["clauses"]
+
     def request_patches(self, r: Request) -> np.ndarray:
         nw = (r.k + 63) // 64
         return self.pool[r.payload][:self.pool_off[r.payload][nw]]
@@ -461,6 +491,8 @@ class Server:
         last_cl = 0.0
         last_log = 0.0
         backlog: list[Request] = []
+        st = srv.stats()
+        samples = [(0.0, st["steps"], st["device_ms"], st["clauses"])]
         start = time.perf_counter()
 
         def collect(ids):
@@ -504,14 +536,20 @@ class Server:
             if now - last_log > 5.0:
                 last_log = now
                 st = srv.stats()
+                samples.append((now, st["steps"], st["device_ms"], st["clauses"]))
+                dms = 0.0
+                if len(samples) > 1 and samples[-1][1] > samples[-2][1]:
+                    dms = (samples[-1][2] - samples[-2][2]) / (samples[-1][1] - samples[-2][1])
                 self.log(f"t={now:.1f}s steps {st['steps']} served {len(lat)} queue "
-                         f"{len(by_id) + len(backlog)} DB {st['clauses']} clauses")
+                         f"{len(by_id) + len(backlog)} DB {st['clauses']} clauses, "
+                         f"{dms:.3f} device ms per step")
         elapsed = time.perf_counter() - start
         st = srv.stats()
         return ServeResult(len(lat), st["steps"] - st0["steps"], elapsed,
                            st["checks"] - st0["checks"], np.array(lat), st["clauses"],
                            st["literals"], st["records"] - st0["records"], max_queue,
-                           st["device_ms"] - st0["device_ms"], np.array(arrive), kept)
+                           st["device_ms"] - st0["device_ms"], np.array(arrive), kept,
+                           samples + [(elapsed, st["steps"], st["device_ms"], st["clauses"])])
 
     def close(self):
         self.srv.close()

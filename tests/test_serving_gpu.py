@@ -220,11 +220,16 @@ def _same(got, exp, what):
         assert np.array_equal(got[f], exp[f]), (what, f)
 
 
-def test_native_loop_request_alone_equals_in_mix_and_oracle():
+@pytest.mark.parametrize("prefilter", ["1", "0"])
+def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilter):
     """Through the C ABI: request R alone, then copies of R inside a burst
     of other requests (admitted together on the device, at other word
     offsets): every copy's records equal R alone, and the oracle's on R's
-    explicit assignments."""
+    explicit assignments.  With and without the serving steps' aggregate
+    prefilter (GPUSHARESAT_SERVE_PREFILTER, read when the server's engine is
+    created); the patches make literals false in many words, so the
+    prefilter's rows are cleared by patches as well as set from the bases."""
+    monkeypatch.setenv("GPUSHARESAT_SERVE_PREFILTER", prefilter)
     rng = np.random.default_rng(21)
     V, W, N = 500, 16, 4000
     lens = rng.integers(1, 13, size=N).astype(np.int32)
