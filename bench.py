@@ -177,6 +177,8 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--preload", type=int, default=0,
                     help="serving-32: clauses in the DB before the run (default none)")
+    ap.add_argument("--flips", type=int, default=32,
+                    help="serving-32: variables each assignment redraws from its solver's base")
     ap.add_argument("--records", default="full", choices=["full", "ids"],
                     help="batched-16, serving-32: records with masks (full) or clause and word "
                          "only (ids, what a solver receives)")
@@ -546,7 +548,7 @@ def run_serving(args) -> int:
     from gpusharesat_amd import serving as sv
     cfg = sv.ServeConfig(duration=args.duration, rate=args.rate, max_inflight=args.inflight,
                          records=sv.RECORDS_IDS if args.records == "ids" else sv.RECORDS_FULL,
-                         planted=args.signs == "planted")
+                         planted=args.signs == "planted", flips=args.flips)
     log(f"serving-32: {cfg}")
     srv = sv.Server(cfg, log=log)
     try:
@@ -596,7 +598,7 @@ def run_serving(args) -> int:
                    "steps_in_flight": cfg.max_inflight, "loop": "native (gss_server)",
                    "prefilter": os.environ.get("GPUSHARESAT_SERVE_PREFILTER", "1") != "0",
                    "prefilter_vote": int(os.environ.get("GPUSHARESAT_SERVE_VOTE", "2")),
-                   "preloaded_clauses": preloaded},
+                   "preloaded_clauses": preloaded, "flips": cfg.flips},
         "latency_ms": {"p50": pct[50], "p99": pct[99], "p999": pct[99.9],
                        "max": float(lat.max()),
                        "what": "arrival -> the request's records handed to it on the host"},
