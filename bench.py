@@ -177,6 +177,9 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=2000.0, help="serving-32: requests per second")
     ap.add_argument("--preload", type=int, default=0,
                     help="serving-32: clauses in the DB before the run (default none)")
+    ap.add_argument("--screen", action="store_true",
+                    help="batched-16: steps submitted with GSS_STEP_SCREEN (clauses screened "
+                         "against the table's aggregate before the exact check)")
     ap.add_argument("--flips", type=int, default=32,
                     help="serving-32: variables each assignment redraws from its solver's base")
     ap.add_argument("--records", default="full", choices=["full", "ids"],
@@ -412,7 +415,7 @@ def run_batched(args) -> int:
     verified = 0
     for k in range(args.warmup):
         st = plan[k]
-        e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
+        e.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos, screen=args.screen)
         n, _ = e.step_wait()
         got = _sorted_hits(e.step_hits())
         ref.append(st.lits, st.lens)
@@ -468,7 +471,7 @@ def run_batched(args) -> int:
         t = time.perf_counter()
         pending.append(t)
         sv.submit(e, st.valid, lits=st.lits, lens=st.lens, ups=st.ups, last_pos=st.last_pos,
-                  records=rec)
+                  records=rec, flags=sv.STEP_SCREEN if args.screen else 0)
         submit_s.append(time.perf_counter() - t)
         lits_db += int(st.lits.size)
         checks += lits_db * st.assignments
@@ -516,6 +519,7 @@ def run_batched(args) -> int:
             "clauses_end": int(clauses_end), "literals_end": int(lits_db),
             "hit_records": int(hits_total), "tuned": tuned, "records": args.records,
             "verified_steps": verified, "steps_in_flight": args.inflight,
+            "screen": args.screen,
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),

@@ -375,10 +375,12 @@ struct gss_engine {
     DevBuf<uint64_t> serve_tail;
     DevBuf<ServeAdmitted> serve_adm;
     DevBuf<uint32_t> serve_cnt;
-    // serving steps: aggregate rows ahead of the exact check (SweepArgs::agg);
-    // GPUSHARESAT_SERVE_PREFILTER=0 leaves them out (same records)
+    // aggregate entries screened ahead of the exact check (SweepArgs::agg), by
+    // serving steps (from the requests' bases; GPUSHARESAT_SERVE_PREFILTER=0
+    // leaves them out, same records) and by batched steps submitted with
+    // GSS_STEP_SCREEN (from the table)
     bool serve_prefilter = true;
-    int serve_vote = 2;  // aggregate entries per vote (GPUSHARESAT_SERVE_VOTE: 1, 2, 4 or 8)
+    int serve_vote = 2;  // entries per screening vote (GPUSHARESAT_SERVE_VOTE: 1, 2, 4 or 8)
     DevBuf<uint8_t> serve_agg;         // 2 * vars + 2 bytes, padded to 4
     DevBuf<uint64_t> serve_agg_words;  // [8]
     DevBuf<gss_var_patch> serve_dpatches;  // the step's admitted patches (ring capacity)
@@ -1564,7 +1566,7 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
 
 // Captures the step's fixed launch sequence over this slot's buffers.
 static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse,
-                            bool serve) {
+                            bool serve, bool screen) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
     sl.exec = nullptr;
@@ -1604,7 +1606,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     a.fuse_len = fuse;
     a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
     a.rec_words = sl.rec_words;
-    uint8_t *agg = serve && e->serve_prefilter ? e->serve_agg.p : nullptr;
+    uint8_t *agg = (serve && e->serve_prefilter) || screen ? e->serve_agg.p : nullptr;
     a.agg = agg;
     a.agg_words = agg ? e->serve_agg_words.p : nullptr;
     a.agg_vote = e->serve_vote;
@@ -1622,7 +1624,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     if (he == hipSuccess && sl.g_bases)
         he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
                                    e->table.p, e->words, e->vars, cs);
-    if (he == hipSuccess && agg)
+    if (he == hipSuccess && agg && serve)
         he = launch_serve_agg(e->serve, dh, sl.d_loads.p, e->bases.p, e->bases_vw, e->vars, agg,
                               e->serve_agg_words.p, cs);
     if (he == hipSuccess) he = launch_assig_apply_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, cs);
@@ -1630,6 +1632,9 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
         he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, agg, cs)
                    : launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words,
                                             cs);
+    // batched steps: the aggregate of the table as the sweep will read it
+    if (he == hipSuccess && screen)
+        he = launch_table_agg(e->table.p, dh, e->vars, e->words, agg, e->serve_agg_words.p, cs);
     if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
     if (he == hipSuccess) he = launch_sweep(a, e->words, e->tune_unroll, blocks, cs);
     if (he == hipSuccess)
@@ -1769,6 +1774,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         return set_err(GSS_E_INVAL, "bad step arguments");
     if (desc->records != GSS_RECORDS_FULL && desc->records != GSS_RECORDS_IDS)
         return set_err(GSS_E_INVAL, "records must be GSS_RECORDS_FULL or GSS_RECORDS_IDS");
+    if (desc->flags & ~GSS_STEP_SCREEN) return set_err(GSS_E_INVAL, "unknown step flags");
+    // serving steps screen against their requests' bases instead
+    const bool screen = !serve && (desc->flags & GSS_STEP_SCREEN);
     if (e->pending) return set_err(GSS_E_STATE, "a plain sweep is in flight");
     if (e->tagged && n > 0) return set_err(GSS_E_STATE, "tagged DB: append with gss_db_append_tagged");
     const int S = e->next_slot;
@@ -1930,7 +1938,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         tasks += (c + tc - 1) / tc;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
-    const int blocks = step_blocks(e, tasks, serve && e->serve_prefilter);
+    const int blocks = step_blocks(e, tasks, (serve && e->serve_prefilter) || screen);
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
     const uint64_t ub = (uint64_t)(e->total_clauses + n) * (uint64_t)active_words;
@@ -1951,9 +1959,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     if (serve && ((rc = sl.h_report.reserve(1)) || (rc = sl.h_group.reserve(GSS_MAX_WORDS + 2))))
         return rc;
     if (serve && (rc = sl.d_wcount.reserve_async(2 * GSS_MAX_WORDS, e->stream))) return rc;
-    const bool prefilter = serve && e->serve_prefilter;
-    if (prefilter && ((rc = e->serve_agg.reserve_async((size_t)((2 * e->vars + 2 + 3) / 4 * 4), es)) ||
-                      (rc = e->serve_agg_words.reserve_async(8, es))))
+    const bool use_agg = (serve && e->serve_prefilter) || screen;
+    if (use_agg && ((rc = e->serve_agg.reserve_async((size_t)((2 * e->vars + 2 + 3) / 4 * 4), es)) ||
+                    (rc = e->serve_agg_words.reserve_async(8, es))))
         return rc;
     if ((rc = sl.d_hdr.reserve_async(1, es)) || (rc = sl.d_lits.reserve_async(sl.h_lits.cap, es)) ||
         (rc = sl.d_cls.reserve_async(sl.h_cls.cap, es)) ||
@@ -2002,11 +2010,11 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (const void *)(intptr_t)sl.chunk_cap, (const void *)(intptr_t)sl.rec_words,
         (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr,
         sl.h_group.p, sl.d_wcount.p, (const void *)(intptr_t)(sl.g_bases + 2 * sl.g_patches),
-        prefilter ? (const void *)e->serve_agg.p : nullptr,
-        prefilter ? (const void *)e->serve_agg_words.p : nullptr};
+        use_agg ? (const void *)e->serve_agg.p : nullptr,
+        use_agg ? (const void *)e->serve_agg_words.p : nullptr, (const void *)(intptr_t)screen};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
-        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve))) return rc;
+        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve, screen))) return rc;
         sl.key = key;
         sl.graph_dense = sl.d_dense.cap;
     }

@@ -146,6 +146,13 @@ constexpr int kServeMaxRequests = 64;  // one admission wave: at most 64 request
 // the report.  One wave.
 hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load *loads,
                               ServeReport *report, int words, hipStream_t s);
+// Aggregate entries of a batched step from the table itself (see
+// SweepArgs::agg): group b = words [b * words / 8, (b + 1) * words / 8) (one
+// word per group below 8 words); bit b of agg[l] set if row l is all ones on
+// the valid slots of every word of group b; agg_words[b] = the group's words
+// with a valid slot.  Reads the table once (valid words from the header).
+hipError_t launch_table_agg(const uint64_t *table, const StepHeader *dh, int64_t vars, int words,
+                            uint8_t *agg, uint64_t *agg_words, hipStream_t s);
 // Copies the admitted requests' patches from the pinned ring to r.dpatches
 // (consecutive lanes read consecutive 4-byte words: one pass over PCIe).
 hipError_t launch_serve_stage_patches(const ServeRing &r, const StepHeader *dh, hipStream_t s);

@@ -61,6 +61,7 @@
 // counter still counts and the host re-runs the sweep.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <hipcub/hipcub.hpp>
 #include <utility>
@@ -938,6 +939,38 @@ __global__ void k_serve_stage_patches(ServeRing r, const StepHeader *__restrict_
     }
 }
 
+template <int W>
+__global__ void k_table_agg(const uint64_t *__restrict__ table, const StepHeader *__restrict__ dh,
+                            int64_t rows, uint8_t *__restrict__ agg,
+                            uint64_t *__restrict__ agg_words) {
+    constexpr int NG = W < 8 ? W : 8;  // word groups
+    constexpr int GW = W / NG;         // words per group
+    __shared__ uint64_t s_valid[W];
+    if ((int)threadIdx.x < W) s_valid[threadIdx.x] = dh->valid[threadIdx.x];
+    __syncthreads();
+    if (blockIdx.x == 0 && (int)threadIdx.x < 8) {
+        uint64_t m = 0;
+        if ((int)threadIdx.x < NG)
+            for (int w = (int)threadIdx.x * GW; w < ((int)threadIdx.x + 1) * GW; w++)
+                if (s_valid[w]) m |= 1ull << w;
+        agg_words[threadIdx.x] = m;
+    }
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < rows;
+         l += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t *row = table + (size_t)l * W;
+        uint32_t bits = 0xFFu;
+#pragma unroll
+        for (int b = 0; b < NG; b++) {
+            bool strong = true;
+#pragma unroll
+            for (int w = b * GW; w < (b + 1) * GW; w++)
+                strong = strong && ((row[w] | ~s_valid[w]) == ~0ull);
+            if (!strong) bits &= ~(1u << b);
+        }
+        agg[l] = (uint8_t)bits;
+    }
+}
+
 // Aggregate entries of a serving step: bit b of agg[2v] (agg[2v + 1]) is set
 // if literal v (~v) is not false in the bases of all of group b's requests
 // (r % 8 = b), or if the group is empty.  A request's words are loaded whole
@@ -1274,6 +1307,27 @@ hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load 
                               ServeReport *report, int words, hipStream_t s) {
     hipLaunchKernelGGL(k_serve_admit, dim3(1), dim3(64), 0, s, r, dh, loads, report, words);
     return hipGetLastError();
+}
+
+hipError_t launch_table_agg(const uint64_t *table, const StepHeader *dh, int64_t vars, int words,
+                            uint8_t *agg, uint64_t *agg_words, hipStream_t s) {
+    const int64_t rows = 2 * vars + 2;
+    const unsigned blocks = (unsigned)std::min<int64_t>((rows + 255) / 256, 8192);
+#define GSS_TABLE_AGG(W_)                                                                   \
+    if (words == W_) {                                                                      \
+        hipLaunchKernelGGL(k_table_agg<W_>, dim3(blocks), dim3(256), 0, s, table, dh, rows, \
+                           agg, agg_words);                                                 \
+        return hipGetLastError();                                                           \
+    }
+    GSS_TABLE_AGG(1)
+    GSS_TABLE_AGG(2)
+    GSS_TABLE_AGG(4)
+    GSS_TABLE_AGG(8)
+    GSS_TABLE_AGG(16)
+    GSS_TABLE_AGG(32)
+    GSS_TABLE_AGG(64)
+#undef GSS_TABLE_AGG
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_serve_stage_patches(const ServeRing &r, const StepHeader *dh, hipStream_t s) {

@@ -267,9 +267,22 @@ class Engine:
         return dict(zip(TUNE_KEYS, list(best))), ms.value
 
     # -- batched steps --------------------------------------------------------
-    def step_enqueue(self, lits=None, lens=None, ups=None, valid=0, last_pos=None) -> np.ndarray:
+    def step_enqueue(self, lits=None, lens=None, ups=None, valid=0, last_pos=None,
+                     screen: bool = False) -> np.ndarray:
         """Enqueues one step (append, apply, sweep, roll forward) as a graph
-        replay; returns the bucket rows of the appended clauses."""
+        replay; returns the bucket rows of the appended clauses.  screen: the
+        step screens clauses against the table's aggregate first
+        (GSS_STEP_SCREEN; same records)."""
+        if screen:
+            from . import serving
+            lits = np.ascontiguousarray(np.zeros(0) if lits is None else lits, dtype=np.uint32)
+            lens = np.ascontiguousarray(np.zeros(0) if lens is None else lens, dtype=np.int32)
+            if int(lens.sum(dtype=np.int64)) != lits.size:
+                raise ValueError("sum(lens) != len(lits)")
+            rows = np.empty(lens.size, dtype=np.uint32)
+            serving.submit(self, self._valid(valid), lits=lits, lens=lens, ups=ups,
+                           last_pos=last_pos, flags=serving.STEP_SCREEN, out_rows=rows)
+            return rows
         lits = np.ascontiguousarray(np.zeros(0) if lits is None else lits, dtype=np.uint32)
         lens = np.ascontiguousarray(np.zeros(0) if lens is None else lens, dtype=np.int32)
         if int(lens.sum(dtype=np.int64)) != lits.size:

@@ -55,7 +55,7 @@ class _StepDesc(ctypes.Structure):
                 ("n_base_ups", ctypes.c_int64), ("base_ups", ctypes.c_void_p),
                 ("n_loads", ctypes.c_int64), ("loads", ctypes.c_void_p),
                 ("valid", ctypes.c_void_p), ("last_pos", ctypes.c_void_p),
-                ("records", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("records", ctypes.c_int32), ("flags", ctypes.c_int32)]
 
 
 def _p(a):
@@ -64,14 +64,19 @@ def _p(a):
 
 RECORDS_FULL = 0
 RECORDS_IDS = 1
+STEP_SCREEN = 1  # gss_step_desc.flags
 
 
 def submit(e: Engine, valid, lits=None, lens=None, patches=None, base_ups=None, loads=None,
-           ups=None, last_pos=None, records: int = RECORDS_FULL) -> None:
+           ups=None, last_pos=None, records: int = RECORDS_FULL, flags: int = 0,
+           out_rows: np.ndarray | None = None) -> None:
     """gss_step_submit with numpy arrays (kept alive by the caller until the
     call returns; the engine copies them into its pinned staging)."""
     d = _StepDesc()
     d.records = records
+    d.flags = flags
+    if out_rows is not None and out_rows.size:
+        d.out_rows = out_rows.ctypes.data
     v = np.zeros(e.words, np.uint64)
     v[:len(valid)] = valid
     arrs = {"valid": v}

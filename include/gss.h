@@ -295,10 +295,18 @@ typedef struct gss_step_desc {
                                           gss_step_hits_map / gss_step_hits_copy;
                                           GSS_RECORDS_IDS (1): gss_hit_id records, read
                                           with gss_step_ids_map */
-    int32_t reserved;                  /* 0 */
+    int32_t flags;                     /* 0, or GSS_STEP_SCREEN */
 } gss_step_desc;
 #define GSS_RECORDS_FULL 0
 #define GSS_RECORDS_IDS 1
+/* Screens every clause, one per lane, against an aggregate of the table
+ * (per literal, a bit per group of words: non-false in every valid slot of
+ * the group, from one read of the table after the step's updates) and runs
+ * the exact check only on the clauses and word groups the screening leaves.
+ * The records are the same with or without it; it pays where the slots of a
+ * word group share most of their values (consecutive assignments of one
+ * solver), and costs one read of the table per step where they do not. */
+#define GSS_STEP_SCREEN 1
 int gss_step_submit(gss_engine *e, const gss_step_desc *d);
 /* (Re)creates nbases solver base planes over the current variable count,
  * every variable undefined (2 bits per variable per base).  A table that

@@ -37,7 +37,11 @@ def assert_same(got, exp, what):
         assert np.array_equal(got[f], exp[f]), (what, f)
 
 
-def test_batched16_steps_match_plain_and_oracle():
+@pytest.mark.parametrize("screen", [False, True])
+def test_batched16_steps_match_plain_and_oracle(screen):
+    """batched-16 itself: six graph steps, each equal to the plain sequence
+    and to the oracle on the table at sweep time; with GSS_STEP_SCREEN the
+    steps screen clauses against the table's aggregate first."""
     gen = workloads.SolverStreams(seed=3, var_count=1_000_000)
     d0, t0 = gen.initial_table()
     lits, lens = gen.initial_clauses(1_000_000)
@@ -54,7 +58,8 @@ def test_batched16_steps_match_plain_and_oracle():
             st = gen.step()
             print(f"step {k}: {st.lens.size} clauses, {st.ups.size} updates, "
                   f"{st.assignments} assignments", flush=True)
-            got_rows = a.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos)
+            got_rows = a.step_enqueue(st.lits, st.lens, st.ups, st.valid, st.last_pos,
+                                      screen=screen)
             n, _ = a.step_wait()
             step_hits = a.step_hits()
             assert step_hits.size == n

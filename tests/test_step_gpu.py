@@ -23,15 +23,22 @@ pytestmark = pytest.mark.gpu
 ALL = np.uint64(0xFFFFFFFFFFFFFFFF)
 
 
-def make_step(rng, V, W, n_new, maxlen, n_ups, p_def=0.6):
+def make_step(rng, V, W, n_new, maxlen, n_ups, p_def=0.6, uniform=False):
+    """uniform: every update sets all 64 slots of its word alike, so the
+    slots of a word agree (the case the screening of batched steps prunes)."""
     lens = rng.integers(1, maxlen + 1, size=n_new).astype(np.int32)
     lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
     sel = rng.choice(V * W, size=min(n_ups, V * W), replace=False)
     ups = np.zeros(sel.size, VAR_UPDATE_DTYPE)
     ups["var"] = sel // W
     ups["word"] = sel % W
-    ups["is_def"] = workloads.bernoulli_words(rng, sel.size, 64, p_def)[:, 0]
-    ups["is_true"] = rng.integers(0, 2**63, size=sel.size, dtype=np.uint64) << np.uint64(1)
+    if uniform:
+        ups["is_def"] = np.where(rng.random(sel.size) < p_def, ALL, np.uint64(0))
+        ups["is_true"] = np.where(rng.random(sel.size) < 0.5, ALL, np.uint64(0))
+    else:
+        ups["is_def"] = workloads.bernoulli_words(rng, sel.size, 64, p_def)[:, 0]
+        ups["is_true"] = (rng.integers(0, 2**63, size=sel.size, dtype=np.uint64)
+                          << np.uint64(1))
     valid = rng.integers(0, 2**63, size=W, dtype=np.uint64) | np.uint64(1)
     valid[rng.random(W) < 0.3] = ALL
     if W > 1:
@@ -87,7 +94,8 @@ class Plain:
 
 
 def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
-              grow_at=None, eager=True, tuning=(), page_limit=None):
+              grow_at=None, eager=True, tuning=(), page_limit=None, screen=False,
+              uniform=False):
     rng = np.random.default_rng(seed)
     a, b = engine_factory(), engine_factory()
     for key, val in tuning:
@@ -126,10 +134,11 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
             a.grow_vars(V)
             b.grow_vars(V)
         nn = int(rng.integers(0, n_new + 1)) if k % 3 else n_new
-        lits, lens, ups, valid, last_pos = make_step(rng, V, W, nn, maxlen, n_ups)
+        lits, lens, ups, valid, last_pos = make_step(rng, V, W, nn, maxlen, n_ups,
+                                                     uniform=uniform)
         print(f"step {k}: {nn} clauses, {ups.size} updates, {V} vars, {W} words", flush=True)
         expected.append(plain.step(lits, lens, ups, valid, last_pos))
-        rows = a.step_enqueue(lits, lens, ups, valid, last_pos)
+        rows = a.step_enqueue(lits, lens, ups, valid, last_pos, screen=screen)
         lens_a = np.concatenate([lens_a, lens])
         rows_a = np.concatenate([rows_a, rows])
         if not pipelined:
@@ -154,6 +163,18 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
 def test_steps_match_plain_and_oracle(engine_factory, W):
     run_steps(engine_factory, seed=300 + W, V=2000, W=W, nsteps=8, n_new=600, maxlen=30,
               n_ups=900, pipelined=False)
+
+
+@pytest.mark.parametrize("uniform", [False, True])
+@pytest.mark.parametrize("W", [1, 4, 16, 64])
+def test_screened_steps_match_plain_and_oracle(engine_factory, W, uniform):
+    """GSS_STEP_SCREEN: the same records as the plain sequence and the
+    oracle, both where the screening keeps nearly every clause (slots drawn
+    independently) and where it drops most (slots of a word alike), pipelined
+    and across steps whose slot masks change."""
+    run_steps(engine_factory, seed=1500 + W + 7 * uniform, V=2000, W=W, nsteps=8, n_new=600,
+              maxlen=30, n_ups=900 if not uniform else 2000 * W, pipelined=True, screen=True,
+              uniform=uniform)
 
 
 @pytest.mark.parametrize("W", [8, 64])
