@@ -240,6 +240,17 @@ bool EngineClauseSharer::start_run(InFlight &run) {
         run.active = true;
         return true;
     }
+    // A run with no clause to append, no update and no valid slot would
+    // change nothing on the device (the roll-forward only touches updated
+    // variables) and report nothing: it is delivered without a step.
+    bool any_valid = false;
+    for (uint64_t v : valid) any_valid = any_valid || v != 0;
+    if (b.lens.empty() && run.updates.empty() && !any_valid) {
+        run.idle = true;
+        run.clause_count = clauses_.clause_count();
+        run.active = true;
+        return true;
+    }
     std::vector<uint32_t> rows(b.lens.size());
     gss_step_desc d{};
     d.n_clauses = (int64_t)b.lens.size();
@@ -285,6 +296,10 @@ void EngineClauseSharer::collect(InFlight &run) {
         run.records = n;
         if (policy_ == Policy::Learned) train_policy();
         round_++;
+        return;
+    }
+    if (run.idle) {
+        run.records = 0;
         return;
     }
     int64_t n = 0;

@@ -247,6 +247,7 @@ private:
         int64_t records = 0;  // hit records of the waited step
         bool plain = false;   // a plain sweep (device policies) instead of a step
         bool swept = false;   // plain: a sweep was launched (some slot was valid)
+        bool idle = false;    // nothing to append, apply or check: no device work
     };
 
     void run_once(bool can_start);
