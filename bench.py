@@ -186,8 +186,8 @@ def main() -> int:
                     help="batched-16, serving-32: records with masks (full) or clause and word "
                          "only (ids, what a solver receives)")
     ap.add_argument("--signs", default="planted", choices=["planted", "random"],
-                    help="serving-32: clauses and assignments near a planted assignment, or "
-                         "random signs and polarities")
+                    help="serving-32, batched-16: clauses and assignments near a planted "
+                         "assignment, or random signs and polarities")
     ap.add_argument("--res-scale", type=float, default=1.0, help="residency: workload scale")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batched-16, serving-32: default tile settings")
@@ -385,7 +385,11 @@ def run_batched(args) -> int:
     steps_total = args.warmup + args.steps
     t0 = time.time()
     log("generating workload batched-16 ...")
-    gen = workloads.SolverStreams(seed=3, var_count=1_000_000)
+    # random signs: clause signs and assigned values uniform, 40 % of the
+    # variables assigned (as the headline sweep); planted: near a solution
+    gen = workloads.SolverStreams(seed=3, var_count=1_000_000,
+                                  **({} if args.signs == "planted" else
+                                     dict(p_assigned=0.4, agree=0.5, lit_true=0.5)))
     d0, t0p = gen.initial_table()
     init_lits, init_lens = gen.initial_clauses(1_000_000)
     plan = [gen.step() for _ in range(steps_total)]
@@ -520,6 +524,7 @@ def run_batched(args) -> int:
             "hit_records": int(hits_total), "tuned": tuned, "records": args.records,
             "verified_steps": verified, "steps_in_flight": args.inflight,
             "screen": args.screen,
+            "signs": args.signs,
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),
