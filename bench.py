@@ -22,11 +22,12 @@ before timing starts.  Prints one JSON line.
                  the threads of this process's CPU share (the box's share
                  for one GPU); nproc and the CPU model are recorded beside it
   latency_ms     after the timed region: submit -> every hit of the sweep in
-                 pinned host memory as the solvers receive it (clause and
-                 solver word, 8 bytes per record), one sweep in flight,
-                 through the step path (graph replay, device compaction,
-                 PCIe copy); latency_full_records_ms the same with the
-                 24-byte records that also carry the masks
+                 pinned host memory as the drop-in gpuRun() delivers it (one
+                 16-byte record per clause with its solver words), one sweep
+                 in flight, through the step path (graph replay, device
+                 compaction, PCIe copy); latency_id_records_ms the same with
+                 one 8-byte record per (clause, word), latency_full_records_ms
+                 with the 24-byte records that also carry the masks
 
 --workload batched-16 measures the batched form instead: 16 solver streams
 over 1M vars, each step inserting Poisson(16 x 2800) clauses of Pareto(1.5)
@@ -182,9 +183,10 @@ def main() -> int:
                          "against the table's aggregate before the exact check)")
     ap.add_argument("--flips", type=int, default=32,
                     help="serving-32: variables each assignment redraws from its solver's base")
-    ap.add_argument("--records", default="full", choices=["full", "ids"],
-                    help="batched-16, serving-32: records with masks (full) or clause and word "
-                         "only (ids, what a solver receives)")
+    ap.add_argument("--records", default="full", choices=["full", "ids", "clauses"],
+                    help="batched-16, serving-32: records with masks (full), clause and word "
+                         "only (ids), or (batched-16) one record per clause with its words "
+                         "(clauses, what gpuRun delivers)")
     ap.add_argument("--signs", default="planted", choices=["planted", "random"],
                     help="serving-32, batched-16: clauses and assignments near a planted "
                          "assignment, or random signs and polarities")
@@ -266,43 +268,56 @@ def main() -> int:
     # records the solvers receive (clause and solver word), then with the
     # full records.
     from gpusharesat_amd import serving as sv
-    first = None
+    views = {}
+    sizes = {sv.RECORDS_CLAUSES: 16, sv.RECORDS_IDS: 8, sv.RECORDS_FULL: 24}
 
     def latency_of(records: int) -> dict | None:
-        nonlocal first
         lat = []
         for k in range(args.lat_steps + 2):
             t_sub = time.perf_counter()
             sv.submit(e, valid, records=records)
             n_step, _ = e.step_wait()
-            view = e.step_ids_view() if records == sv.RECORDS_IDS else e.step_hits_view()
+            view = (e.step_clauses_view() if records == sv.RECORDS_CLAUSES else
+                    e.step_ids_view() if records == sv.RECORDS_IDS else e.step_hits_view())
             t_rep = time.perf_counter()
-            if n_step != hits or view.size != hits:
+            if view.size != n_step or (records != sv.RECORDS_CLAUSES and n_step != hits):
                 raise RuntimeError(f"step path gave {n_step} records, the sweep {hits}")
-            if records == sv.RECORDS_IDS and first is None:
-                first = view.copy()
+            if records not in views:
+                views[records] = view.copy()
             if k >= 2:  # the first replays capture the graph and size the buffers
                 lat.append((t_rep - t_sub) * 1e3)
         if not lat:
             return None
-        size = 8 if records == sv.RECORDS_IDS else 24
         return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                 "p999": float(np.percentile(lat, 99.9)), "max": float(max(lat)),
-                "sweeps": len(lat), "bytes_per_record": size,
+                "sweeps": len(lat), "records": int(views[records].size),
+                "bytes_per_record": sizes[records],
                 "what": "submit -> every hit record of the sweep in pinned host memory"}
 
-    latency = latency_of(sv.RECORDS_IDS) if args.lat_steps > 0 else None
-    latency_full = latency_of(sv.RECORDS_FULL) if args.lat_steps > 0 else None
-    if latency:
-        log(f"report latency (8-byte records) p50 {latency['p50']:.2f} ms, p99 "
-            f"{latency['p99']:.2f} ms; 24-byte records p50 {latency_full['p50']:.2f} ms, p99 "
-            f"{latency_full['p99']:.2f} ms ({hits * 8 / 1e6:.0f} / {hits * 24 / 1e6:.0f} MB)")
-    # the ids the solvers receive are the first word of the full records
-    if first is not None:
-        full = e.step_hits_view()
-        if not np.array_equal(np.sort(first.view(np.uint64)),
+    latency = latency_ids = latency_full = None
+    if args.lat_steps > 0:
+        latency = latency_of(sv.RECORDS_CLAUSES)
+        latency_ids = latency_of(sv.RECORDS_IDS)
+        latency_full = latency_of(sv.RECORDS_FULL)
+        log(f"report latency, one 16-byte record per clause ({latency['records']}): p50 "
+            f"{latency['p50']:.2f} ms, p99 {latency['p99']:.2f} ms; 8-byte per (clause, word) "
+            f"p99 {latency_ids['p99']:.2f} ms; 24-byte p99 {latency_full['p99']:.2f} ms")
+        # the three forms carry the same reports: ids = the full records'
+        # first words; clause records = the ids grouped by clause
+        ids = views[sv.RECORDS_IDS]
+        full = views[sv.RECORDS_FULL]
+        if not np.array_equal(np.sort(ids.view(np.uint64)),
                               np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0])):
             raise RuntimeError("id records differ from the full records' first words")
+        key = ids["row"].astype(np.uint64) | (ids["len"].astype(np.uint64) << np.uint64(32))
+        uk, inv = np.unique(key, return_inverse=True)
+        words = np.zeros(uk.size, np.uint64)
+        np.bitwise_or.at(words, inv, np.uint64(1) << ids["word"].astype(np.uint64))
+        cl = views[sv.RECORDS_CLAUSES]
+        ck = cl["row"].astype(np.uint64) | (cl["len"].astype(np.uint64) << np.uint64(32))
+        o = np.argsort(ck)
+        if not (np.array_equal(ck[o], uk) and np.array_equal(cl["words"][o], words)):
+            raise RuntimeError("clause records differ from the id records grouped by clause")
     # The device's own ceiling for this access pattern: random 128-byte line
     # gathers (independent, 4 in flight per lane) from a table of the same size.
     kernel_name = "k_sweep<16,1,4,true>"
@@ -356,6 +371,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "latency_ms": latency,
+        "latency_id_records_ms": latency_ids,
         "latency_full_records_ms": latency_full,
     }
     print(json.dumps(out), flush=True)
@@ -465,7 +481,8 @@ def run_batched(args) -> int:
     todo = list(range(args.warmup, steps_total))
 
     from gpusharesat_amd import serving as sv
-    rec = sv.RECORDS_IDS if args.records == "ids" else sv.RECORDS_FULL
+    rec = {"ids": sv.RECORDS_IDS, "clauses": sv.RECORDS_CLAUSES}.get(args.records,
+                                                                    sv.RECORDS_FULL)
 
     submit_s = []
 
@@ -492,7 +509,9 @@ def run_batched(args) -> int:
         if todo:
             enqueue_next()
         # records on the host (pinned, no copy)
-        hits_total += (e.step_ids_view() if rec == sv.RECORDS_IDS else e.step_hits_view()).size
+        hits_total += (e.step_ids_view() if rec == sv.RECORDS_IDS else
+                       e.step_clauses_view() if rec == sv.RECORDS_CLAUSES else
+                       e.step_hits_view()).size
         done_t.append(time.perf_counter())
         lat.append(done_t[-1] - pending.pop(0))
         step_ms.append(ms)
@@ -555,6 +574,9 @@ def run_serving(args) -> int:
     on the device by the step that starts next; latency = arrival -> the
     request's records handed to it on the host."""
     from gpusharesat_amd import serving as sv
+    if args.records == "clauses":
+        log("serving-32 hands each request its records by word: --records full or ids")
+        return 2
     cfg = sv.ServeConfig(duration=args.duration, rate=args.rate, max_inflight=args.inflight,
                          records=sv.RECORDS_IDS if args.records == "ids" else sv.RECORDS_FULL,
                          planted=args.signs == "planted", flips=args.flips)

@@ -263,7 +263,7 @@ struct StepSlot {
     Pinned<uint64_t> h_group;
     bool results = false;  // h_out / d_hits hold the results of the last waited step
     int rec_words = 3;     // record form of the slot's last submitted step (3: gss_hit,
-                           // 1: gss_hit_id)
+                           // 1: gss_hit_id, 2: gss_hit_clause)
     int res_words = 3;     // the same, of its last waited step (the results)
     std::vector<const void *> key;  // everything baked into the captured graph
 };
@@ -339,7 +339,7 @@ struct gss_engine {
     float last_ms = 0.f;
     double total_ms = 0.0;
     int64_t nsweeps = 0;
-    int occ[7][9][2][2] = {};  // [log2 words][unroll][fused][prefilter] resident blocks per CU
+    int occ[7][9][2][3] = {};  // [log2 words][unroll][fused][sweep mode] resident blocks per CU
 
     // tuning
     int tune_waves_per_cu = 0;
@@ -414,13 +414,13 @@ int fuse_of(const gss_engine *e) {
 }
 
 // Resident 256-thread blocks per CU of the sweep variant the engine runs.
-int resident_blocks(gss_engine *e, bool agg = false) {
+int resident_blocks(gss_engine *e, int mode = 0) {
     int l = 0;
     while ((1 << l) < e->words) l++;
     const int ui = std::min(e->tune_unroll, 8);
     const int fi = fuse_of(e) > 0;
-    int &o = e->occ[l][ui][fi][agg];
-    if (!o) o = sweep_occupancy(e->words, e->tune_unroll, fi, agg);
+    int &o = e->occ[l][ui][fi][mode];
+    if (!o) o = sweep_occupancy(e->words, e->tune_unroll, fi, mode);
     return o;
 }
 
@@ -1511,8 +1511,8 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
 // Persistent grid of a step: enough 4-wave blocks for the tasks, rounded up
 // to a power of two (so a growing DB re-captures the graph rarely), at most
 // the resident maximum.
-static int step_blocks(gss_engine *e, int64_t tasks, bool agg) {
-    int per_cu = resident_blocks(e, agg);
+static int step_blocks(gss_engine *e, int64_t tasks, int mode) {
+    int per_cu = resident_blocks(e, mode);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
     const int64_t full = (int64_t)e->cu * per_cu;
     int64_t b = 1;
@@ -1772,8 +1772,13 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         n_patches < 0 || (n_patches > 0 && !desc->patches) || n_bups < 0 ||
         (n_bups > 0 && !desc->base_ups) || n_loads < 0 || (n_loads > 0 && !desc->loads))
         return set_err(GSS_E_INVAL, "bad step arguments");
-    if (desc->records != GSS_RECORDS_FULL && desc->records != GSS_RECORDS_IDS)
-        return set_err(GSS_E_INVAL, "records must be GSS_RECORDS_FULL or GSS_RECORDS_IDS");
+    if (desc->records != GSS_RECORDS_FULL && desc->records != GSS_RECORDS_IDS &&
+        desc->records != GSS_RECORDS_CLAUSES)
+        return set_err(GSS_E_INVAL, "records must be GSS_RECORDS_FULL, _IDS or _CLAUSES");
+    if (serve && desc->records == GSS_RECORDS_CLAUSES)
+        return set_err(GSS_E_INVAL, "serving steps group records by word: full or id records");
+    if ((desc->flags & GSS_STEP_SCREEN) && desc->records == GSS_RECORDS_CLAUSES)
+        return set_err(GSS_E_INVAL, "GSS_STEP_SCREEN takes full or id records");
     if (desc->flags & ~GSS_STEP_SCREEN) return set_err(GSS_E_INVAL, "unknown step flags");
     // serving steps screen against their requests' bases instead
     const bool screen = !serve && (desc->flags & GSS_STEP_SCREEN);
@@ -1938,10 +1943,16 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         tasks += (c + tc - 1) / tc;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
-    const int blocks = step_blocks(e, tasks, (serve && e->serve_prefilter) || screen);
+    const int blocks =
+        step_blocks(e, tasks, (serve && e->serve_prefilter) || screen ? 1
+                              : desc->records == GSS_RECORDS_CLAUSES   ? 2
+                                                                       : 0);
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
-    const uint64_t ub = (uint64_t)(e->total_clauses + n) * (uint64_t)active_words;
+    // (one record per clause in the clause form)
+    const uint64_t ub = (uint64_t)(e->total_clauses + n) *
+                        (uint64_t)(desc->records == GSS_RECORDS_CLAUSES ? std::min(active_words, 1)
+                                                                         : active_words);
     uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
     if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_cls.reserve(std::max<int64_t>(n, 1))) ||
         (rc = sl.h_desc.reserve(std::max(nb, 1))) ||
@@ -1972,7 +1983,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (rc = sl.d_loads.reserve_async(std::max<size_t>(sl.h_loads.cap, GSS_MAX_WORDS), es)))
         return rc;
     // the hit buffer holds chunks of records of this step's form
-    const int rw = desc->records == GSS_RECORDS_IDS ? 1 : 3;
+    const int rw = desc->records == GSS_RECORDS_IDS ? 1 : desc->records == GSS_RECORDS_CLAUSES ? 2 : 3;
     auto chunk_room = [&] {
         return (uint32_t)std::min<size_t>(
             std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / ((size_t)kHitChunk * rw)),
@@ -2161,7 +2172,8 @@ int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n) {
         return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
     StepSlot &sl = e->slots[e->last_step_slot];
     if (sl.res_words != 3)
-        return set_err(GSS_E_STATE, "the last waited step recorded ids only (gss_step_ids_map)");
+        return set_err(GSS_E_STATE, "the last waited step recorded ids or clauses only "
+                                    "(gss_step_ids_map, gss_step_clauses_map)");
     const int64_t total = e->last_step_hits;
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
@@ -2179,12 +2191,30 @@ int gss_step_ids_map(gss_engine *e, const gss_hit_id **out, int64_t *n) {
         return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
     StepSlot &sl = e->slots[e->last_step_slot];
     if (sl.res_words != 1)
-        return set_err(GSS_E_STATE, "the last waited step recorded full hits (gss_step_hits_map)");
+        return set_err(GSS_E_STATE, "the last waited step recorded another form");
     const int64_t total = e->last_step_hits;
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
     *out = (const gss_hit_id *)sl.h_hits.p;
+    return GSS_OK;
+}
+
+int gss_step_clauses_map(gss_engine *e, const gss_hit_clause **out, int64_t *n) {
+    int rc = check_engine(e);
+    if (rc) return rc;
+    if (!out) return set_err(GSS_E_INVAL, "null output");
+    *out = nullptr;
+    if (e->last_step_slot < 0 || !e->slots[e->last_step_slot].results)
+        return set_err(GSS_E_STATE, "no waited step with results (the slot was reused)");
+    StepSlot &sl = e->slots[e->last_step_slot];
+    if (sl.res_words != 2)
+        return set_err(GSS_E_STATE, "the last waited step recorded another form");
+    const int64_t total = e->last_step_hits;
+    if (n) *n = total;
+    if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
+    sl.host_ready = true;
+    *out = (const gss_hit_clause *)sl.h_hits.p;
     return GSS_OK;
 }
 

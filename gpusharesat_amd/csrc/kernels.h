@@ -36,7 +36,8 @@ struct SweepArgs {
                              // (<= 0: none, and the kernel without that path runs)
     uint32_t sentinel;       // stored row offset of the sentinel literal
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
-    int rec_words;           // 8-byte words per record: 3 (gss_hit) or 1 (gss_hit_id)
+    int rec_words;           // 8-byte words per record: 3 (gss_hit), 1 (gss_hit_id) or 2
+                             // (gss_hit_clause: one record per clause, its words as a mask)
     // Serving steps (null otherwise): bit b of agg[l] is set only if literal
     // l is non-false in every valid slot of every request of group b (the
     // step's admitted requests r with r % 8 = b); 2 * vars + 2 bytes (padded
@@ -184,8 +185,8 @@ struct CompactRef {
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
 // Max resident 256-thread blocks per CU for the sweep kernel variant (with or
-// without the fused schedule, with or without the serving prefilter).
-int sweep_occupancy(int words, int unroll, bool fused, bool agg = false);
+// without the fused schedule; mode 0: plain, 1: screened, 2: clause records).
+int sweep_occupancy(int words, int unroll, bool fused, int mode = 0);
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_scatter(const ScatterSeg *segs, int nseg, const uint32_t *src, int shift,
@@ -217,7 +218,8 @@ hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, ui
 // place): offs = exclusive prefix of the fills of the claimed chunks (cap + 1
 // entries), dense = the records, when they all fit in dense_cap records;
 // counts are read on the device.  Records are rec_words 8-byte words (3: a
-// gss_hit, 1: a gss_hit_id); chunk c starts at word c * kHitChunk * rec_words.
+// gss_hit, 1: a gss_hit_id, 2: a gss_hit_clause); chunk c starts at word
+// c * kHitChunk * rec_words.
 hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                                   uint64_t *offs, const void *hits, void *dense,
                                   uint64_t dense_cap, int rec_words, hipStream_t s);

@@ -79,6 +79,9 @@ static_assert(sizeof(gss_hit) == 24 && offsetof(gss_hit, len) == 4 &&
 static_assert(sizeof(gss_hit_id) == 8 && offsetof(gss_hit_id, len) == 4 &&
                   offsetof(gss_hit_id, word) == 6,
               "gss_hit_id is the first word of a gss_hit");
+static_assert(sizeof(gss_hit_clause) == 16 && offsetof(gss_hit_clause, len) == 4 &&
+                  offsetof(gss_hit_clause, words) == 8,
+              "gss_hit_clause: {row, len, 0} then the word mask");
 
 namespace {
 
@@ -371,8 +374,14 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
     return lo;
 }
 
-template <int W, int S, int U, bool FUSE, bool AGG>
+// MODE: 0 = every clause checked, records per (clause, word) (full or id
+// form, a.rec_words); 1 = clauses screened against a.agg first (serving and
+// screened steps), same record forms; 2 = every clause checked, one record
+// per clause with its words (a.rec_words = 2).
+template <int W, int S, int U, bool FUSE, int MODE>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
+    constexpr bool AGG = MODE == 1;
+    constexpr bool CLAUSE_RECORDS = MODE == 2;
     constexpr int K = W > 16 ? W / 16 : 1;  // table words per lane
     constexpr int LW = W / K;               // lanes across one row
     constexpr int G = LW * S;
@@ -405,37 +414,73 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     int fill = 0;
     // Appends this pass's records (all lanes call it; `active` marks the
     // groups that checked a clause).
+    auto claim = [&](unsigned long long bal) {
+        const int n = __popcll(bal);
+        if (chunk < 0 || fill + n > kHitChunk) {
+            if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
+                a.fills[chunk] = (uint32_t)fill;
+            uint32_t c = 0;
+            if (lane == 0) c = atomicAdd(a.alloc, 1u);
+            chunk = uniform((int)__shfl(c, 0));
+            fill = 0;
+        }
+    };
     auto emit = [&](bool active, int64_t row, int len, const uint64_t (&fm)[K],
                     const uint64_t (&um)[K]) {
+        if constexpr (CLAUSE_RECORDS) {
+            // one record per clause: the words with a report of the group
+            uint64_t wm = 0;
+            if constexpr (K == 1) {
+                // word lw is lane gbase + lw (sub-lane 0): one ballot
+                const unsigned long long bw =
+                    __ballot(active && sub == 0 && (fm[0] | um[0]) != 0);
+                wm = (bw >> gbase) & (LW == 64 ? ~0ull : ((1ull << LW) - 1ull));
+            } else {
+                // each lane holds K consecutive words: OR over the group's lanes
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const bool hit = active && sub == 0 && (fm[k] | um[k]) != 0;
+                for (int k = 0; k < K; k++)
+                    if (active && (fm[k] | um[k]) != 0) wm |= 1ull << (lw * K + k);
+#pragma unroll
+                for (int d = 1; d < G; d <<= 1) wm |= __shfl_xor(wm, d);
+            }
+            const bool hit = active && q == 0 && wm != 0;
             const unsigned long long bal = __ballot(hit);
             if (bal) {
-                const int n = __popcll(bal);
-                if (chunk < 0 || fill + n > kHitChunk) {
-                    if (chunk >= 0 && lane == 0 && (uint32_t)chunk < a.chunk_cap)
-                        a.fills[chunk] = (uint32_t)fill;
-                    uint32_t c = 0;
-                    if (lane == 0) c = atomicAdd(a.alloc, 1u);
-                    chunk = uniform((int)__shfl(c, 0));
-                    fill = 0;
-                }
+                claim(bal);
                 if (hit && (uint32_t)chunk < a.chunk_cap) {
-                    // Records are streamed out and never read by the kernel:
-                    // non-temporal stores keep them from displacing table
-                    // rows in L2 and the memory-side cache.
-                    const uint64_t id = (uint64_t)(uint32_t)row | ((uint64_t)len << 32) |
-                                        ((uint64_t)(lw * K + k) << 48);
                     const size_t r = (size_t)chunk * kHitChunk + fill + prefix_count(bal);
-                    uint64_t *dst = (uint64_t *)a.hits + r * (size_t)a.rec_words;
-                    __builtin_nontemporal_store(id, dst);
-                    if (a.rec_words == 3) {
-                        __builtin_nontemporal_store(fm[k], dst + 1);
-                        __builtin_nontemporal_store(um[k], dst + 2);
-                    }
+                    uint64_t *dst = (uint64_t *)a.hits + r * 2;
+                    __builtin_nontemporal_store((uint64_t)(uint32_t)row | ((uint64_t)len << 32),
+                                                dst);
+                    __builtin_nontemporal_store(wm, dst + 1);
                 }
-                fill += n;
+                fill += __popcll(bal);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const bool hit = active && sub == 0 && (fm[k] | um[k]) != 0;
+                const unsigned long long bal = __ballot(hit);
+                if (bal) {
+                    const int n = __popcll(bal);
+                    claim(bal);
+                    if (hit && (uint32_t)chunk < a.chunk_cap) {
+                        // Records are streamed out and never read by the
+                        // kernel: non-temporal stores keep them from
+                        // displacing table rows in L2 and the memory-side
+                        // cache.
+                        const uint64_t id = (uint64_t)(uint32_t)row | ((uint64_t)len << 32) |
+                                            ((uint64_t)(lw * K + k) << 48);
+                        const size_t r = (size_t)chunk * kHitChunk + fill + prefix_count(bal);
+                        uint64_t *dst = (uint64_t *)a.hits + r * (size_t)a.rec_words;
+                        __builtin_nontemporal_store(id, dst);
+                        if (a.rec_words == 3) {
+                            __builtin_nontemporal_store(fm[k], dst + 1);
+                            __builtin_nontemporal_store(um[k], dst + 2);
+                        }
+                    }
+                    fill += n;
+                }
             }
         }
     };
@@ -1098,37 +1143,44 @@ __global__ void k_compact_move(const CompactRef *__restrict__ refs, int nb, int6
 template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
     const dim3 g(blocks), b(256);
-    if (a.agg) {
-        if (a.fuse_len > 0) hipLaunchKernelGGL((k_sweep<W, S, U, true, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_sweep<W, S, U, false, true>), g, b, 0, s, a);
-    } else {
-        if (a.fuse_len > 0) hipLaunchKernelGGL((k_sweep<W, S, U, true, false>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_sweep<W, S, U, false, false>), g, b, 0, s, a);
+    const bool fused = a.fuse_len > 0;
+    if (a.agg && a.rec_words == 2) return hipErrorInvalidValue;  // no such variant
+    const int mode = a.agg ? 1 : a.rec_words == 2 ? 2 : 0;
+#define GSS_LAUNCH(F_, M_)                                                     \
+    if (fused == F_ && mode == M_) {                                           \
+        hipLaunchKernelGGL((k_sweep<W, S, U, F_, M_>), g, b, 0, s, a);         \
+        return hipGetLastError();                                              \
     }
-    return hipGetLastError();
+    GSS_LAUNCH(true, 0)
+    GSS_LAUNCH(false, 0)
+    GSS_LAUNCH(true, 1)
+    GSS_LAUNCH(false, 1)
+    GSS_LAUNCH(true, 2)
+    GSS_LAUNCH(false, 2)
+#undef GSS_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 template <int W, int S, int U>
-int occupancy_t(bool fused, bool agg) {
+int occupancy_t(bool fused, int mode) {
     int nb = 0;
-    hipError_t e;
-    if (agg)
-        e = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, true, true>,
-                                                                 256, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, k_sweep<W, S, U, false, true>, 256, 0);
-    else
-        e = fused ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, k_sweep<W, S, U, true, false>, 256, 0)
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, k_sweep<W, S, U, false, false>, 256, 0);
+    hipError_t e = hipErrorInvalidValue;
+#define GSS_OCC(F_, M_)                                                                     \
+    if (fused == F_ && mode == M_)                                                          \
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<W, S, U, F_, M_>, 256, 0);
+    GSS_OCC(true, 0)
+    GSS_OCC(false, 0)
+    GSS_OCC(true, 1)
+    GSS_OCC(false, 1)
+    GSS_OCC(true, 2)
+    GSS_OCC(false, 2)
+#undef GSS_OCC
     if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 
 // Kernel variants: (W, S, U), each with and without the fused schedule
-// (fuse_len <= 0 selects the one without), and with and without the
-// aggregate prefilter of serving steps (SweepArgs::agg set).  `unroll` selects U for W >= 16
+// (fuse_len <= 0 selects the one without), in the three modes of k_sweep.  `unroll` selects U for W >= 16
 // (0 = default).
 #define GSS_SWEEP_VARIANTS(X)                  \
     X(1, 16, 1)                                \
@@ -1179,10 +1231,10 @@ hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, h
     return hipErrorInvalidValue;
 }
 
-int sweep_occupancy(int words, int unroll, bool fused, bool agg) {
+int sweep_occupancy(int words, int unroll, bool fused, int mode) {
     const Variant v = pick(words, unroll);
 #define X(W_, S_, U_) \
-    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>(fused, agg);
+    if (v.w == W_ && v.s == S_ && v.u == U_) return occupancy_t<W_, S_, U_>(fused, mode);
     GSS_SWEEP_VARIANTS(X)
 #undef X
     return 1;
@@ -1277,7 +1329,7 @@ hipError_t launch_count_records(const uint32_t *alloc, const uint32_t *fills, ui
 hipError_t launch_compact_records(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                                   uint64_t *offs, const void *hits, void *dense,
                                   uint64_t dense_cap, int rec_words, hipStream_t s) {
-    if (rec_words != 1 && rec_words != 3) return hipErrorInvalidValue;
+    if (rec_words < 1 || rec_words > 3) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, s, alloc, fills, cap, offs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

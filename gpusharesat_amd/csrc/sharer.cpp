@@ -104,6 +104,14 @@ EngineClauseSharer::EngineClauseSharer(GpuClauseSharerOptions opts,
         else if (!v.empty() && v != "host")
             throw std::invalid_argument("GPUSHARESAT_REDUCE_POLICY must be host, device or learned");
     }
+    // record form of a run's reports: one per clause with its solvers
+    // (default) or one per (clause, solver)
+    if (const char *r = std::getenv("GPUSHARESAT_RECORDS")) {
+        const std::string v = r;
+        if (v == "ids") clause_records_ = false;
+        else if (!v.empty() && v != "clauses")
+            throw std::invalid_argument("GPUSHARESAT_RECORDS must be clauses or ids");
+    }
     if (policy_ != Policy::Host) {
         max_tags_ = 1 << 22;
         if (const char *t = std::getenv("GPUSHARESAT_RESIDENCY_MAX_TAGS"))
@@ -261,7 +269,7 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     d.ups = run.updates.data();
     d.valid = valid.data();
     d.last_pos = run.last_pos.data();
-    d.records = GSS_RECORDS_IDS;
+    d.records = clause_records_ ? GSS_RECORDS_CLAUSES : GSS_RECORDS_IDS;
     check_rc(gss_step_submit(engine_, &d), "gss_step_submit");
     for (size_t i = 0; i < rows.size(); i++)
         if (rows[i] != b.rows[i])
@@ -375,14 +383,21 @@ bool EngineClauseSharer::add_on_device(const ClauseMirror::Batch &b) {
 }
 
 void EngineClauseSharer::deliver(InFlight &run) {
+    // the run's reports: (clause, solver word) ids from a plain sweep, or one
+    // record per clause with its solver words from a step
     const gss_hit_id *ids = nullptr;
-    int64_t n = 0;
+    const gss_hit_clause *cls = nullptr;
+    int64_t n_ids = 0, n_cls = 0;
     if (run.plain) {
         ids = id_buf_.data();
-        n = (int64_t)id_buf_.size();
+        n_ids = (int64_t)id_buf_.size();
+    } else if (run.records && clause_records_) {
+        check_rc(gss_step_clauses_map(engine_, &cls, &n_cls), "gss_step_clauses_map");
     } else if (run.records) {
-        check_rc(gss_step_ids_map(engine_, &ids, &n), "gss_step_ids_map");
+        check_rc(gss_step_ids_map(engine_, &ids, &n_ids), "gss_step_ids_map");
     }
+    int64_t n = n_ids;
+    for (int64_t i = 0; i < n_cls; i++) n += __builtin_popcountll(cls[i].words);
     global_stats_[gpuRuns]++;
     long assigs = 0, solvers_with = 0;
     for (const auto &t : run.taken) {
@@ -394,7 +409,11 @@ void EngineClauseSharer::deliver(InFlight &run) {
     global_stats_[clauseTestsOnGroups] += cl;
     global_stats_[clauseTestsOnAssigs] += cl * (unsigned long)solvers_with;
     global_stats_[gpuReports] += (unsigned long)n;
-    for (int64_t i = 0; i < n; i++) clauses_.bump(ids[i].len, ids[i].row);
+    // one activity bump per (clause, solver) report (GpuRunner.cu:375-378)
+    for (int64_t i = 0; i < n_ids; i++) clauses_.bump(ids[i].len, ids[i].row);
+    for (int64_t i = 0; i < n_cls; i++)
+        for (int k = __builtin_popcountll(cls[i].words); k > 0; k--)
+            clauses_.bump(cls[i].len, cls[i].row);
     Gauge g(global_stats_[timeSpentFillingReported], opts_.quickProf);
     const int S = (int)run.taken.size();
     std::vector<ClauseBatch *> batch(S, nullptr);
@@ -404,15 +423,17 @@ void EngineClauseSharer::deliver(InFlight &run) {
         batch[s]->start_assig = run.taken[s].start_id;
         batch[s]->assig_count = run.taken[s].count;
     }
-    for (int64_t i = 0; i < n; i++) {
-        const gss_hit_id &h = ids[i];
-        const int s = h.word;
-        if (s >= S || !batch[s]) continue;  // cannot happen: hits only in valid slots
+    auto add = [&](int s, int len, uint32_t row) {
+        if (s >= S || !batch[s]) return;  // cannot happen: hits only in valid slots
         ClauseBatch &b = *batch[s];
-        b.clauses.push_back({clauses_.meta(h.len, h.row).id, (int)b.lits.size()});
-        const uint32_t *l = clauses_.row_lits(h.len, h.row);
-        b.lits.insert(b.lits.end(), l, l + h.len);
-    }
+        b.clauses.push_back({clauses_.meta(len, row).id, (int)b.lits.size()});
+        const uint32_t *l = clauses_.row_lits(len, row);
+        b.lits.insert(b.lits.end(), l, l + len);
+    };
+    for (int64_t i = 0; i < n_ids; i++) add(ids[i].word, ids[i].len, ids[i].row);
+    for (int64_t i = 0; i < n_cls; i++)
+        for (uint64_t w = cls[i].words; w; w &= w - 1)
+            add(__builtin_ctzll(w), cls[i].len, cls[i].row);
     for (int s = 0; s < S; s++)
         if (batch[s]) reports_[s]->publish(batch[s]);
 }

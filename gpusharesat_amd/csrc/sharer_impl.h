@@ -293,6 +293,7 @@ private:
     // gpuRun as a plain sweep whose records feed the per-clause state.
     enum class Policy { Host, Device, Learned };
     Policy policy_ = Policy::Host;
+    bool clause_records_ = true;  // GPUSHARESAT_RECORDS: clauses (default) or ids
     gss_res *res_ = nullptr;
     int64_t max_tags_ = 0;
     std::vector<uint32_t> free_tags_;

@@ -15,6 +15,7 @@ from . import _lib
 HIT_DTYPE = np.dtype([("row", "<u4"), ("len", "<u2"), ("word", "<u2"),
                       ("false_mask", "<u8"), ("unit_mask", "<u8")])
 HIT_ID_DTYPE = np.dtype([("row", "<u4"), ("len", "<u2"), ("word", "<u2")])
+HIT_CLAUSE_DTYPE = np.dtype([("row", "<u4"), ("len", "<u2"), ("pad", "<u2"), ("words", "<u8")])
 VAR_UPDATE_DTYPE = np.dtype([("var", "<u4"), ("word", "<u4"),
                              ("is_def", "<u8"), ("is_true", "<u8")])
 assert HIT_DTYPE.itemsize == 24 and VAR_UPDATE_DTYPE.itemsize == 24 and HIT_ID_DTYPE.itemsize == 8
@@ -326,6 +327,17 @@ class Engine:
             return np.zeros(0, dtype=HIT_ID_DTYPE)
         buf = (ctypes.c_char * (n.value * HIT_ID_DTYPE.itemsize)).from_address(p.value)
         return np.frombuffer(buf, dtype=HIT_ID_DTYPE)
+
+    def step_clauses_view(self) -> np.ndarray:
+        """gss_hit_clause records of the last waited step (submitted with
+        GSS_RECORDS_CLAUSES) as a view of pinned memory, valid as
+        step_hits_view's."""
+        p, n = ctypes.c_void_p(), ctypes.c_int64()
+        _lib.call("gss_step_clauses_map", self._h, ctypes.byref(p), ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros(0, dtype=HIT_CLAUSE_DTYPE)
+        buf = (ctypes.c_char * (n.value * HIT_CLAUSE_DTYPE.itemsize)).from_address(p.value)
+        return np.frombuffer(buf, dtype=HIT_CLAUSE_DTYPE)
 
     def step_hits(self) -> np.ndarray:
         n = ctypes.c_int64()

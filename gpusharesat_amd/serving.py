@@ -64,6 +64,7 @@ def _p(a):
 
 RECORDS_FULL = 0
 RECORDS_IDS = 1
+RECORDS_CLAUSES = 2
 STEP_SCREEN = 1  # gss_step_desc.flags
 
 

@@ -78,6 +78,18 @@ typedef struct gss_hit_id {
     uint16_t word;
 } gss_hit_id;
 
+/* One record per clause for delivery (GSS_RECORDS_CLAUSES): the clause and
+ * the set of assignment words in which it has a report.  In the drop-in
+ * interface word s holds solver s's slots, so `words` is the set of solvers
+ * that receive the clause; a clause reported in many words costs 16 bytes
+ * instead of 8 per word. */
+typedef struct gss_hit_clause {
+    uint32_t row;
+    uint16_t len;
+    uint16_t pad;   /* 0 */
+    uint64_t words; /* bit w: a report in word w */
+} gss_hit_clause;
+
 typedef struct gss_var_update {
     uint32_t var;
     uint32_t word;
@@ -294,11 +306,14 @@ typedef struct gss_step_desc {
     int32_t records;                   /* GSS_RECORDS_FULL (0): gss_hit records, read with
                                           gss_step_hits_map / gss_step_hits_copy;
                                           GSS_RECORDS_IDS (1): gss_hit_id records, read
-                                          with gss_step_ids_map */
+                                          with gss_step_ids_map;
+                                          GSS_RECORDS_CLAUSES (2): gss_hit_clause records,
+                                          read with gss_step_clauses_map */
     int32_t flags;                     /* 0, or GSS_STEP_SCREEN */
 } gss_step_desc;
 #define GSS_RECORDS_FULL 0
 #define GSS_RECORDS_IDS 1
+#define GSS_RECORDS_CLAUSES 2
 /* Screens every clause, one per lane, against an aggregate of the table
  * (per literal, a bit per group of words: non-false in every valid slot of
  * the group, from one read of the table after the step's updates) and runs
@@ -336,6 +351,10 @@ int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n);
  * gss_step_hits_map's.  Each is the first word of the gss_hit the full form
  * records for the same (clause, word). */
 int gss_step_ids_map(gss_engine *e, const gss_hit_id **out, int64_t *n);
+/* The same for a step submitted with GSS_RECORDS_CLAUSES: one gss_hit_clause
+ * per clause with a report in some word (the words of its gss_hit_id records,
+ * as a mask), in no particular order. */
+int gss_step_clauses_map(gss_engine *e, const gss_hit_clause **out, int64_t *n);
 /* Makes room for n more clauses (lens[i] literals each) in their length
  * buckets without adding any: a step that appends them then needs no bucket
  * growth.  GSS_E_NOMEM when the device (or the DB byte cap) cannot hold them,

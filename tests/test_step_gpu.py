@@ -366,7 +366,7 @@ def test_id_records_are_the_full_records_first_words(engine_factory):
                   records=sv.RECORDS_IDS)
         n, _ = a.step_wait()
         ids = a.step_ids_view().copy()
-        with pytest.raises(GssError, match="ids only"):
+        with pytest.raises(GssError, match="ids or clauses only"):
             a.step_hits_view()
         print(f"step {k}: {n} id records", flush=True)
         # the rows the step gave the new clauses: read back from the DB order
@@ -386,9 +386,57 @@ def test_id_records_are_the_full_records_first_words(engine_factory):
     a.step_enqueue(valid=valid)
     a.step_wait()
     full = a.step_hits_view()
-    with pytest.raises(GssError, match="full hits"):
+    with pytest.raises(GssError, match="another form"):
         a.step_ids_view()
     assert np.array_equal(ids, np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0]))
+
+
+@pytest.mark.parametrize("W", [1, 4, 16, 64])
+def test_clause_records_are_the_reports_by_clause(engine_factory, W):
+    """A step submitted with GSS_RECORDS_CLAUSES records one gss_hit_clause
+    per clause with a report in some word: its row and length, and exactly
+    the words of the oracle's reports for that clause; for steps that append,
+    update and roll forward in between, at every width (sub-lanes below 16
+    words, several words per lane above); the other readers refuse it."""
+    from gpusharesat_amd import serving as sv
+    rng = np.random.default_rng(730 + W)
+    V = 1500
+    a, b = engine_factory(), engine_factory()
+    for e in (a, b):
+        e.reset_assignments(V, W)
+    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    a.upload(d0, t0)
+    b.upload(d0, t0)
+    plain = Plain(b)
+    lens_a, rows_a = np.zeros(0, np.int32), np.zeros(0, np.uint32)
+    for k in range(5):
+        lits, lens, ups, valid, last_pos = make_step(rng, V, W, int(rng.integers(200, 900)), 25,
+                                                     600)
+        exp = plain.step(lits, lens, ups, valid, last_pos)
+        sv.submit(a, valid, lits=lits, lens=lens, ups=ups, last_pos=last_pos,
+                  records=sv.RECORDS_CLAUSES)
+        n, _ = a.step_wait()
+        cls = a.step_clauses_view().copy()
+        with pytest.raises(GssError, match="ids or clauses only"):
+            a.step_hits_view()
+        with pytest.raises(GssError, match="another form"):
+            a.step_ids_view()
+        lens_a = np.concatenate([lens_a, lens])
+        rows_a = np.concatenate([rows_a, plain.rows[-lens.size:]])
+        # expected: one record per clause of the oracle's reports, its words ORed
+        uc, inv = np.unique(exp["clause"], return_inverse=True)
+        words = np.zeros(uc.size, np.uint64)
+        np.bitwise_or.at(words, inv, np.uint64(1) << exp["word"].astype(np.uint64))
+        print(f"W {W} step {k}: {n} clause records, {exp.size} reports", flush=True)
+        assert cls.size == n == uc.size
+        assert (cls["pad"] == 0).all()
+        key_got = cls["row"].astype(np.uint64) | (cls["len"].astype(np.uint64) << np.uint64(32))
+        key_exp = (rows_a[uc].astype(np.uint64)
+                   | (lens_a[uc].astype(np.uint64) << np.uint64(32)))
+        og, oe = np.argsort(key_got), np.argsort(key_exp)
+        assert np.array_equal(key_got[og], key_exp[oe])
+        assert np.array_equal(cls["words"][og], words[oe])
 
 
 def test_step_with_no_valid_slot_still_appends_and_updates(engine_factory):
