@@ -635,6 +635,13 @@ def run_serving(args) -> int:
                        "what": "arrival -> the request's records handed to it on the host"},
         "latency_by_arrival_10s": windows,
         "device_ms_per_step": out.device_ms / max(1, out.steps),
+        "host_grouped_steps": out.host_grouped, "graph_captures": out.graph_captures,
+        "serving_thread": out.thread,
+        "driver_lag_ms": {"p99": float(np.percentile(out.lag_ms, 99)),
+                          "p999": float(np.percentile(out.lag_ms, 99.9)),
+                          "max": float(out.lag_ms.max()),
+                          "what": "arrival -> published in the ring by the Python driver "
+                                  "(part of latency_ms)"},
         "device_ms_per_step_by_window": [
             {"from_s": round(a[0], 1), "to_s": round(b[0], 1), "clauses_end": int(b[3]),
              "steps": int(b[1] - a[1]),

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -393,6 +394,7 @@ struct gss_engine {
     int64_t last_step_hits = 0;
     uint32_t last_step_chunks = 0;
     float last_step_ms = 0.f;
+    std::atomic<int64_t> graph_captures{0};  // step graphs captured (read by other threads)
     // host threads for the staging copies of large steps
     // (GPUSHARESAT_STAGE_THREADS, default 4; 0 or 1 = the calling thread only)
     int stage_threads = 4;
@@ -1508,16 +1510,13 @@ int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, floa
 
 /* ---- batched steps -------------------------------------------------------- */
 
-// Persistent grid of a step: enough 4-wave blocks for the tasks, rounded up
-// to a power of two (so a growing DB re-captures the graph rarely), at most
-// the resident maximum.
-static int step_blocks(gss_engine *e, int64_t tasks, int mode) {
+// Persistent grid of a step: the resident maximum of 4-wave blocks, whatever
+// the tasks (waves without a task end at once), so a growing DB never
+// re-captures the graph for its grid.
+static int step_blocks(gss_engine *e, int mode) {
     int per_cu = resident_blocks(e, mode);
     if (e->tune_waves_per_cu > 0) per_cu = std::max(1, std::min(per_cu, e->tune_waves_per_cu / 4));
-    const int64_t full = (int64_t)e->cu * per_cu;
-    int64_t b = 1;
-    while (b < (tasks + 3) / 4 && b < full) b *= 2;
-    return (int)std::min(b, full);
+    return (int)std::max<int64_t>(1, (int64_t)e->cu * per_cu);
 }
 
 // Queues the copy of a waited step's `total` records (in `used` chunks) to the
@@ -1530,12 +1529,16 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
     if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
     if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
     hipStream_t cs = sl.copy;
-    if ((size_t)total > sl.h_hits.cap) {
-        // the previous copy into this slot's pinned buffer must end before
-        // the buffer is replaced
+    if (2 * (size_t)total > sl.h_hits.cap) {
+        // grown four-fold once the records pass half of it, to at least 1M
+        // records (a pinned allocation stalls the caller for milliseconds
+        // per 100 MB: growing well ahead of the volume keeps that rare); the
+        // previous copy into this slot's pinned buffer must end before the
+        // buffer is replaced
         if (sl.copy_queued) HIP_TRY(hipEventSynchronize(sl.ev_copied));
         sl.copy_queued = false;
-        if ((rc = sl.h_hits.reserve((size_t)total))) return rc;
+        if ((rc = sl.h_hits.reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
+            return rc;
     }
     if ((uint64_t)total > sl.step_dense) {
         if ((rc = sl.d_dense.reserve_async((size_t)total, cs)) ||
@@ -1663,6 +1666,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     if (he2 != hipSuccess) return set_err(GSS_E_HIP, "step capture: %s", hipGetErrorString(he2));
     sl.graph = g;
     HIP_TRY(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    e->graph_captures.fetch_add(1, std::memory_order_relaxed);
     return GSS_OK;
 }
 
@@ -1703,6 +1707,10 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
 }
 
 }  // extern "C"
+
+int64_t gss::engine_graph_captures(const gss_engine *e) {
+    return e ? e->graph_captures.load(std::memory_order_relaxed) : 0;
+}
 
 int gss::engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,
                              const gss_var_patch *patches, uint64_t patch_cap,
@@ -1944,9 +1952,9 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
     const int blocks =
-        step_blocks(e, tasks, (serve && e->serve_prefilter) || screen ? 1
-                              : desc->records == GSS_RECORDS_CLAUSES   ? 2
-                                                                       : 0);
+        step_blocks(e, (serve && e->serve_prefilter) || screen ? 1
+                       : desc->records == GSS_RECORDS_CLAUSES   ? 2
+                                                                : 0);
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
     // (one record per clause in the clause form)
@@ -2139,6 +2147,11 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     sl.host_ready = false;
     sl.rec_total = tot;
     sl.res_words = sl.rec_words;
+    // the slot's next capture compacts into a buffer with room to spare once
+    // the records pass half of the current one (a step whose records do not
+    // fit is compacted, or grouped, outside the graph)
+    if (2 * (uint64_t)tot > sl.step_dense)
+        sl.dense_want = std::max<uint64_t>(sl.dense_want, 4 * (uint64_t)tot);
     e->last_step_slot = S;
     e->last_step_hits = tot;
     e->last_step_chunks = used;

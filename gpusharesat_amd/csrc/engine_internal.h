@@ -26,6 +26,8 @@ hipStream_t engine_stream(gss_engine *e);
 // Serving (gss_server): attaches the request and patch rings (fine-grained
 // pinned host memory the host publishes into, head = requests published) to
 // the engine; the device-side admission cursor starts at request 0.
+// Step graphs captured by the engine so far.
+int64_t engine_graph_captures(const gss_engine *e);
 int engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,
                         const gss_var_patch *patches, uint64_t patch_cap, const uint64_t *head);
 // gss_step_submit, or with serve a step that admits requests from the ring

@@ -113,6 +113,9 @@ struct gss_server {
     std::deque<int64_t> ready;
     int64_t records = 0, requests_done = 0;
     double checks = 0.0, device_ms = 0.0;
+    int64_t host_grouped = 0;  // steps grouped by this thread (under mu)
+    // host time of the serving thread's step submissions and waits (under mu)
+    double submit_ms_total = 0.0, submit_ms_max = 0.0, wait_ms_max = 0.0;
     bool stop = false;
     int err_code = 0;
     std::string err;
@@ -195,8 +198,12 @@ void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
     d.base_ups = bu.data();
     d.valid = valid;
     d.records = s->cfg.records;
+    const Clock::time_point t_sub = Clock::now();
     const int rc = gss::step_submit_ex(s->e, &d, true);
+    const double sub_ms = std::chrono::duration<double, std::milli>(Clock::now() - t_sub).count();
     lk.lock();
+    s->submit_ms_total += sub_ms;
+    s->submit_ms_max = std::max(s->submit_ms_max, sub_ms);
     if (rc) {
         set_failed(s, rc, "serving step");
         return;
@@ -226,22 +233,15 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     lk.unlock();
     int64_t n = 0;
     float ms = 0.f;
+    const Clock::time_point t_wait = Clock::now();
     int rc = gss_step_wait(s->e, &n, &ms);
+    const double wait_ms =
+        std::chrono::duration<double, std::milli>(Clock::now() - t_wait).count();
+    // what the step admitted and where its records of each word start (both
+    // written by the step's graph into pinned memory) are read before the
+    // next step is launched: it reuses this slot's pinned header
     ServeReport rep{};
     if (!rc) rc = gss::step_serve_report(s->e, &rep);
-    const uint64_t *raw = nullptr;
-    const int rw = s->cfg.records == GSS_RECORDS_IDS ? 1 : 3;
-    if (!rc && n > 0) {
-        if (rw == 1) {
-            const gss_hit_id *p = nullptr;
-            rc = gss_step_ids_map(s->e, &p, &n);
-            raw = (const uint64_t *)p;
-        } else {
-            const gss_hit *p = nullptr;
-            rc = gss_step_hits_map(s->e, &p, &n);
-            raw = (const uint64_t *)p;
-        }
-    }
     uint64_t goff[GSS_MAX_WORDS + 1];
     bool grouped = false;
     if (!rc && n > 0) rc = gss::step_word_groups(s->e, goff, &grouped);
@@ -263,6 +263,22 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
             launch(s, lk);
         lk.unlock();
     }
+    // the records themselves: waits for their copy to pinned memory, which
+    // the step just launched does not touch (its graph waits for that copy
+    // on the device before it reuses the slot's device buffers)
+    const uint64_t *raw = nullptr;
+    const int rw = s->cfg.records == GSS_RECORDS_IDS ? 1 : 3;
+    if (!rc && n > 0) {
+        if (rw == 1) {
+            const gss_hit_id *p = nullptr;
+            rc = gss_step_ids_map(s->e, &p, &n);
+            raw = (const uint64_t *)p;
+        } else {
+            const gss_hit *p = nullptr;
+            rc = gss_step_hits_map(s->e, &p, &n);
+            raw = (const uint64_t *)p;
+        }
+    }
     int owner[GSS_MAX_WORDS];
     uint32_t w0[kServeMaxRequests];
     std::fill(owner, owner + GSS_MAX_WORDS, -1);
@@ -279,6 +295,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     step->rw = rw;
     std::vector<size_t> first(cnt + 1, 0);
     std::vector<size_t> last(cnt, 0);
+    bool grouped_here = false;  // the records outgrew the device grouping
     if (order_ok && n > 0 && grouped) {
         // grouped by word on the device: each request's records are one
         // range of the engine's pinned buffer, lent until the slot's reuse
@@ -291,6 +308,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
             last[i] = goff[std::min<uint32_t>(w0[i] + nw, GSS_MAX_WORDS)];
         }
     } else if (order_ok && n > 0) {
+        grouped_here = true;
         std::vector<size_t> count(cnt, 0);
         for (int64_t r = 0; r < n; r++) {
             const uint32_t word = (uint32_t)(raw[r * rw] >> 48);
@@ -314,6 +332,8 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
         std::fill(first.begin(), first.end(), 0);
     }
     lk.lock();
+    if (grouped_here) s->host_grouped++;
+    s->wait_ms_max = std::max(s->wait_ms_max, wait_ms);
     const StepInfo si = s->inflight.front();
     s->inflight.pop_front();
     if (rc) {
@@ -650,6 +670,11 @@ int gss_server_stats_get(gss_server *s, gss_server_stats *out) {
     out->literals = s->literals_admitted;
     out->checks = s->checks;
     out->device_ms = s->device_ms;
+    out->host_grouped = s->host_grouped;
+    out->graph_captures = gss::engine_graph_captures(s->e);
+    out->submit_ms_total = s->submit_ms_total;
+    out->submit_ms_max = s->submit_ms_max;
+    out->wait_ms_max = s->wait_ms_max;
     return GSS_OK;
 }
 

@@ -188,6 +188,10 @@ class ServeResult:
     kept: dict = field(default_factory=dict)  # request id -> (result, records), sampled
     # (seconds, steps, device ms, clauses) of the server, about every 5 s
     samples: list = field(default_factory=list)
+    host_grouped: int = 0    # steps whose records the serving thread grouped itself
+    graph_captures: int = 0  # step graphs the engine captured
+    thread: dict = field(default_factory=dict)  # serving thread's submit / wait host times
+    lag_ms: np.ndarray = None  # arrival -> published in the ring (this driver's part)
 
     @property
     def checks_per_s(self) -> float:
@@ -220,7 +224,9 @@ class _Stats(ctypes.Structure):
                 ("requests_pending", ctypes.c_int64), ("records", ctypes.c_int64),
                 ("clauses", ctypes.c_int64), ("clauses_refused", ctypes.c_int64),
                 ("literals", ctypes.c_int64), ("checks", ctypes.c_double),
-                ("device_ms", ctypes.c_double)]
+                ("device_ms", ctypes.c_double), ("host_grouped", ctypes.c_int64),
+                ("graph_captures", ctypes.c_int64), ("submit_ms_total", ctypes.c_double),
+                ("submit_ms_max", ctypes.c_double), ("wait_ms_max", ctypes.c_double)]
 
 
 class GpuServer:
@@ -489,7 +495,7 @@ class Server:
         self.log(f"{len(reqs)} requests over {c.duration:.0f} s")
         by_id: dict[int, Request] = {}
         t_sub: dict[int, float] = {}
-        lat, arrive = [], []
+        lat, arrive, lags = [], [], []
         kept = {}
         st0 = srv.stats()
         ai = 0
@@ -507,7 +513,9 @@ class Server:
                 want = keep is not None and keep(r)
                 meta, recs = srv.result(rid, records=want)
                 srv.release(rid)
-                lat.append((t_sub.pop(rid) - r.t_arrive) * 1e3 + meta["latency_ms"])
+                lag = (t_sub.pop(rid) - r.t_arrive) * 1e3
+                lags.append(lag)
+                lat.append(lag + meta["latency_ms"])
                 arrive.append(r.t_arrive)
                 if want:
                     kept[rid] = (r, meta, recs)
@@ -555,7 +563,13 @@ class Server:
                            st["checks"] - st0["checks"], np.array(lat), st["clauses"],
                            st["literals"], st["records"] - st0["records"], max_queue,
                            st["device_ms"] - st0["device_ms"], np.array(arrive), kept,
-                           samples + [(elapsed, st["steps"], st["device_ms"], st["clauses"])])
+                           samples + [(elapsed, st["steps"], st["device_ms"], st["clauses"])],
+                           st["host_grouped"] - st0["host_grouped"],
+                           st["graph_captures"] - st0["graph_captures"],
+                           {"submit_ms_mean": (st["submit_ms_total"] - st0["submit_ms_total"])
+                            / max(1, st["steps"] - st0["steps"]),
+                            "submit_ms_max": st["submit_ms_max"],
+                            "wait_ms_max": st["wait_ms_max"]}, np.array(lags))
 
     def close(self):
         self.srv.close()

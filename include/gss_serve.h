@@ -82,6 +82,12 @@ typedef struct gss_server_stats {
     int64_t clauses, clauses_refused, literals;
     double checks;            /* literal x assignment checks of the steps so far */
     double device_ms;         /* sum of the steps' device times */
+    int64_t host_grouped;     /* steps whose records outgrew the device grouping buffer and
+                                 were grouped by the serving thread */
+    int64_t graph_captures;   /* step graphs captured (a new one whenever buffers grow) */
+    double submit_ms_total;   /* serving thread: host time staging and launching steps */
+    double submit_ms_max;     /* the longest of them */
+    double wait_ms_max;       /* the longest wait for a step to end */
 } gss_server_stats;
 
 /* Defaults: 64 words, 2 steps in flight, full records, 4096 requests and
