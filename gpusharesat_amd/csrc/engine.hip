@@ -48,6 +48,7 @@ using namespace gss;
 // k_chunk_offsets rewrites entries [0, chunks] each time, so nothing in it
 // needs clearing; two gathers that overlap on one device race.
 static constexpr uint32_t kGatherMaxChunks = 262144;  // 2 MiB + 8 B per device
+constexpr int kMaxHostRecordBuffers = 4;  // pinned record buffers per step slot
 static __device__ __align__(256) uint64_t scratch[kGatherMaxChunks + 1];
 
 namespace {
@@ -233,17 +234,21 @@ struct StepSlot {
     uint64_t dense_want = 0;     // records the graph's compaction should hold
     uint64_t graph_dense = 0;    // records the captured graph compacts into d_dense (0 = none)
     uint64_t step_dense = 0;     // the same, for the last submitted step
-    Pinned<gss_hit> h_hits;      // the same records on the host (gss_step_hits_map)
-    // The records of a waited step go to h_hits on the slot's own stream as
-    // soon as gss_step_wait has seen the step end (one DMA copy when the graph
-    // compacted them); the slot's next graph waits for that copy on the
-    // engine stream, so the records stay valid until the slot's next step is
-    // waited and the host can stage that step while they travel.
+    // The records of a waited step go to a pinned host buffer on the slot's
+    // own stream as soon as gss_step_wait has seen the step end (one DMA copy
+    // when the graph compacted them); the slot's next graph waits for that
+    // copy on the engine stream, so the records stay valid until the slot's
+    // next step is waited and the host can stage that step while they travel.
+    // With h_n > 1 buffers (serving engines) the waits rotate through them,
+    // and a step's records stay in place for h_n waits of the slot.
+    Pinned<gss_hit> h_ring[kMaxHostRecordBuffers];
+    int h_at = 0, h_n = 1;
+    Pinned<gss_hit> &h_hits() { return h_ring[h_at]; }
     hipStream_t copy = nullptr;
     hipEvent_t ev_copied = nullptr;
     bool copy_queued = false;
     int64_t rec_total = 0;  // records of the slot's last waited step
-    bool host_ready = false;     // h_hits holds the last waited step's records
+    bool host_ready = false;     // h_hits() holds the last waited step's records
     Pinned<ScatterSeg> h_cseg;   // compaction segments (one per non-empty chunk)
     DevBuf<ScatterSeg> d_cseg;
     uint32_t *d_alloc = nullptr;
@@ -576,7 +581,8 @@ int gss_engine_create(int device, gss_engine **out) {
     // copy endpoints only: staging of the plain path, the steps' records
     e->h_stage.page_limit = &e->page_limit;
     e->h_ups.page_limit = &e->page_limit;
-    for (StepSlot &sl : e->slots) sl.h_hits.page_limit = &e->page_limit;
+    for (StepSlot &sl : e->slots)
+        for (auto &h : sl.h_ring) h.page_limit = &e->page_limit;
     if ((rc = gss_assig_reset(e, 0, 1))) return fail(rc);
     *out = e;
     return GSS_OK;
@@ -1529,7 +1535,8 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
     if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
     if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
     hipStream_t cs = sl.copy;
-    if (2 * (size_t)total > sl.h_hits.cap) {
+    sl.h_at = (sl.h_at + 1) % sl.h_n;  // the buffer of the slot's h_n-th last wait
+    if (2 * (size_t)total > sl.h_hits().cap) {
         // grown four-fold once the records pass half of it, to at least 1M
         // records (a pinned allocation stalls the caller for milliseconds
         // per 100 MB: growing well ahead of the volume keeps that rare); the
@@ -1537,7 +1544,7 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
         // buffer is replaced
         if (sl.copy_queued) HIP_TRY(hipEventSynchronize(sl.ev_copied));
         sl.copy_queued = false;
-        if ((rc = sl.h_hits.reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
+        if ((rc = sl.h_hits().reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
             return rc;
     }
     if ((uint64_t)total > sl.step_dense) {
@@ -1560,7 +1567,7 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
         HIP_TRY(launch_scatter(sl.d_cseg.p, ns, (const uint32_t *)sl.d_hits.p, 0, cs));
         sl.dense_want = std::max<uint64_t>(sl.dense_want, 2 * (uint64_t)total);
     }
-    HIP_TRY(hipMemcpyAsync(sl.h_hits.p, sl.d_dense.p, (size_t)total * rw * sizeof(uint64_t),
+    HIP_TRY(hipMemcpyAsync(sl.h_hits().p, sl.d_dense.p, (size_t)total * rw * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, cs));
     HIP_TRY(hipEventRecord(sl.ev_copied, cs));
     sl.copy_queued = true;
@@ -1707,6 +1714,17 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
 }
 
 }  // extern "C"
+
+int gss::engine_set_record_buffers(gss_engine *e, int n) {
+    if (!e || n < 1 || n > kMaxHostRecordBuffers)
+        return set_err(GSS_E_INVAL, "record buffers per slot: 1 .. %d", kMaxHostRecordBuffers);
+    if (!e->inflight.empty()) return set_err(GSS_E_STATE, "steps in flight");
+    for (StepSlot &sl : e->slots) {
+        sl.h_n = n;
+        sl.h_at = std::min(sl.h_at, n - 1);
+    }
+    return GSS_OK;
+}
 
 int64_t gss::engine_graph_captures(const gss_engine *e) {
     return e ? e->graph_captures.load(std::memory_order_relaxed) : 0;
@@ -2191,7 +2209,7 @@ int gss_step_hits_map(gss_engine *e, const gss_hit **out, int64_t *n) {
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
-    *out = sl.h_hits.p;
+    *out = sl.h_hits().p;
     return GSS_OK;
 }
 
@@ -2209,7 +2227,7 @@ int gss_step_ids_map(gss_engine *e, const gss_hit_id **out, int64_t *n) {
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
-    *out = (const gss_hit_id *)sl.h_hits.p;
+    *out = (const gss_hit_id *)sl.h_hits().p;
     return GSS_OK;
 }
 
@@ -2227,7 +2245,7 @@ int gss_step_clauses_map(gss_engine *e, const gss_hit_clause **out, int64_t *n) 
     if (n) *n = total;
     if (!sl.host_ready && total > 0) HIP_TRY(hipEventSynchronize(sl.ev_copied));
     sl.host_ready = true;
-    *out = (const gss_hit_clause *)sl.h_hits.p;
+    *out = (const gss_hit_clause *)sl.h_hits().p;
     return GSS_OK;
 }
 

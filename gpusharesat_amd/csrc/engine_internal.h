@@ -26,6 +26,9 @@ hipStream_t engine_stream(gss_engine *e);
 // Serving (gss_server): attaches the request and patch rings (fine-grained
 // pinned host memory the host publishes into, head = requests published) to
 // the engine; the device-side admission cursor starts at request 0.
+// Pinned record buffers per step slot (1 .. 4): a waited step's records then
+// stay in place for that many waits of its slot (2n steps with two slots).
+int engine_set_record_buffers(gss_engine *e, int n);
 // Step graphs captured by the engine so far.
 int64_t engine_graph_captures(const gss_engine *e);
 int engine_serve_attach(gss_engine *e, const ServeReq *reqs, uint32_t req_cap,

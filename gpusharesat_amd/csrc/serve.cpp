@@ -32,6 +32,11 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Pinned record buffers per engine step slot: a step's records are lent from
+// them for 2 * kRecordBuffers steps (two slots), so a client that releases a
+// result within a few steps never costs a copy.
+constexpr int kRecordBuffers = 4;
+
 int fail(int code, const std::string &msg) { return gss::record_error(code, msg.c_str()); }
 
 // A published request, until its step completes.
@@ -58,9 +63,9 @@ struct StepInfo {
 // word made relative to the request).
 //
 // Grouped on the device, a step's records are lent straight from the engine's
-// pinned buffer (valid until that step's slot is waited for again, two steps
-// on); before that wait, the records of a step whose results a client still
-// holds unread are copied (materialize).
+// pinned buffers (valid for 2 * kRecordBuffers steps); before a step that may
+// reuse the buffer is waited for, the records of a step whose results a
+// client still holds unread are copied (materialize).
 struct StepRecords {
     std::vector<uint64_t> words;
     const uint64_t *lent = nullptr;  // engine pinned memory, while valid
@@ -223,10 +228,12 @@ void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
 // request, then copy grouped by request), outside the lock; clause ids are
 // looked up only when a client reads a request's records.
 void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
-    // the step about to be waited for reuses the slot of the step two before
-    // it: records lent from there that a client still holds are copied now
+    // the step about to be waited for may reuse the pinned record buffer of
+    // the step 2 * kRecordBuffers before it (its slot's kRecordBuffers-th
+    // last wait with records, at the earliest): records lent from there that a
+    // client still holds are copied now
     const int64_t next = s->inflight.front().index;
-    while (!s->lent.empty() && s->lent.front().first <= next - 2) {
+    while (!s->lent.empty() && s->lent.front().first <= next - 2 * kRecordBuffers) {
         if (auto sp = s->lent.front().second.lock()) sp->materialize();
         s->lent.pop_front();
     }
@@ -475,7 +482,10 @@ int gss_server_create(int device, const gss_server_config *c, gss_server **out) 
         return fail(GSS_E_NOMEM, "pinned ring allocation failed");
     }
     __atomic_store_n(s->head, (uint64_t)0, __ATOMIC_RELEASE);
-    if ((rc = gss::engine_serve_attach(s->e, s->reqs, (uint32_t)c->request_ring, s->patches,
+    // records lent to clients stay in the engine's pinned buffers for
+    // 2 * kRecordBuffers steps before a held one has to be copied
+    if ((rc = gss::engine_set_record_buffers(s->e, kRecordBuffers)) ||
+        (rc = gss::engine_serve_attach(s->e, s->reqs, (uint32_t)c->request_ring, s->patches,
                                        (uint64_t)c->patch_ring, s->head))) {
         const std::string m = gss_last_error();
         destroy(s);
