@@ -2016,7 +2016,11 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
             sl.h_out.cap ? sl.h_out.cap - 1 : 0);
     };
     if (chunks > chunk_room()) {
-        chunks *= 2;  // headroom for the DB growing over the next steps
+        // headroom for the DB growing over the next steps; a serving engine
+        // (records of every step, the DB growing for its whole life) starts
+        // at 64k chunks and grows four-fold, so that multi-gigabyte
+        // reallocations on its stream stay rare
+        chunks = serve ? std::max<uint64_t>(4 * chunks, 65536) : 2 * chunks;
         if ((rc = sl.d_hits.reserve_async((size_t)chunks * kHitChunk * rw, e->stream)) ||
             (rc = sl.d_fills.reserve_async((size_t)chunks, e->stream)) ||
             (rc = sl.h_out.reserve(chunks + 1)))
