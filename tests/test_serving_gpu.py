@@ -282,6 +282,65 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
         assert st["requests_done"] == 1 + len(burst) and st["requests_pending"] == 0
 
 
+@pytest.mark.parametrize("records", [sv.RECORDS_IDS, sv.RECORDS_FULL])
+def test_result_held_across_many_steps_keeps_its_records(records):
+    """A client holds a finished request's result unread while 24 more
+    requests run through their own steps: its records, lent from the
+    engine's pinned buffers, are copied before those buffers are reused, and
+    read late they still equal the oracle's on its explicit assignments (as
+    do the records of requests read at once)."""
+    rng = np.random.default_rng(23 + records)
+    V, W, N = 500, 16, 4000
+    lens = rng.integers(1, 13, size=N).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    off = np.zeros(N + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    bd, bt = random_bases(rng, 3, V)
+    full = records == sv.RECORDS_FULL
+
+    def check(meta, recs, solver, k, patches, what):
+        d, t = _explicit_request(bd[solver], bt[solver], patches, k)
+        exp = _oracle_request(off, lits, N, d, t, k)
+        got = _as_oracle(recs)
+        if full:
+            _same(got, exp, what)
+        else:  # id records: clause and word only
+            assert got.size == exp.size, (what, got.size, exp.size)
+            assert np.array_equal(got["clause"], exp["clause"]), what
+            assert np.array_equal(got["word"], exp["word"]), what
+
+    with sv.GpuServer(V, 3, words=W, records=records) as srv:
+        assert srv.add_clauses(lits, lens) == 0
+        srv.update_base(base_updates(bd, bt))
+
+        def req(k):
+            p = sv.slot_patches(rng, V, k, 6, 0.6)
+            return p[p["word"] < (k + 63) // 64]
+
+        def finish(rid):
+            while rid not in done:
+                done.update(srv.poll(64, 2_000_000))
+
+        done = set()
+        held_k = 200
+        held_p = req(held_k)
+        held = srv.submit(1, held_k, held_p)
+        finish(held)
+        for i in range(24):  # one request at a time: a step each
+            s_, k = i % 3, int(rng.integers(1, 400))
+            p = req(k)
+            rid = srv.submit(s_, k, p)
+            finish(rid)
+            meta, recs = srv.result(rid)
+            srv.release(rid)
+            check(meta, recs, s_, k, p, f"request {i}")
+        meta, recs = srv.result(held)
+        assert meta["clauses"] == N and recs.size > 0
+        check(meta, recs, 1, held_k, held_p, "held request read 24 steps later")
+        srv.release(held)
+        assert srv.stats()["steps"] >= 25
+
+
 def test_serving32_workload_sampled_requests_match_oracle():
     """serving-32 through the native loop for two seconds (32 solvers over
     2M variables, Poisson 2000 requests/s of U[64, 1024] assignments, clause
