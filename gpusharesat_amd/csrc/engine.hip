@@ -253,6 +253,15 @@ struct StepSlot {
     DevBuf<ScatterSeg> d_cseg;
     uint32_t *d_alloc = nullptr;
     uint32_t chunk_cap = 0;
+    // chunks that hold every record the step could make; a buffer sized by
+    // the records seen (chunk_cap < worst_chunks) may overflow, and the step
+    // is then recovered (its sweep re-run) before the table moves on
+    uint64_t worst_chunks = 0;
+    bool settled = false;       // the step in flight is known to have kept its records
+    bool records_lost = false;  // its recovery could not grow the buffer
+    // launch settings baked into the graph, for a recovery's re-run
+    int g_blocks = 0, g_tc = 0, g_fuse = 0, g_unroll = 0;
+    bool g_serve = false, g_screen = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -400,6 +409,14 @@ struct gss_engine {
     uint32_t last_step_chunks = 0;
     float last_step_ms = 0.f;
     std::atomic<int64_t> graph_captures{0};  // step graphs captured (read by other threads)
+    // Step hit buffers are sized by the records steps have made: twice the
+    // most chunks a step claimed (at least a floor), never beyond the worst
+    // case.  A step that overflows its buffer sets d_halt (device, zero while
+    // running); later steps in flight then stage nothing until its wait (or
+    // a plain call) re-runs its sweep with room and replays them.
+    uint32_t *d_halt = nullptr;
+    uint64_t step_hit_want = 0;
+    std::atomic<int64_t> step_recoveries{0};
     // host threads for the staging copies of large steps
     // (GPUSHARESAT_STAGE_THREADS, default 4; 0 or 1 = the calling thread only)
     int stage_threads = 4;
@@ -413,6 +430,12 @@ int check_engine(gss_engine *e) {
     return hipSetDevice(e->device) == hipSuccess ? GSS_OK
                                                  : set_err(GSS_E_HIP, "hipSetDevice failed");
 }
+
+}  // namespace
+
+static int settle_steps(gss_engine *e);
+
+namespace {
 
 // Fuse length the kernel runs with: <= 0 selects the variant without the
 // fused schedule.
@@ -574,8 +597,11 @@ int gss_engine_create(int device, gss_engine **out) {
     }
     if (hipMalloc((void **)&e->d_valid, GSS_MAX_WORDS * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc((void **)&e->d_alloc, 64) != hipSuccess ||
+        hipMalloc((void **)&e->d_halt, 64) != hipSuccess ||
         hipMalloc((void **)&e->d_lastpos, GSS_MAX_WORDS) != hipSuccess)
         return fail(set_err(GSS_E_NOMEM, "device allocation failed"));
+    if (hipMemset(e->d_halt, 0, 64) != hipSuccess)
+        return fail(set_err(GSS_E_HIP, "device memset failed"));
     if ((rc = e->h_valid.reserve(GSS_MAX_WORDS)) || (rc = e->h_lastpos.reserve(GSS_MAX_WORDS)))
         return fail(rc);
     // copy endpoints only: staging of the plain path, the steps' records
@@ -639,6 +665,7 @@ int gss_engine_destroy(gss_engine *e) {
     e->serve_dpatches.release();
     if (e->d_valid) (void)hipFree(e->d_valid);
     if (e->d_alloc) (void)hipFree(e->d_alloc);
+    if (e->d_halt) (void)hipFree(e->d_halt);
     if (e->d_lastpos) (void)hipFree(e->d_lastpos);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
@@ -659,6 +686,7 @@ int gss_engine_cu_count(gss_engine *e, int *out) {
 int gss_engine_sync(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     HIP_TRY(hipStreamSynchronize(e->stream));
     return GSS_OK;
 }
@@ -686,6 +714,7 @@ static int db_append(gss_engine *e, int64_t n, const uint32_t *lits, const int32
                      const uint32_t *tags, uint32_t *out_rows) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (n < 0 || (n > 0 && (!lits || !lens))) return set_err(GSS_E_INVAL, "bad clause batch");
     if (e->tagged && n > 0 && !tags) return set_err(GSS_E_INVAL, "tagged DB: tags required");
     if (n == 0) return GSS_OK;
@@ -792,6 +821,7 @@ int gss_db_append_tagged(gss_engine *e, int64_t n, const uint32_t *lits, const i
 int gss_db_enable_tags(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (e->tagged) return GSS_OK;
     if (e->total_clauses) return set_err(GSS_E_STATE, "tags must be enabled on an empty DB");
     for (Bucket &b : e->buckets)
@@ -804,6 +834,7 @@ int gss_db_enable_tags(gss_engine *e) {
 int gss_db_read_tags(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!e->tagged) return set_err(GSS_E_STATE, "the DB is not tagged");
     if (len < 1 || len >= (int)e->buckets.size() || row0 < 0 || n < 0 ||
         row0 + n > e->buckets[len].count || (n && !out))
@@ -932,6 +963,7 @@ int gss_db_compact(gss_engine *e, const uint8_t *keep, int64_t ntags) {
 int gss_db_set_bucket(gss_engine *e, int len, int64_t n, const uint32_t *lits) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (e->tagged) return set_err(GSS_E_STATE, "tagged DB: compact with gss_db_compact");
     if (len < 1 || len > GSS_MAX_CLAUSE_LEN || n < 0 || (n > 0 && !lits))
         return set_err(GSS_E_INVAL, "bad bucket replacement");
@@ -989,6 +1021,7 @@ int gss_db_totals(gss_engine *e, int64_t *clauses, int64_t *literals) {
 int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (n == 0 && row0 == 0 && len >= 1) return GSS_OK;  // nothing to read
     if (len < 1 || len >= (int)e->buckets.size() || row0 < 0 || n < 0 ||
         row0 + n > e->buckets[len].count || (n && !out))
@@ -1005,6 +1038,7 @@ int gss_db_read(gss_engine *e, int len, int64_t row0, int64_t n, uint32_t *out) 
 int gss_db_clear(gss_engine *e) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     for (Bucket &b : e->buckets) b.count = 0;
     e->rows_version++;
     e->state_version++;
@@ -1046,6 +1080,7 @@ static int write_sentinels(gss_engine *e) {
 int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (var_count < 0 || var_count > (1ll << 31) - 2 || words < 1 || words > GSS_MAX_WORDS ||
         log2_exact(words) < 0)
         return set_err(GSS_E_INVAL, "bad table shape (%lld vars, %d words; words must be a power "
@@ -1070,6 +1105,7 @@ int gss_assig_reset(gss_engine *e, int64_t var_count, int words) {
 int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (var_count < e->vars || var_count > (1ll << 31) - 2)
         return set_err(GSS_E_INVAL, "variable count can only grow");
     if ((rc = table_fits(var_count, e->words))) return rc;
@@ -1088,6 +1124,7 @@ int gss_assig_grow_vars(gss_engine *e, int64_t var_count) {
 int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_true) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!is_def || !is_true) return set_err(GSS_E_INVAL, "null plane");
     const int W = e->words;
     const int64_t V = e->vars;
@@ -1116,6 +1153,7 @@ int gss_assig_upload(gss_engine *e, const uint64_t *is_def, const uint64_t *is_t
 int gss_assig_download(gss_engine *e, uint64_t *is_def, uint64_t *is_true) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!is_def || !is_true) return set_err(GSS_E_INVAL, "null plane");
     const int W = e->words;
     const int64_t V = e->vars;
@@ -1151,6 +1189,7 @@ static int upload_updates(gss_engine *e, int64_t n, const gss_var_update *ups) {
 int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (n < 0 || (n && !ups)) return set_err(GSS_E_INVAL, "bad update batch");
     if (!n) {
         e->applied = 0;
@@ -1166,6 +1205,7 @@ int gss_assig_apply(gss_engine *e, int64_t n, const gss_var_update *ups) {
 int gss_assig_roll_forward_applied(gss_engine *e, const int8_t *last_pos) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!last_pos) return set_err(GSS_E_INVAL, "null last_pos");
     if (!e->applied) return GSS_OK;
     for (int w = 0; w < e->words; w++)
@@ -1186,6 +1226,7 @@ int gss_assig_roll_forward(gss_engine *e, int64_t n, const gss_var_update *ups,
                            const int8_t *last_pos) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (n < 0 || (n && (!ups || !last_pos))) return set_err(GSS_E_INVAL, "bad roll-forward");
     if (!n) return GSS_OK;
     for (int w = 0; w < e->words; w++)
@@ -1205,6 +1246,7 @@ int gss_assig_roll_forward(gss_engine *e, int64_t n, const gss_var_update *ups,
 int gss_sweep_async(gss_engine *e, const uint64_t *valid) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!valid) return set_err(GSS_E_INVAL, "null valid mask");
     if (e->pending) return set_err(GSS_E_STATE, "previous sweep not waited for");
     if (e->total_clauses > 0 && e->max_lit >= 2 * e->vars)
@@ -1449,6 +1491,7 @@ int gss_set_tuning(gss_engine *e, int key, int value) {
 int gss_autotune(gss_engine *e, const uint64_t *valid, int reps, int *best, float *best_ms) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (!valid || reps < 1) return set_err(GSS_E_INVAL, "bad autotune arguments");
     if (e->pending) return set_err(GSS_E_STATE, "a sweep is in flight");
     struct Cfg {
@@ -1574,6 +1617,57 @@ static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
     return GSS_OK;
 }
 
+// Chunks a step's hit buffer can hold (the smallest of its parts).
+static uint32_t chunk_room(const StepSlot &sl, int rw) {
+    return (uint32_t)std::min<size_t>(
+        std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / ((size_t)kHitChunk * rw)),
+        sl.h_out.cap ? sl.h_out.cap - 1 : 0);
+}
+
+// The part of a step after its table updates, over the slot's buffers and
+// the settings of its graph: the sweep, the roll forward (held back if the
+// records overflowed), the chunk fills to the host and the records made
+// dense on the device right after the sweep, so that reading them on the
+// host is one DMA copy (no compute units taken from the next step's sweep;
+// serving steps: grouped by assignment word, so each request's records are
+// one range).
+static hipError_t launch_step_tail(gss_engine *e, StepSlot &sl, hipStream_t s) {
+    StepHeader *dh = sl.d_hdr.p;
+    SweepArgs a{};
+    a.buckets = sl.d_desc.p;
+    a.dyn = &dh->nbuckets;  // {nbuckets, ntasks}
+    a.task_clauses = sl.g_tc;
+    a.table = e->table.p;
+    a.valid = dh->valid;
+    a.hits = (gss_hit *)sl.d_hits.p;
+    a.fills = sl.d_fills.p;
+    a.alloc = sl.d_alloc;
+    a.chunk_cap = sl.chunk_cap;
+    a.fuse_len = sl.g_fuse;
+    a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
+    a.rec_words = sl.rec_words;
+    uint8_t *agg =
+        (sl.g_serve && e->serve_prefilter) || sl.g_screen ? e->serve_agg.p : nullptr;
+    a.agg = agg;
+    a.agg_words = agg ? e->serve_agg_words.p : nullptr;
+    a.agg_vote = e->serve_vote;
+    hipError_t he = launch_reset_u32(sl.d_alloc, 1, s);
+    if (he == hipSuccess) he = launch_sweep(a, e->words, sl.g_unroll, sl.g_blocks, s);
+    if (he == hipSuccess)
+        he = launch_assig_roll_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, dh->last_pos,
+                                   sl.d_alloc, sl.chunk_cap, e->d_halt, s);
+    if (he == hipSuccess)
+        he = launch_stage_out(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.h_out.p, s);
+    if (he == hipSuccess && sl.d_dense.cap > 0)
+        he = sl.g_serve ? launch_group_by_word(sl.d_alloc, sl.d_fills.p, sl.chunk_cap,
+                                               sl.d_hits.p, sl.rec_words, sl.d_wcount.p,
+                                               sl.d_dense.p, sl.d_dense.cap, sl.h_group.p, s)
+                        : launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap,
+                                                 sl.d_offs.p, sl.d_hits.p, sl.d_dense.p,
+                                                 sl.d_dense.cap, sl.rec_words, s);
+    return he;
+}
+
 // Captures the step's fixed launch sequence over this slot's buffers.
 static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse,
                             bool serve, bool screen) {
@@ -1581,6 +1675,12 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
     sl.exec = nullptr;
     sl.graph = nullptr;
+    sl.g_blocks = blocks;
+    sl.g_tc = tc;
+    sl.g_fuse = fuse;
+    sl.g_unroll = e->tune_unroll;
+    sl.g_serve = serve;
+    sl.g_screen = screen;
     if (!e->capture_stream)
         HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
     hipStream_t cs = e->capture_stream;
@@ -1602,24 +1702,9 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     st.d[kRegPatches] = (uint32_t *)sl.d_patches.p;
     st.d[kRegBaseUps] = (uint32_t *)sl.d_bups.p;
     st.d[kRegLoads] = (uint32_t *)sl.d_loads.p;
+    st.halt = e->d_halt;
     StepHeader *dh = sl.d_hdr.p;
-    SweepArgs a{};
-    a.buckets = sl.d_desc.p;
-    a.dyn = &dh->nbuckets;  // {nbuckets, ntasks}
-    a.task_clauses = tc;
-    a.table = e->table.p;
-    a.valid = dh->valid;
-    a.hits = (gss_hit *)sl.d_hits.p;
-    a.fills = sl.d_fills.p;
-    a.alloc = sl.d_alloc;
-    a.chunk_cap = sl.chunk_cap;
-    a.fuse_len = fuse;
-    a.sentinel = (uint32_t)(2 * e->vars) << kLitShift;
-    a.rec_words = sl.rec_words;
     uint8_t *agg = (serve && e->serve_prefilter) || screen ? e->serve_agg.p : nullptr;
-    a.agg = agg;
-    a.agg_words = agg ? e->serve_agg_words.p : nullptr;
-    a.agg_vote = e->serve_vote;
     hipError_t he = hipSuccess;
     if (he == hipSuccess) he = launch_stage_in(st, cs);
     if (he == hipSuccess)
@@ -1645,25 +1730,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     // batched steps: the aggregate of the table as the sweep will read it
     if (he == hipSuccess && screen)
         he = launch_table_agg(e->table.p, dh, e->vars, e->words, agg, e->serve_agg_words.p, cs);
-    if (he == hipSuccess) he = launch_reset_u32(sl.d_alloc, 1, cs);
-    if (he == hipSuccess) he = launch_sweep(a, e->words, e->tune_unroll, blocks, cs);
-    if (he == hipSuccess)
-        he = launch_assig_roll_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, dh->last_pos,
-                                   cs);
-    if (he == hipSuccess)
-        he = launch_stage_out(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.h_out.p, cs);
-    // records made dense on the device right after the sweep, so that reading
-    // them on the host is one DMA copy (no compute units taken from the next
-    // step's sweep)
-    // (serving steps: grouped by assignment word, so each request's records
-    // are one range)
-    if (he == hipSuccess && sl.d_dense.cap > 0)
-        he = serve ? launch_group_by_word(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_hits.p,
-                                          sl.rec_words, sl.d_wcount.p, sl.d_dense.p,
-                                          sl.d_dense.cap, sl.h_group.p, cs)
-                   : launch_compact_records(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.d_offs.p,
-                                            sl.d_hits.p, sl.d_dense.p, sl.d_dense.cap,
-                                            sl.rec_words, cs);
+    if (he == hipSuccess) he = launch_step_tail(e, sl, cs);
     hipGraph_t g = nullptr;
     hipError_t he2 = hipStreamEndCapture(cs, &g);
     if (he != hipSuccess) {
@@ -1674,6 +1741,86 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     sl.graph = g;
     HIP_TRY(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
     e->graph_captures.fetch_add(1, std::memory_order_relaxed);
+    return GSS_OK;
+}
+
+// Recovers step `at` of the steps in flight, whose sweep claimed `used`
+// chunks, more than its buffer held (the steps before it kept their
+// records).  Its graph left the table as the sweep read it and halted the
+// engine, so the later step in flight (if any) staged nothing.  Grows the
+// buffer to what the sweep claimed, re-runs the step's sweep and what follows
+// it, and replays the later step.  If the buffer cannot grow, the table
+// still rolls forward and the later step is replayed, but this step's
+// records are lost (its wait says so).
+static int recover_step(gss_engine *e, size_t at, uint32_t used) {
+    StepSlot &sl = e->slots[e->inflight[at]];
+    hipStream_t es = e->stream;
+    HIP_TRY(hipStreamSynchronize(es));
+    // a later step's submit may have moved bucket arrays (growth copies the
+    // rows in stream order): the step's rows [0, count) are the same rows
+    // there
+    const int nb = sl.h_hdr.p->nbuckets;
+    for (int i = 0; i < nb; i++) sl.h_desc.p[i].lits = e->buckets[sl.h_desc.p[i].len].d;
+    if (nb)
+        HIP_TRY(hipMemcpyAsync(sl.d_desc.p, sl.h_desc.p, (size_t)nb * sizeof(BucketDesc),
+                               hipMemcpyHostToDevice, es));
+    const int rw = sl.rec_words;
+    const uint64_t want = (uint64_t)used + used / 8 + 16;
+    int rc = GSS_OK;
+    if (want > chunk_room(sl, rw) &&
+        ((rc = sl.d_hits.reserve_async((size_t)want * kHitChunk * rw, es)) ||
+         (rc = sl.d_fills.reserve_async((size_t)want, es)) || (rc = sl.h_out.reserve(want + 1))))
+        sl.records_lost = true;
+    if (!sl.records_lost) {
+        sl.chunk_cap = chunk_room(sl, rw);
+        if ((rc = sl.d_offs.reserve_async((size_t)sl.chunk_cap + 1, es))) sl.records_lost = true;
+    }
+    HIP_TRY(launch_reset_u32(e->d_halt, 1, es));
+    if (sl.records_lost) {
+        // no room: roll the table forward (the gate passes with no limit)
+        HIP_TRY(launch_assig_roll_dyn(sl.d_ups.p, &sl.d_hdr.p->nups, e->table.p, e->words,
+                                      sl.d_hdr.p->last_pos, sl.d_alloc, UINT32_MAX, e->d_halt,
+                                      es));
+    } else {
+        HIP_TRY(launch_step_tail(e, sl, es));
+    }
+    HIP_TRY(hipEventRecord(sl.t1, es));
+    // the later step, which staged nothing, runs again
+    for (size_t i = at + 1; i < e->inflight.size(); i++) {
+        StepSlot &nx = e->slots[e->inflight[i]];
+        HIP_TRY(hipEventRecord(nx.t0, es));
+        HIP_TRY(hipGraphLaunch(nx.exec, es));
+        HIP_TRY(hipEventRecord(nx.t1, es));
+        nx.settled = false;
+    }
+    e->step_hit_want = std::max<uint64_t>(e->step_hit_want, 2 * (uint64_t)used);
+    e->step_recoveries++;
+    return sl.records_lost ? set_err(GSS_E_NOMEM, "a step's hit buffer overflowed (%u chunks) and "
+                                                  "could not grow: its records are lost",
+                                     used)
+                           : GSS_OK;
+}
+
+// Plain calls that read or change the clause DB or the assignment table
+// while steps are in flight first make sure that no step in flight has
+// halted the engine (else they would run between that step and its
+// recovery): a step whose buffer could overflow is waited for on the device
+// and recovered if it did.  Free when every buffer holds the worst case.
+static int settle_steps(gss_engine *e) {
+    for (size_t i = 0; i < e->inflight.size(); i++) {
+        StepSlot &sl = e->slots[e->inflight[i]];
+        if (sl.settled) continue;
+        if (sl.chunk_cap < sl.worst_chunks && !sl.records_lost) {
+            HIP_TRY(hipEventSynchronize(sl.t1));
+            const uint32_t used = sl.h_out.p[0];
+            if (used > sl.chunk_cap) {
+                int rc = recover_step(e, i, used);
+                if (rc && !sl.records_lost) return rc;
+                HIP_TRY(hipEventSynchronize(sl.t1));
+            }
+        }
+        sl.settled = true;
+    }
     return GSS_OK;
 }
 
@@ -1711,6 +1858,17 @@ int gss_bases_reset(gss_engine *e, int nbases) {
 
 int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     return gss::step_submit_ex(e, desc, false);
+}
+
+int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks) {
+    if (!e) return set_err(GSS_E_INVAL, "null engine");
+    if (reruns) *reruns = e->step_recoveries.load(std::memory_order_relaxed);
+    if (hit_chunks) {
+        int64_t c = 0;
+        for (const StepSlot &sl : e->slots) c += std::min(sl.d_fills.cap, sl.h_out.cap);
+        *hit_chunks = c;
+    }
+    return GSS_OK;
 }
 
 }  // extern "C"
@@ -1975,11 +2133,11 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
                                                                 : 0);
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
-    // (one record per clause in the clause form)
+    // (one record per clause in the clause form) fit in `worst` chunks
     const uint64_t ub = (uint64_t)(e->total_clauses + n) *
                         (uint64_t)(desc->records == GSS_RECORDS_CLAUSES ? std::min(active_words, 1)
                                                                          : active_words);
-    uint64_t chunks = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
+    const uint64_t worst = ub / (kHitChunk - 64) + 1 + (uint64_t)blocks * 4;
     if ((rc = sl.h_hdr.reserve(1)) || (rc = sl.h_cls.reserve(std::max<int64_t>(n, 1))) ||
         (rc = sl.h_desc.reserve(std::max(nb, 1))) ||
         (rc = sl.h_patches.reserve(std::max<int64_t>(n_patches, 1))) ||
@@ -2008,25 +2166,28 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (rc = sl.d_bups.reserve_async(sl.h_bups.cap, es)) ||
         (rc = sl.d_loads.reserve_async(std::max<size_t>(sl.h_loads.cap, GSS_MAX_WORDS), es)))
         return rc;
-    // the hit buffer holds chunks of records of this step's form
+    // The hit buffer holds chunks of records of this step's form, as many as
+    // twice the most a step has claimed (at least a floor: a partial chunk
+    // per wave and 4M records, 16M for a serving engine), never more than the
+    // worst case; it grows to twice that, so that a growing DB regrows it
+    // rarely.  A step whose records overflow it is recovered (settle_steps,
+    // recover_step).
     const int rw = desc->records == GSS_RECORDS_IDS ? 1 : desc->records == GSS_RECORDS_CLAUSES ? 2 : 3;
-    auto chunk_room = [&] {
-        return (uint32_t)std::min<size_t>(
-            std::min<size_t>(sl.d_fills.cap, sl.d_hits.cap / ((size_t)kHitChunk * rw)),
-            sl.h_out.cap ? sl.h_out.cap - 1 : 0);
-    };
-    if (chunks > chunk_room()) {
-        // headroom for the DB growing over the next steps; a serving engine
-        // (records of every step, the DB growing for its whole life) starts
-        // at 64k chunks and grows four-fold, so that multi-gigabyte
-        // reallocations on its stream stay rare
-        chunks = serve ? std::max<uint64_t>(4 * chunks, 65536) : 2 * chunks;
+    const uint64_t floor_chunks = (serve ? 16384 : 4096) + (uint64_t)blocks * 4;
+    uint64_t want = std::min(worst, std::max(floor_chunks, e->step_hit_want));
+    if (e->tune_max_chunks > 0) want = std::min<uint64_t>(want, e->tune_max_chunks);
+    if (want > chunk_room(sl, rw)) {
+        const uint64_t chunks = e->tune_max_chunks > 0 ? want : 2 * want;
         if ((rc = sl.d_hits.reserve_async((size_t)chunks * kHitChunk * rw, e->stream)) ||
             (rc = sl.d_fills.reserve_async((size_t)chunks, e->stream)) ||
             (rc = sl.h_out.reserve(chunks + 1)))
             return rc;
     }
-    sl.chunk_cap = chunk_room();
+    // (GSS_TUNE_MAX_HIT_CHUNKS holds every step to that many chunks)
+    sl.chunk_cap = e->tune_max_chunks > 0
+                       ? (uint32_t)std::min<uint64_t>(chunk_room(sl, rw), e->tune_max_chunks)
+                       : chunk_room(sl, rw);
+    sl.worst_chunks = worst;
     sl.rec_words = rw;
     if ((rc = sl.d_offs.reserve_async((size_t)sl.chunk_cap + 1, es))) return rc;
     if (sl.dense_want > sl.d_dense.cap && (rc = sl.d_dense.reserve_async(sl.dense_want, es)))
@@ -2137,6 +2298,8 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));
     sl.busy = true;
+    sl.settled = false;
+    sl.records_lost = false;
     sl.serve = serve;
     // the previous records of this slot stay readable only if their copy
     // was queued (or there were none)
@@ -2155,12 +2318,28 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     const int S = e->inflight.front();
     StepSlot &sl = e->slots[S];
     HIP_TRY(hipEventSynchronize(sl.t1));
+    uint32_t used = sl.h_out.p[0];
+    if (used > sl.chunk_cap && !sl.records_lost) {
+        // the records overflowed the buffer: re-run the sweep with room
+        rc = recover_step(e, 0, used);
+        if (rc && !sl.records_lost) {
+            e->inflight.erase(e->inflight.begin());
+            sl.busy = false;
+            return rc;
+        }
+        HIP_TRY(hipEventSynchronize(sl.t1));
+        used = sl.h_out.p[0];
+    }
     e->inflight.erase(e->inflight.begin());
     sl.busy = false;
-    const uint32_t used = sl.h_out.p[0];
+    if (sl.records_lost)
+        return set_err(GSS_E_NOMEM, "the step's hit buffer overflowed and could not grow: its "
+                                    "records are lost (the table did roll forward)");
     if (used > sl.chunk_cap)
         return set_err(GSS_E_NOMEM, "step hit buffer overflow (%u > %u chunks)", used,
                        sl.chunk_cap);
+    // the next steps' buffers hold twice the most chunks a step has claimed
+    e->step_hit_want = std::max<uint64_t>(e->step_hit_want, 2 * (uint64_t)used);
     int64_t tot = 0;
     for (uint32_t c = 0; c < used; c++) tot += sl.h_out.p[1 + c];
     float ms = 0.f;
@@ -2256,6 +2435,7 @@ int gss_step_clauses_map(gss_engine *e, const gss_hit_clause **out, int64_t *n) 
 int gss_db_reserve(gss_engine *e, int64_t n, const int32_t *lens) {
     int rc = check_engine(e);
     if (rc) return rc;
+    if ((rc = settle_steps(e))) return rc;
     if (n < 0 || (n > 0 && !lens)) return set_err(GSS_E_INVAL, "bad clause batch");
     if (e->tagged && n > 0) return set_err(GSS_E_STATE, "tagged DB");
     int maxlen = 0;

@@ -70,7 +70,9 @@ struct StepHeader {
     int32_t nbuckets;
     int32_t ntasks;
     int32_t nclauses;
-    int32_t pad0;
+    // 1 in the device copy when the step found the engine halted: every
+    // count is then 0 and the step does nothing (it is replayed later)
+    int32_t halted;
     int64_t nups, npatches, nbups, nloads;
     int64_t region_words[kStageRegions];  // uint32 words of each region to copy
     uint64_t valid[GSS_MAX_WORDS];
@@ -80,6 +82,10 @@ struct StepHeader {
 struct StageArgs {
     const uint32_t *h[kStageRegions];  // pinned host (h[kRegHeader] = the header)
     uint32_t *d[kStageRegions];        // device
+    // device word, nonzero while the engine is halted: an earlier step's
+    // records overflowed its hit buffer and its sweep is to be re-run before
+    // the table moves on
+    const uint32_t *halt;
 };
 
 struct ScatterSeg {
@@ -198,7 +204,11 @@ hipError_t launch_assig_roll(const gss_var_update *ups, int64_t n, uint64_t *tab
 hipError_t launch_fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
 
 // Fixed-shape launches of a batched step (counts read from the device header,
-// so a captured graph can be replayed with new data).
+// so a captured graph can be replayed with new data).  While the engine is
+// halted k_stage_in writes a header with no work and `halted` set instead of
+// staging: every later kernel of the step then has nothing to do (the
+// serving kernels and k_table_agg, whose work is not counted by the header,
+// test `halted` themselves).
 hipError_t launch_stage_in(const StageArgs &a, hipStream_t s);
 // Copies each staged clause (n read on the device) to its bucket row,
 // shifting literals left by `shift`.
@@ -206,8 +216,13 @@ hipError_t launch_scatter_clauses(const ClauseSeg *cls, const int32_t *n, const 
                                   int shift, hipStream_t s);
 hipError_t launch_assig_apply_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
                                   int words, hipStream_t s);
+// Rolls the step's updated entries forward -- unless the sweep before it
+// claimed more than `cap` hit chunks: then the table stays as the sweep read
+// it and *halt is set, so that the sweep can be re-run with room for every
+// record.
 hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
-                                 int words, const int8_t *last_pos, hipStream_t s);
+                                 int words, const int8_t *last_pos, const uint32_t *alloc,
+                                 uint32_t cap, uint32_t *halt, hipStream_t s);
 hipError_t launch_stage_out(const uint32_t *alloc, const uint32_t *fills, uint32_t cap,
                             uint32_t *h_out, hipStream_t s);
 // Chunks claimed and records of a sweep to pinned host memory: h_out[0] =

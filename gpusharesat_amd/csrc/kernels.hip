@@ -608,6 +608,16 @@ __global__ void k_assig_roll(const gss_var_update *__restrict__ ups, int64_t n,
 // Copies a step's staged inputs from pinned host memory to the device.  The
 // region sizes are read once per block from the pinned header.
 __global__ void k_stage_in(StageArgs a) {
+    if (*a.halt) {
+        // halted: a header with no work (and `halted` set) instead of the step
+        if (blockIdx.x == 0) {
+            uint32_t *d = a.d[kRegHeader];
+            constexpr int kFlag = (int)(offsetof(StepHeader, halted) / 4);
+            for (int i = (int)threadIdx.x; i < (int)(sizeof(StepHeader) / 4); i += (int)blockDim.x)
+                d[i] = i == kFlag ? 1u : 0u;
+        }
+        return;
+    }
     __shared__ int64_t n[kStageRegions];
     if (threadIdx.x < kStageRegions)
         n[threadIdx.x] = ((const StepHeader *)a.h[kRegHeader])->region_words[threadIdx.x];
@@ -699,7 +709,13 @@ __global__ void k_assig_apply_dyn(const gss_var_update *__restrict__ ups, const 
 
 __global__ void k_assig_roll_dyn(const gss_var_update *__restrict__ ups, const int64_t *n_p,
                                  uint64_t *__restrict__ table, int words,
-                                 const int8_t *__restrict__ last_pos) {
+                                 const int8_t *__restrict__ last_pos, const uint32_t *alloc,
+                                 uint32_t cap, uint32_t *halt) {
+    if (*alloc > cap) {
+        // records were dropped: keep the table as the sweep read it, and halt
+        if (blockIdx.x == 0 && threadIdx.x == 0) *halt = 1u;
+        return;
+    }
     const int64_t n = *n_p;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -894,6 +910,7 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, int lane) {
 __global__ __launch_bounds__(64) void k_serve_admit(ServeRing r, StepHeader *dh,
                                                     gss_word_load *loads, ServeReport *report,
                                                     int words) {
+    if (dh->halted) return;  // the ring waits for the step's replay
     const int lane = (int)threadIdx.x;
     __shared__ uint64_t s_head;
     if (lane == 0) {
@@ -990,6 +1007,7 @@ __global__ void k_table_agg(const uint64_t *__restrict__ table, const StepHeader
                             uint64_t *__restrict__ agg_words) {
     constexpr int NG = W < 8 ? W : 8;  // word groups
     constexpr int GW = W / NG;         // words per group
+    if (dh->halted) return;            // the aggregate stays the halted step's
     __shared__ uint64_t s_valid[W];
     if ((int)threadIdx.x < W) s_valid[threadIdx.x] = dh->valid[threadIdx.x];
     __syncthreads();
@@ -1025,6 +1043,7 @@ __global__ void k_serve_agg(ServeRing r, const StepHeader *__restrict__ dh,
                             const gss_word_load *__restrict__ loads,
                             const uint64_t *__restrict__ bases, int64_t vw, int64_t vars,
                             uint8_t *__restrict__ agg, uint64_t *__restrict__ agg_words) {
+    if (dh->halted) return;  // the aggregate stays the halted step's
     __shared__ uint32_t s_base[kServeMaxRequests];
     const int cnt = (int)*r.adm_count;
     if ((int)threadIdx.x < cnt) s_base[threadIdx.x] = loads[r.adm[threadIdx.x].w0].base;
@@ -1308,9 +1327,10 @@ hipError_t launch_assig_apply_dyn(const gss_var_update *ups, const int64_t *n, u
 }
 
 hipError_t launch_assig_roll_dyn(const gss_var_update *ups, const int64_t *n, uint64_t *table,
-                                 int words, const int8_t *last_pos, hipStream_t s) {
+                                 int words, const int8_t *last_pos, const uint32_t *alloc,
+                                 uint32_t cap, uint32_t *halt, hipStream_t s) {
     hipLaunchKernelGGL(k_assig_roll_dyn, dim3(1024), dim3(256), 0, s, ups, n, table, words,
-                       last_pos);
+                       last_pos, alloc, cap, halt);
     return hipGetLastError();
 }
 

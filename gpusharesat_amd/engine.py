@@ -339,6 +339,13 @@ class Engine:
         buf = (ctypes.c_char * (n.value * HIT_CLAUSE_DTYPE.itemsize)).from_address(p.value)
         return np.frombuffer(buf, dtype=HIT_CLAUSE_DTYPE)
 
+    def step_stats(self) -> dict:
+        """Steps re-run because their records overflowed the hit buffer, and
+        the chunks of 1024 records the step hit buffers hold."""
+        r, c = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("gss_step_stats", self._h, ctypes.byref(r), ctypes.byref(c))
+        return {"reruns": r.value, "hit_chunks": c.value}
+
     def step_hits(self) -> np.ndarray:
         n = ctypes.c_int64()
         _lib.call("gss_step_hits_copy", self._h, None, 0, ctypes.byref(n))

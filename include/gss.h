@@ -243,8 +243,10 @@ int gss_set_tuning(gss_engine *e, int key, int value);
 #define GSS_TUNE_WAVES_PER_CU 1   /* persistent waves per CU (0 = occupancy) */
 #define GSS_TUNE_TASK_CLAUSES 2   /* clauses per task */
 #define GSS_TUNE_MAX_HIT_CHUNKS 3 /* cap on the hit buffer a sweep starts with
-                                     (0 = sized to the worst case); a sweep
-                                     that needs more re-runs with a larger one */
+                                     (0 = sized to the worst case, for steps
+                                     by the records seen); a sweep that needs
+                                     more re-runs with a larger one.  Steps are
+                                     held to it on every submit */
 #define GSS_TUNE_UNROLL 4         /* table rows in flight per lane: 1, 2, 4 or 8
                                      at 16 words (default 4), 2 or 4 at 32
                                      (default 4), 1, 2 or 4 at 64 (default 2);
@@ -334,8 +336,22 @@ int gss_bases_reset(gss_engine *e, int nbases);
  * start travelling to pinned host memory at once, on a stream of the slot's
  * own, while the caller goes on (typically enqueueing the next step); the
  * slot's next step does not overwrite them on the device before that copy is
- * done. */
+ * done.
+ * A step's hit buffer holds twice the most records a step of the engine has
+ * made (at least 4M records, 16M on a serving engine), never more than
+ * clauses x valid words.  A step whose records overflow it keeps the table
+ * as its sweep read it and halts the engine: the step in flight behind it
+ * then stages nothing.  Its wait -- or a call that reads or changes the DB
+ * or the table while it is in flight -- grows the buffer, re-runs the sweep
+ * and the roll forward, and replays the later step, so no record is lost
+ * (GSS_E_NOMEM if the buffer cannot grow: the step's records are lost, the
+ * table still rolls forward).  The reference drops the reports past a full
+ * buffer (Reporter.cuh:46-48) and doubles it for later runs
+ * (GpuRunner.cu:363-365). */
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
+/* Steps re-run because their records overflowed the hit buffer, and the
+ * chunks of 1024 records the step hit buffers of the engine hold. */
+int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks);
 /* Copy the hit records of the last waited step to host (up to cap; *n =
  * total), waiting for their transfer if it is still running.  Valid until
  * the next gss_step_wait of that step's slot (two waits later), also when

@@ -13,12 +13,14 @@ loads and patches) and the native serving loop (include/gss_serve.h).
   growth) for two seconds: sampled requests' records equal the oracle's on
   the DB and base their step had.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 import _oracle as orc
-from gpusharesat_amd import serving as sv
-from gpusharesat_amd.engine import ClauseMap
+from gpusharesat_amd import _lib, serving as sv
+from gpusharesat_amd.engine import TUNE_MAX_HIT_CHUNKS, ClauseMap
 
 pytestmark = pytest.mark.gpu
 
@@ -220,15 +222,18 @@ def _same(got, exp, what):
         assert np.array_equal(got[f], exp[f]), (what, f)
 
 
-@pytest.mark.parametrize("prefilter", ["1", "0"])
-def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilter):
+@pytest.mark.parametrize("prefilter, max_chunks", [("1", 0), ("0", 0), ("1", 1)])
+def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilter, max_chunks):
     """Through the C ABI: request R alone, then copies of R inside a burst
     of other requests (admitted together on the device, at other word
     offsets): every copy's records equal R alone, and the oracle's on R's
     explicit assignments.  With and without the serving steps' aggregate
     prefilter (GPUSHARESAT_SERVE_PREFILTER, read when the server's engine is
     created); the patches make literals false in many words, so the
-    prefilter's rows are cleared by patches as well as set from the bases."""
+    prefilter's rows are cleared by patches as well as set from the bases.
+    With one hit chunk per step every step with records overflows: its wait
+    re-runs its sweep and replays the step launched behind it, which found
+    the engine halted and admitted nothing."""
     monkeypatch.setenv("GPUSHARESAT_SERVE_PREFILTER", prefilter)
     rng = np.random.default_rng(21)
     V, W, N = 500, 16, 4000
@@ -238,6 +243,8 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
     np.cumsum(lens, out=off[1:])
     bd, bt = random_bases(rng, 3, V)
     with sv.GpuServer(V, 3, words=W) as srv:
+        if max_chunks:  # before the first step
+            _lib.call("gss_set_tuning", srv.engine_handle(), TUNE_MAX_HIT_CHUNKS, max_chunks)
         assert srv.add_clauses(lits, lens) == 0
         srv.update_base(base_updates(bd, bt))
 
@@ -280,6 +287,11 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
         assert mixed >= 1, steps
         st = srv.stats()
         assert st["requests_done"] == 1 + len(burst) and st["requests_pending"] == 0
+        # (the serving thread is idle: every request is done)
+        reruns, chunks = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("gss_step_stats", srv.engine_handle(), ctypes.byref(reruns),
+                  ctypes.byref(chunks))
+        assert (reruns.value > 0) == bool(max_chunks), reruns.value
 
 
 @pytest.mark.parametrize("records", [sv.RECORDS_IDS, sv.RECORDS_FULL])

@@ -16,7 +16,8 @@ import pytest
 import _oracle as orc
 from gpusharesat_amd import workloads
 from gpusharesat_amd._lib import GssError
-from gpusharesat_amd.engine import ClauseMap, Engine, TUNE_FUSE_LEN, VAR_UPDATE_DTYPE
+from gpusharesat_amd.engine import (ClauseMap, Engine, TUNE_FUSE_LEN, TUNE_MAX_HIT_CHUNKS,
+                                    VAR_UPDATE_DTYPE)
 
 pytestmark = pytest.mark.gpu
 
@@ -95,7 +96,7 @@ class Plain:
 
 def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
               grow_at=None, eager=True, tuning=(), page_limit=None, screen=False,
-              uniform=False):
+              uniform=False, read_in_flight=False):
     rng = np.random.default_rng(seed)
     a, b = engine_factory(), engine_factory()
     for key, val in tuning:
@@ -147,10 +148,20 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
                 da, ta = a.download()
                 db, tb = b.download()
                 assert np.array_equal(da, db) and np.array_equal(ta, tb), f"table after {k}"
-        elif waited < k:
-            check_one()
+        else:
+            if read_in_flight:  # a plain call while steps are in flight
+                da, ta = a.download()
+                db, tb = b.download()
+                assert np.array_equal(da, db) and np.array_equal(ta, tb), f"table at {k}"
+            if waited < k:
+                check_one()
     while waited < nsteps:
         check_one()
+    # steps held to a few hit chunks overflow and re-run; the others (a
+    # buffer of the worst case at these sizes) never do
+    held = any(key == TUNE_MAX_HIT_CHUNKS and val for key, val in tuning)
+    reruns = a.step_stats()["reruns"]
+    assert reruns > 0 if held else reruns == 0, reruns
     da, ta = a.download()
     db, tb = b.download()
     assert np.array_equal(da, db) and np.array_equal(ta, tb)
@@ -249,10 +260,35 @@ def test_table_regrowth_recaptures(engine_factory):
               n_ups=500, pipelined=True, grow_at=4)
 
 
-def test_bucket_reallocation_mid_sequence(engine_factory):
-    # steps of many short clauses make every bucket outgrow its capacity
+@pytest.mark.parametrize("max_chunks", [0, 1])
+def test_bucket_reallocation_mid_sequence(engine_factory, max_chunks):
+    # steps of many short clauses make every bucket outgrow its capacity; with
+    # one hit chunk per step, a step's re-run finds its rows moved by the
+    # next step's growth
     run_steps(engine_factory, seed=600, V=800, W=4, nsteps=5, n_new=20000, maxlen=4,
-              n_ups=300, pipelined=True)
+              n_ups=300, pipelined=True,
+              tuning=((TUNE_MAX_HIT_CHUNKS, max_chunks),) if max_chunks else ())
+
+
+@pytest.mark.parametrize("read_in_flight", [False, True])
+@pytest.mark.parametrize("W", [4, 64])
+def test_overflowing_steps_are_re_run(engine_factory, W, read_in_flight):
+    """Steps held to one hit chunk (GSS_TUNE_MAX_HIT_CHUNKS) overflow their
+    buffers every time: the step's graph leaves the table as its sweep read it
+    and halts the engine, so the step in flight behind it stages nothing; the
+    wait (or a plain call made while steps are in flight) grows the buffer,
+    re-runs the sweep and the roll forward, and replays the later step.
+    Records and tables as the plain sequence and the oracle give them."""
+    run_steps(engine_factory, seed=1700 + W + read_in_flight, V=2000, W=W, nsteps=8,
+              n_new=600, maxlen=30, n_ups=900, pipelined=True,
+              tuning=((TUNE_MAX_HIT_CHUNKS, 1),), read_in_flight=read_in_flight)
+
+
+def test_overflowing_screened_steps_are_re_run(engine_factory):
+    # the replayed step rebuilds the aggregate the halted one left alone
+    run_steps(engine_factory, seed=1750, V=2000, W=16, nsteps=6, n_new=600, maxlen=30,
+              n_ups=2000 * 16, pipelined=True, screen=True, uniform=True,
+              tuning=((TUNE_MAX_HIT_CHUNKS, 1),))
 
 
 def test_sweep_only_step_and_errors(engine_factory):
@@ -391,17 +427,20 @@ def test_id_records_are_the_full_records_first_words(engine_factory):
     assert np.array_equal(ids, np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0]))
 
 
-@pytest.mark.parametrize("W", [1, 4, 16, 64])
-def test_clause_records_are_the_reports_by_clause(engine_factory, W):
+@pytest.mark.parametrize("W, max_chunks", [(1, 0), (4, 0), (16, 0), (64, 0), (16, 1), (64, 1)])
+def test_clause_records_are_the_reports_by_clause(engine_factory, W, max_chunks):
     """A step submitted with GSS_RECORDS_CLAUSES records one gss_hit_clause
     per clause with a report in some word: its row and length, and exactly
     the words of the oracle's reports for that clause; for steps that append,
     update and roll forward in between, at every width (sub-lanes below 16
-    words, several words per lane above); the other readers refuse it."""
+    words, several words per lane above), also when one hit chunk per step
+    makes every step overflow and re-run; the other readers refuse it."""
     from gpusharesat_amd import serving as sv
     rng = np.random.default_rng(730 + W)
     V = 1500
     a, b = engine_factory(), engine_factory()
+    if max_chunks:
+        a.set_tuning(TUNE_MAX_HIT_CHUNKS, max_chunks)
     for e in (a, b):
         e.reset_assignments(V, W)
     d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
@@ -437,6 +476,7 @@ def test_clause_records_are_the_reports_by_clause(engine_factory, W):
         og, oe = np.argsort(key_got), np.argsort(key_exp)
         assert np.array_equal(key_got[og], key_exp[oe])
         assert np.array_equal(cls["words"][og], words[oe])
+    assert (a.step_stats()["reruns"] > 0) == bool(max_chunks)
 
 
 def test_step_with_no_valid_slot_still_appends_and_updates(engine_factory):
