@@ -373,6 +373,7 @@ def main() -> int:
         "latency_ms": latency,
         "latency_id_records_ms": latency_ids,
         "latency_full_records_ms": latency_full,
+        "step_hit_buffer": e.step_stats(),
     }
     print(json.dumps(out), flush=True)
     e.close()
@@ -552,6 +553,7 @@ def run_batched(args) -> int:
         "graph_ms": {"mean": float(np.mean(step_ms)), "max": float(np.max(step_ms))},
         "host_submit_ms": {"mean": float(np.mean(submit_s)) * 1e3,
                            "max": float(np.max(submit_s)) * 1e3},
+        "step_hit_buffer": e.step_stats(),
     }
     if n_plain:
         plain_rate = sum(plain_checks) / plain_s
@@ -636,6 +638,7 @@ def run_serving(args) -> int:
         "latency_by_arrival_10s": windows,
         "device_ms_per_step": out.device_ms / max(1, out.steps),
         "host_grouped_steps": out.host_grouped, "graph_captures": out.graph_captures,
+        "step_hit_buffer": out.step_stats,
         "serving_thread": out.thread,
         "driver_lag_ms": {"p99": float(np.percentile(out.lag_ms, 99)),
                           "p999": float(np.percentile(out.lag_ms, 99.9)),

@@ -417,6 +417,7 @@ struct gss_engine {
     uint32_t *d_halt = nullptr;
     uint64_t step_hit_want = 0;
     std::atomic<int64_t> step_recoveries{0};
+    std::atomic<int64_t> step_hit_chunks{0};  // chunks the two slots' hit buffers hold
     // host threads for the staging copies of large steps
     // (GPUSHARESAT_STAGE_THREADS, default 4; 0 or 1 = the calling thread only)
     int stage_threads = 4;
@@ -1624,6 +1625,14 @@ static uint32_t chunk_room(const StepSlot &sl, int rw) {
         sl.h_out.cap ? sl.h_out.cap - 1 : 0);
 }
 
+// Publishes the chunks the step hit buffers hold (read by gss_step_stats
+// from any thread).
+static void note_hit_chunks(gss_engine *e) {
+    int64_t c = 0;
+    for (const StepSlot &sl : e->slots) c += (int64_t)std::min(sl.d_fills.cap, sl.h_out.cap);
+    e->step_hit_chunks.store(c, std::memory_order_relaxed);
+}
+
 // The part of a step after its table updates, over the slot's buffers and
 // the settings of its graph: the sweep, the roll forward (held back if the
 // records overflowed), the chunk fills to the host and the records made
@@ -1775,6 +1784,7 @@ static int recover_step(gss_engine *e, size_t at, uint32_t used) {
         sl.chunk_cap = chunk_room(sl, rw);
         if ((rc = sl.d_offs.reserve_async((size_t)sl.chunk_cap + 1, es))) sl.records_lost = true;
     }
+    note_hit_chunks(e);
     HIP_TRY(launch_reset_u32(e->d_halt, 1, es));
     if (sl.records_lost) {
         // no room: roll the table forward (the gate passes with no limit)
@@ -1863,11 +1873,7 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
 int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks) {
     if (!e) return set_err(GSS_E_INVAL, "null engine");
     if (reruns) *reruns = e->step_recoveries.load(std::memory_order_relaxed);
-    if (hit_chunks) {
-        int64_t c = 0;
-        for (const StepSlot &sl : e->slots) c += std::min(sl.d_fills.cap, sl.h_out.cap);
-        *hit_chunks = c;
-    }
+    if (hit_chunks) *hit_chunks = e->step_hit_chunks.load(std::memory_order_relaxed);
     return GSS_OK;
 }
 
@@ -2182,6 +2188,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
             (rc = sl.d_fills.reserve_async((size_t)chunks, e->stream)) ||
             (rc = sl.h_out.reserve(chunks + 1)))
             return rc;
+        note_hit_chunks(e);
     }
     // (GSS_TUNE_MAX_HIT_CHUNKS holds every step to that many chunks)
     sl.chunk_cap = e->tune_max_chunks > 0

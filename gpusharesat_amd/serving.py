@@ -192,6 +192,7 @@ class ServeResult:
     graph_captures: int = 0  # step graphs the engine captured
     thread: dict = field(default_factory=dict)  # serving thread's submit / wait host times
     lag_ms: np.ndarray = None  # arrival -> published in the ring (this driver's part)
+    step_stats: dict = field(default_factory=dict)  # hit-buffer re-runs and size
 
     @property
     def checks_per_s(self) -> float:
@@ -271,6 +272,13 @@ class GpuServer:
         e = self.engine_handle()
         for name, value in settings.items():
             _lib.call("gss_set_tuning", e, TUNE_KEYS[name], int(value))
+
+    def step_stats(self) -> dict:
+        """The engine's step re-runs (records that overflowed a step's hit
+        buffer) and the chunks of 1024 records its hit buffers hold."""
+        r, c = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("gss_step_stats", self.engine_handle(), ctypes.byref(r), ctypes.byref(c))
+        return {"reruns": r.value, "hit_chunks": c.value}
 
     def update_base(self, ups: np.ndarray) -> None:
         ups = np.ascontiguousarray(ups, BASE_UP_DTYPE)
@@ -569,7 +577,8 @@ class Server:
                            {"submit_ms_mean": (st["submit_ms_total"] - st0["submit_ms_total"])
                             / max(1, st["steps"] - st0["steps"]),
                             "submit_ms_max": st["submit_ms_max"],
-                            "wait_ms_max": st["wait_ms_max"]}, np.array(lags))
+                            "wait_ms_max": st["wait_ms_max"]}, np.array(lags),
+                           srv.step_stats())
 
     def close(self):
         self.srv.close()

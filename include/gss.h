@@ -350,7 +350,8 @@ int gss_bases_reset(gss_engine *e, int nbases);
  * (GpuRunner.cu:363-365). */
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
 /* Steps re-run because their records overflowed the hit buffer, and the
- * chunks of 1024 records the step hit buffers of the engine hold. */
+ * chunks of 1024 records the step hit buffers of the engine hold (also from
+ * a thread other than the one stepping the engine, e.g. a server's). */
 int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks);
 /* Copy the hit records of the last waited step to host (up to cap; *n =
  * total), waiting for their transfer if it is still running.  Valid until
