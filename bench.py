@@ -783,10 +783,21 @@ def run_ceilings(args) -> int:
                             "frac": gbps.value / HBM_PEAK_GBS, "ms": ms.value})
             log(f"gather {table >> 20:5d} MiB table, {unroll} lines in flight per lane: "
                 f"{gbps.value:.0f} GB/s of lines")
+    # reads of part of a line, or of two lines, from the 256 MiB table: what a
+    # half-row (512 slots) or a row of both signs (2048 slots) would cost
+    blocks = []
+    for block in (64, 128, 256):
+        _lib.call("gss_ceiling_gather_block", 0, 256 << 20, block, reads, 4, 5,
+                  ctypes.byref(gbps), ctypes.byref(ms))
+        blocks.append({"table_bytes": 256 << 20, "block_bytes": block, "unroll": 4,
+                       "gbps": gbps.value, "reads_per_s": reads / (ms.value * 1e-3),
+                       "ms": ms.value})
+        log(f"gather 256 MiB table, random {block}-byte blocks: {gbps.value:.0f} GB/s, "
+            f"{reads / (ms.value * 1e-3) / 1e9:.1f} G reads/s")
     out = {"metric": "device memory ceilings (copy; random 128-byte line gathers)",
            "value": copy["gbps"], "unit": "GB/s", "n_gpus": 1, "higher_is_better": True,
            "data": "synthetic", "config": {"workload": "ceilings", "device": dev},
-           "copy": copy, "pcie": pcie, "gather": gathers}
+           "copy": copy, "pcie": pcie, "gather": gathers, "gather_blocks": blocks}
     print(json.dumps(out), flush=True)
     return 0
 

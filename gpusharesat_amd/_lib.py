@@ -103,6 +103,7 @@ SIGNATURES: dict[str, list] = {
     "gss_device_query": [c_int, c_p],
     "gss_ceiling_copy": [c_int, c_i64, c_int, c_p, c_p],
     "gss_ceiling_gather": [c_int, c_i64, c_i64, c_int, c_int, c_p, c_p],
+    "gss_ceiling_gather_block": [c_int, c_i64, c_int, c_i64, c_int, c_int, c_p, c_p],
     "gss_ceiling_pcie": [c_int, c_i64, c_int, c_p, c_p],
     # include/gss_sharer.h
     "gss_sharer_default_options": [c_p],

@@ -55,25 +55,26 @@ __device__ __forceinline__ uint32_t random_line(uint64_t r, uint32_t seed, uint3
     return (uint32_t)(((uint64_t)h * nlines) >> 32);
 }
 
-// Each 16-lane group reads U independent lines per iteration (lane q reads
-// word q of each); reads past n_reads fall on line 0.  The xor of everything
-// read is written per thread so the loads are kept.
-template <int U>
+// Each group of G lanes reads U independent blocks of 8 G bytes per
+// iteration (lane q reads word q of each; G = 16 is a 128-byte line, as the
+// 16-word sweep reads a table row); reads past n_reads fall on block 0.  The
+// xor of everything read is written per thread so the loads are kept.
+template <int G, int U>
 __global__ __launch_bounds__(256) void k_ceiling_gather(const uint64_t *__restrict__ table,
-                                                        uint32_t nlines, uint64_t nreads,
+                                                        uint32_t nblocks, uint64_t nreads,
                                                         uint32_t seed, uint64_t *__restrict__ out) {
     const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    const uint32_t q = threadIdx.x & 15u;
-    const uint64_t group = tid >> 4;
-    const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) >> 4;
+    const uint32_t q = threadIdx.x % G;
+    const uint64_t group = tid / G;
+    const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / G;
     uint64_t acc = 0;
     for (uint64_t base = group * U; base < nreads; base += ngroups * U) {
         uint64_t v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint64_t r = base + u;
-            const uint32_t line = r < nreads ? random_line(r, seed, nlines) : 0u;
-            v[u] = table[(uint64_t)line * 16u + q];
+            const uint32_t b = r < nreads ? random_line(r, seed, nblocks) : 0u;
+            v[u] = table[(uint64_t)b * G + q];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) acc ^= v[u];
@@ -137,6 +138,18 @@ int time_reps(Scope &sc, int reps, Launch &&launch, double *ms) {
     return GSS_OK;
 }
 
+template <int G>
+hipError_t launch_gather(int unroll, int blocks, hipStream_t s, const uint64_t *t,
+                                uint32_t nblocks, uint64_t nr, uint32_t seed, uint64_t *o) {
+    switch (unroll) {
+        case 1: hipLaunchKernelGGL((k_ceiling_gather<G, 1>), dim3(blocks), dim3(256), 0, s, t, nblocks, nr, seed, o); break;
+        case 2: hipLaunchKernelGGL((k_ceiling_gather<G, 2>), dim3(blocks), dim3(256), 0, s, t, nblocks, nr, seed, o); break;
+        case 4: hipLaunchKernelGGL((k_ceiling_gather<G, 4>), dim3(blocks), dim3(256), 0, s, t, nblocks, nr, seed, o); break;
+        default: hipLaunchKernelGGL((k_ceiling_gather<G, 8>), dim3(blocks), dim3(256), 0, s, t, nblocks, nr, seed, o); break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 }  // namespace gss
 
@@ -188,10 +201,17 @@ int gss_ceiling_copy(int device, int64_t bytes, int reps, double *gbps, double *
 
 int gss_ceiling_gather(int device, int64_t table_bytes, int64_t n_reads, int unroll, int reps,
                        double *gbps, double *ms) {
-    if (table_bytes <= 0 || table_bytes % 128 || table_bytes / 128 > 0xFFFFFFFFll ||
+    return gss_ceiling_gather_block(device, table_bytes, 128, n_reads, unroll, reps, gbps, ms);
+}
+
+int gss_ceiling_gather_block(int device, int64_t table_bytes, int block_bytes, int64_t n_reads,
+                             int unroll, int reps, double *gbps, double *ms) {
+    if (block_bytes != 64 && block_bytes != 128 && block_bytes != 256)
+        return set_err(GSS_E_INVAL, "gather: blocks of 64, 128 or 256 bytes");
+    if (table_bytes <= 0 || table_bytes % block_bytes || table_bytes / block_bytes > 0xFFFFFFFFll ||
         n_reads <= 0 || reps < 1 || !gbps || !ms)
-        return set_err(GSS_E_INVAL, "gather: table must be a positive multiple of 128 bytes "
-                                    "(at most 2^32 lines), n_reads > 0, reps >= 1");
+        return set_err(GSS_E_INVAL, "gather: table must be a positive multiple of the block "
+                                    "(at most 2^32 blocks), n_reads > 0, reps >= 1");
     if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8)
         return set_err(GSS_E_INVAL, "gather: unroll must be 1, 2, 4 or 8");
     Scope sc;
@@ -204,23 +224,19 @@ int gss_ceiling_gather(int device, int64_t table_bytes, int64_t n_reads, int unr
         return set_err(GSS_E_NOMEM, "gather: cannot allocate a %lld-byte table",
                        (long long)table_bytes);
     CEIL_HIP(hipMemsetAsync(sc.a, 0xA5, (size_t)table_bytes, sc.s));
-    const uint32_t nlines = (uint32_t)(table_bytes / 128);
+    const uint32_t nblocks = (uint32_t)(table_bytes / block_bytes);
     const uint64_t nr = (uint64_t)n_reads;
     const uint64_t *t = (const uint64_t *)sc.a;
     uint64_t *o = (uint64_t *)sc.b;
     uint32_t seed = 0x1234567u;
     auto launch = [&]() {
         seed = seed * 1664525u + 1013904223u;  // fresh positions every run
-        switch (unroll) {
-            case 1: hipLaunchKernelGGL(k_ceiling_gather<1>, dim3(blocks), dim3(256), 0, sc.s, t, nlines, nr, seed, o); break;
-            case 2: hipLaunchKernelGGL(k_ceiling_gather<2>, dim3(blocks), dim3(256), 0, sc.s, t, nlines, nr, seed, o); break;
-            case 4: hipLaunchKernelGGL(k_ceiling_gather<4>, dim3(blocks), dim3(256), 0, sc.s, t, nlines, nr, seed, o); break;
-            default: hipLaunchKernelGGL(k_ceiling_gather<8>, dim3(blocks), dim3(256), 0, sc.s, t, nlines, nr, seed, o); break;
-        }
-        return hipGetLastError();
+        return block_bytes == 64    ? launch_gather<8>(unroll, blocks, sc.s, t, nblocks, nr, seed, o)
+               : block_bytes == 128 ? launch_gather<16>(unroll, blocks, sc.s, t, nblocks, nr, seed, o)
+                                    : launch_gather<32>(unroll, blocks, sc.s, t, nblocks, nr, seed, o);
     };
     if ((rc = time_reps(sc, reps, launch, ms))) return rc;
-    *gbps = 128.0 * (double)n_reads / (*ms * 1e-3) / 1e9;
+    *gbps = (double)block_bytes * (double)n_reads / (*ms * 1e-3) / 1e9;
     return GSS_OK;
 }
 

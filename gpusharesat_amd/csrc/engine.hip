@@ -2339,12 +2339,16 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     }
     e->inflight.erase(e->inflight.begin());
     sl.busy = false;
-    if (sl.records_lost)
-        return set_err(GSS_E_NOMEM, "the step's hit buffer overflowed and could not grow: its "
-                                    "records are lost (the table did roll forward)");
-    if (used > sl.chunk_cap)
-        return set_err(GSS_E_NOMEM, "step hit buffer overflow (%u > %u chunks)", used,
-                       sl.chunk_cap);
+    if (sl.records_lost || used > sl.chunk_cap) {
+        // no records to read for this step (nor, until the next wait, any other)
+        sl.results = false;
+        e->last_step_slot = -1;
+        return sl.records_lost
+                   ? set_err(GSS_E_NOMEM, "the step's hit buffer overflowed and could not grow: "
+                                          "its records are lost (the table did roll forward)")
+                   : set_err(GSS_E_NOMEM, "step hit buffer overflow (%u > %u chunks)", used,
+                             sl.chunk_cap);
+    }
     // the next steps' buffers hold twice the most chunks a step has claimed
     e->step_hit_want = std::max<uint64_t>(e->step_hit_want, 2 * (uint64_t)used);
     int64_t tot = 0;

@@ -48,6 +48,13 @@ int gss_ceiling_copy(int device, int64_t bytes, int reps, double *gbps, double *
  * average kernel time, *ms = average kernel time. */
 int gss_ceiling_gather(int device, int64_t table_bytes, int64_t n_reads, int unroll, int reps,
                        double *gbps, double *ms);
+/* The same with blocks of block_bytes (64, 128 or 256; table_bytes a positive
+ * multiple of it, at most 2^32 blocks): block_bytes / 8 lanes read one block
+ * at a uniformly random block position.  *gbps = block_bytes * n_reads /
+ * average kernel time.  Whether a read of part of a 128-byte line costs the
+ * device a whole line shows in the rate of 64-byte reads. */
+int gss_ceiling_gather_block(int device, int64_t table_bytes, int block_bytes, int64_t n_reads,
+                             int unroll, int reps, double *gbps, double *ms);
 
 /* Copies `bytes` (positive) between a device buffer and pinned host memory
  * (hipHostMalloc) with hipMemcpyAsync, `reps` times each way after one
