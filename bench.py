@@ -399,9 +399,6 @@ def run_batched(args) -> int:
     from gpusharesat_amd import workloads
     from gpusharesat_amd.engine import Engine
 
-    if args.screen and args.records == "clauses":
-        log("--screen takes full or id records (--records full or ids)")
-        return 2
     steps_total = args.warmup + args.steps
     t0 = time.time()
     log("generating workload batched-16 ...")

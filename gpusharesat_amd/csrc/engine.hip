@@ -262,6 +262,7 @@ struct StepSlot {
     // launch settings baked into the graph, for a recovery's re-run
     int g_blocks = 0, g_tc = 0, g_fuse = 0, g_unroll = 0;
     bool g_serve = false, g_screen = false;
+    bool g_agg_full = false;  // screened: the aggregate is rebuilt from the table
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -354,7 +355,7 @@ struct gss_engine {
     float last_ms = 0.f;
     double total_ms = 0.0;
     int64_t nsweeps = 0;
-    int occ[7][9][2][3] = {};  // [log2 words][unroll][fused][sweep mode] resident blocks per CU
+    int occ[7][9][2][4] = {};  // [log2 words][unroll][fused][sweep mode] resident blocks per CU
 
     // tuning
     int tune_waves_per_cu = 0;
@@ -398,6 +399,11 @@ struct gss_engine {
     int serve_vote = 2;  // entries per screening vote (GPUSHARESAT_SERVE_VOTE: 1, 2, 4 or 8)
     DevBuf<uint8_t> serve_agg;         // 2 * vars + 2 bytes, padded to 4
     DevBuf<uint64_t> serve_agg_words;  // [8]
+    // state_version right after the last screened batched step was submitted:
+    // while it still holds, serve_agg is that step's aggregate kept current
+    // (refreshed from its updates after the roll forward), so the next
+    // screened step only refreshes what its own updates change
+    uint64_t table_agg_at = 0;
     DevBuf<gss_var_patch> serve_dpatches;  // the step's admitted patches (ring capacity)
 
     // batched steps (gss_step_*)
@@ -1665,6 +1671,10 @@ static hipError_t launch_step_tail(gss_engine *e, StepSlot &sl, hipStream_t s) {
     if (he == hipSuccess)
         he = launch_assig_roll_dyn(sl.d_ups.p, &dh->nups, e->table.p, e->words, dh->last_pos,
                                    sl.d_alloc, sl.chunk_cap, e->d_halt, s);
+    // screened steps keep the aggregate current through the roll forward
+    if (he == hipSuccess && sl.g_screen)
+        he = launch_table_agg_update(e->table.p, dh, sl.d_ups.p, nullptr, false, e->words, agg,
+                                     nullptr, s);
     if (he == hipSuccess)
         he = launch_stage_out(sl.d_alloc, sl.d_fills.p, sl.chunk_cap, sl.h_out.p, s);
     if (he == hipSuccess && sl.d_dense.cap > 0)
@@ -1679,7 +1689,7 @@ static hipError_t launch_step_tail(gss_engine *e, StepSlot &sl, hipStream_t s) {
 
 // Captures the step's fixed launch sequence over this slot's buffers.
 static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse,
-                            bool serve, bool screen) {
+                            bool serve, bool screen, bool agg_full) {
     if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
     if (sl.graph) (void)hipGraphDestroy(sl.graph);
     sl.exec = nullptr;
@@ -1690,6 +1700,7 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
     sl.g_unroll = e->tune_unroll;
     sl.g_serve = serve;
     sl.g_screen = screen;
+    sl.g_agg_full = agg_full;
     if (!e->capture_stream)
         HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
     hipStream_t cs = e->capture_stream;
@@ -1736,9 +1747,14 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
         he = serve ? launch_serve_patch(e->serve, dh, e->table.p, e->words, agg, cs)
                    : launch_assig_patch_dyn(sl.d_patches.p, &dh->npatches, e->table.p, e->words,
                                             cs);
-    // batched steps: the aggregate of the table as the sweep will read it
+    // batched steps: the aggregate of the table as the sweep will read it,
+    // rebuilt, or refreshed where this step's updates and patches changed it
     if (he == hipSuccess && screen)
-        he = launch_table_agg(e->table.p, dh, e->vars, e->words, agg, e->serve_agg_words.p, cs);
+        he = agg_full ? launch_table_agg(e->table.p, dh, e->vars, e->words, agg,
+                                         e->serve_agg_words.p, cs)
+                      : launch_table_agg_update(e->table.p, dh, sl.d_ups.p, sl.d_patches.p,
+                                                sl.g_patches, e->words, agg,
+                                                e->serve_agg_words.p, cs);
     if (he == hipSuccess) he = launch_step_tail(e, sl, cs);
     hipGraph_t g = nullptr;
     hipError_t he2 = hipStreamEndCapture(cs, &g);
@@ -1967,8 +1983,6 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         return set_err(GSS_E_INVAL, "records must be GSS_RECORDS_FULL, _IDS or _CLAUSES");
     if (serve && desc->records == GSS_RECORDS_CLAUSES)
         return set_err(GSS_E_INVAL, "serving steps group records by word: full or id records");
-    if ((desc->flags & GSS_STEP_SCREEN) && desc->records == GSS_RECORDS_CLAUSES)
-        return set_err(GSS_E_INVAL, "GSS_STEP_SCREEN takes full or id records");
     if (desc->flags & ~GSS_STEP_SCREEN) return set_err(GSS_E_INVAL, "unknown step flags");
     // serving steps screen against their requests' bases instead
     const bool screen = !serve && (desc->flags & GSS_STEP_SCREEN);
@@ -2133,10 +2147,8 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         tasks += (c + tc - 1) / tc;
     }
     if (tasks > 0x7FFFFFFF) return set_err(GSS_E_INVAL, "too many tasks");
-    const int blocks =
-        step_blocks(e, (serve && e->serve_prefilter) || screen ? 1
-                       : desc->records == GSS_RECORDS_CLAUSES   ? 2
-                                                                : 0);
+    const int blocks = step_blocks(e, ((serve && e->serve_prefilter) || screen ? 1 : 0) +
+                                          (desc->records == GSS_RECORDS_CLAUSES ? 2 : 0));
     // an abandoned chunk holds more than kHitChunk - 64 records and each wave
     // leaves at most one partial chunk: records <= clauses * active words
     // (one record per clause in the clause form) fit in `worst` chunks
@@ -2161,6 +2173,10 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         return rc;
     if (serve && (rc = sl.d_wcount.reserve_async(2 * GSS_MAX_WORDS, e->stream))) return rc;
     const bool use_agg = (serve && e->serve_prefilter) || screen;
+    // a screened step rebuilds the aggregate unless the last change to the
+    // table was a screened step's (word loads change whole words: rebuilt)
+    const bool agg_full =
+        screen && (e->table_agg_at == 0 || e->table_agg_at != e->state_version || n_loads > 0);
     if (use_agg && ((rc = e->serve_agg.reserve_async((size_t)((2 * e->vars + 2 + 3) / 4 * 4), es)) ||
                     (rc = e->serve_agg_words.reserve_async(8, es))))
         return rc;
@@ -2220,10 +2236,11 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         (const void *)(intptr_t)serve, sl.h_report.p, serve ? (const void *)e->serve.reqs : nullptr,
         sl.h_group.p, sl.d_wcount.p, (const void *)(intptr_t)(sl.g_bases + 2 * sl.g_patches),
         use_agg ? (const void *)e->serve_agg.p : nullptr,
-        use_agg ? (const void *)e->serve_agg_words.p : nullptr, (const void *)(intptr_t)screen};
+        use_agg ? (const void *)e->serve_agg_words.p : nullptr,
+        (const void *)(intptr_t)(screen ? (agg_full ? 1 : 2) : 0)};
     if (!sl.exec || key != sl.key) {
         sl.key.clear();
-        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve, screen))) return rc;
+        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve, screen, agg_full))) return rc;
         sl.key = key;
         sl.graph_dense = sl.d_dense.cap;
     }
@@ -2301,6 +2318,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         h.last_pos[w] = (w < W && last_pos) ? last_pos[w] : -1;
     }
     e->state_version++;
+    e->table_agg_at = screen ? e->state_version : 0;
     HIP_TRY(hipEventRecord(sl.t0, e->stream));
     HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));

@@ -156,10 +156,20 @@ hipError_t launch_serve_admit(const ServeRing &r, StepHeader *dh, gss_word_load 
 // Aggregate entries of a batched step from the table itself (see
 // SweepArgs::agg): group b = words [b * words / 8, (b + 1) * words / 8) (one
 // word per group below 8 words); bit b of agg[l] set if row l is all ones on
-// the valid slots of every word of group b; agg_words[b] = the group's words
-// with a valid slot.  Reads the table once (valid words from the header).
+// every slot of group b's words (valid or not: all ones there implies all
+// ones on the valid slots); agg_words[b] = the group's words with a valid
+// slot in this step.  Reads the table once.
 hipError_t launch_table_agg(const uint64_t *table, const StepHeader *dh, int64_t vars, int words,
                             uint8_t *agg, uint64_t *agg_words, hipStream_t s);
+// Keeps such entries current without reading the table: for each of the
+// step's updates (and patches, if with_patches) the bit of the update's word
+// group in both literals of its variable, from the table as it is when the
+// kernel runs; agg_words as above if not null.  Run after the updates are
+// applied and again after the roll forward.
+hipError_t launch_table_agg_update(const uint64_t *table, const StepHeader *dh,
+                                   const gss_var_update *ups, const gss_var_patch *patches,
+                                   bool with_patches, int words, uint8_t *agg,
+                                   uint64_t *agg_words, hipStream_t s);
 // Copies the admitted requests' patches from the pinned ring to r.dpatches
 // (consecutive lanes read consecutive 4-byte words: one pass over PCIe).
 hipError_t launch_serve_stage_patches(const ServeRing &r, const StepHeader *dh, hipStream_t s);
@@ -191,7 +201,8 @@ struct CompactRef {
 hipError_t launch_sweep(const SweepArgs &a, int words, int unroll, int blocks, hipStream_t s);
 int sweep_clauses_per_pass(int words);
 // Max resident 256-thread blocks per CU for the sweep kernel variant (with or
-// without the fused schedule; mode 0: plain, 1: screened, 2: clause records).
+// without the fused schedule; mode 0: plain, 1: screened, 2: clause records,
+// 3: screened with clause records).
 int sweep_occupancy(int words, int unroll, bool fused, int mode = 0);
 
 hipError_t launch_reset_u32(uint32_t *p, int n, hipStream_t s);

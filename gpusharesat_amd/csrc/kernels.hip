@@ -377,11 +377,12 @@ __device__ __forceinline__ int find_bucket(const BucketDesc *__restrict__ b, int
 // MODE: 0 = every clause checked, records per (clause, word) (full or id
 // form, a.rec_words); 1 = clauses screened against a.agg first (serving and
 // screened steps), same record forms; 2 = every clause checked, one record
-// per clause with its words (a.rec_words = 2).
+// per clause with its words (a.rec_words = 2); 3 = screened, one record per
+// clause (screened steps).
 template <int W, int S, int U, bool FUSE, int MODE>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
-    constexpr bool AGG = MODE == 1;
-    constexpr bool CLAUSE_RECORDS = MODE == 2;
+    constexpr bool AGG = MODE == 1 || MODE == 3;
+    constexpr bool CLAUSE_RECORDS = MODE == 2 || MODE == 3;
     constexpr int K = W > 16 ? W / 16 : 1;  // table words per lane
     constexpr int LW = W / K;               // lanes across one row
     constexpr int G = LW * S;
@@ -1001,36 +1002,91 @@ __global__ void k_serve_stage_patches(ServeRing r, const StepHeader *__restrict_
     }
 }
 
+// Batched-step aggregate: bit b of a literal row's entry is set if the row is
+// all ones over every slot of word group b (valid or not, so that the entry
+// follows the table alone and a step's updates refresh it).  All ones on
+// every slot implies all ones on the valid ones, which is what the screening
+// needs.  agg_words[b] = the group's words with a valid slot in this step.
 template <int W>
-__global__ void k_table_agg(const uint64_t *__restrict__ table, const StepHeader *__restrict__ dh,
-                            int64_t rows, uint8_t *__restrict__ agg,
-                            uint64_t *__restrict__ agg_words) {
-    constexpr int NG = W < 8 ? W : 8;  // word groups
-    constexpr int GW = W / NG;         // words per group
-    if (dh->halted) return;            // the aggregate stays the halted step's
-    __shared__ uint64_t s_valid[W];
-    if ((int)threadIdx.x < W) s_valid[threadIdx.x] = dh->valid[threadIdx.x];
-    __syncthreads();
+struct TableGroups {
+    static constexpr int NG = W < 8 ? W : 8;  // word groups
+    static constexpr int GW = W / NG;         // words per group
+};
+
+template <int W>
+__device__ __forceinline__ void table_agg_words(const StepHeader *__restrict__ dh,
+                                                uint64_t *__restrict__ agg_words) {
+    constexpr int NG = TableGroups<W>::NG, GW = TableGroups<W>::GW;
     if (blockIdx.x == 0 && (int)threadIdx.x < 8) {
         uint64_t m = 0;
         if ((int)threadIdx.x < NG)
             for (int w = (int)threadIdx.x * GW; w < ((int)threadIdx.x + 1) * GW; w++)
-                if (s_valid[w]) m |= 1ull << w;
+                if (dh->valid[w]) m |= 1ull << w;
         agg_words[threadIdx.x] = m;
     }
+}
+
+template <int W>
+__device__ __forceinline__ bool group_all_ones(const uint64_t *__restrict__ row, int b) {
+    constexpr int GW = TableGroups<W>::GW;
+    uint64_t x = ~0ull;
+#pragma unroll
+    for (int w = 0; w < GW; w++) x &= row[b * GW + w];
+    return x == ~0ull;
+}
+
+template <int W>
+__global__ void k_table_agg(const uint64_t *__restrict__ table, const StepHeader *__restrict__ dh,
+                            int64_t rows, uint8_t *__restrict__ agg,
+                            uint64_t *__restrict__ agg_words) {
+    constexpr int NG = TableGroups<W>::NG;
+    if (dh->halted) return;  // the aggregate stays the halted step's
+    table_agg_words<W>(dh, agg_words);
     for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < rows;
          l += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t *row = table + (size_t)l * W;
         uint32_t bits = 0xFFu;
 #pragma unroll
-        for (int b = 0; b < NG; b++) {
-            bool strong = true;
-#pragma unroll
-            for (int w = b * GW; w < (b + 1) * GW; w++)
-                strong = strong && ((row[w] | ~s_valid[w]) == ~0ull);
-            if (!strong) bits &= ~(1u << b);
-        }
+        for (int b = 0; b < NG; b++)
+            if (!group_all_ones<W>(row, b)) bits &= ~(1u << b);
         agg[l] = (uint8_t)bits;
+    }
+}
+
+// Refreshes the aggregate bits that the step's updates (and patches) may have
+// changed: for each (var, word), the bit of the word's group in the entries
+// of both literals of var, from the table as it is now.  Entries are bytes,
+// four to a 32-bit word: bits change with 32-bit atomics.  Two updates of one
+// (var, group) write the same bit.
+template <int W>
+__global__ void k_table_agg_update(const uint64_t *__restrict__ table,
+                                   const StepHeader *__restrict__ dh,
+                                   const gss_var_update *__restrict__ ups,
+                                   const gss_var_patch *__restrict__ patches, int with_patches,
+                                   uint32_t *__restrict__ agg32,
+                                   uint64_t *__restrict__ agg_words) {
+    constexpr int GW = TableGroups<W>::GW;
+    if (dh->halted) return;
+    if (agg_words) table_agg_words<W>(dh, agg_words);
+    const int64_t nu = dh->nups, n = nu + (with_patches ? dh->npatches : 0);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t var, word;
+        if (i < nu) {
+            var = ups[i].var;
+            word = ups[i].word;
+        } else {
+            var = patches[i - nu].var;
+            word = patches[i - nu].word;
+        }
+        const int b = (int)(word / GW);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const uint64_t l = 2ull * var + s;
+            const uint32_t bit = 1u << (b + 8 * (int)(l & 3));
+            if (group_all_ones<W>(table + l * W, b)) atomicOr(agg32 + (l >> 2), bit);
+            else atomicAnd(agg32 + (l >> 2), ~bit);
+        }
     }
 }
 
@@ -1163,8 +1219,7 @@ template <int W, int S, int U>
 hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
     const dim3 g(blocks), b(256);
     const bool fused = a.fuse_len > 0;
-    if (a.agg && a.rec_words == 2) return hipErrorInvalidValue;  // no such variant
-    const int mode = a.agg ? 1 : a.rec_words == 2 ? 2 : 0;
+    const int mode = (a.agg ? 1 : 0) + (a.rec_words == 2 ? 2 : 0);
 #define GSS_LAUNCH(F_, M_)                                                     \
     if (fused == F_ && mode == M_) {                                           \
         hipLaunchKernelGGL((k_sweep<W, S, U, F_, M_>), g, b, 0, s, a);         \
@@ -1176,6 +1231,8 @@ hipError_t launch_sweep_t(const SweepArgs &a, int blocks, hipStream_t s) {
     GSS_LAUNCH(false, 1)
     GSS_LAUNCH(true, 2)
     GSS_LAUNCH(false, 2)
+    GSS_LAUNCH(true, 3)
+    GSS_LAUNCH(false, 3)
 #undef GSS_LAUNCH
     return hipErrorInvalidValue;
 }
@@ -1193,13 +1250,15 @@ int occupancy_t(bool fused, int mode) {
     GSS_OCC(false, 1)
     GSS_OCC(true, 2)
     GSS_OCC(false, 2)
+    GSS_OCC(true, 3)
+    GSS_OCC(false, 3)
 #undef GSS_OCC
     if (e != hipSuccess) return 1;
     return nb > 0 ? nb : 1;
 }
 
 // Kernel variants: (W, S, U), each with and without the fused schedule
-// (fuse_len <= 0 selects the one without), in the three modes of k_sweep.  `unroll` selects U for W >= 16
+// (fuse_len <= 0 selects the one without), in the four modes of k_sweep.  `unroll` selects U for W >= 16
 // (0 = default).
 #define GSS_SWEEP_VARIANTS(X)                  \
     X(1, 16, 1)                                \
@@ -1399,6 +1458,29 @@ hipError_t launch_table_agg(const uint64_t *table, const StepHeader *dh, int64_t
     GSS_TABLE_AGG(32)
     GSS_TABLE_AGG(64)
 #undef GSS_TABLE_AGG
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_table_agg_update(const uint64_t *table, const StepHeader *dh,
+                                   const gss_var_update *ups, const gss_var_patch *patches,
+                                   bool with_patches, int words, uint8_t *agg,
+                                   uint64_t *agg_words, hipStream_t s) {
+    uint32_t *agg32 = (uint32_t *)agg;
+    const int wp = with_patches ? 1 : 0;
+#define GSS_TABLE_AGG_UP(W_)                                                                 \
+    if (words == W_) {                                                                       \
+        hipLaunchKernelGGL(k_table_agg_update<W_>, dim3(1024), dim3(256), 0, s, table, dh,   \
+                           ups, patches, wp, agg32, agg_words);                              \
+        return hipGetLastError();                                                            \
+    }
+    GSS_TABLE_AGG_UP(1)
+    GSS_TABLE_AGG_UP(2)
+    GSS_TABLE_AGG_UP(4)
+    GSS_TABLE_AGG_UP(8)
+    GSS_TABLE_AGG_UP(16)
+    GSS_TABLE_AGG_UP(32)
+    GSS_TABLE_AGG_UP(64)
+#undef GSS_TABLE_AGG_UP
     return hipErrorInvalidValue;
 }
 

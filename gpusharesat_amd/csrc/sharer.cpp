@@ -112,6 +112,9 @@ EngineClauseSharer::EngineClauseSharer(GpuClauseSharerOptions opts,
         else if (!v.empty() && v != "clauses")
             throw std::invalid_argument("GPUSHARESAT_RECORDS must be clauses or ids");
     }
+    // runs screened against the table's aggregate first (same reports; a
+    // solver's slots share most of their values, which is where it pays)
+    if (const char *sc = std::getenv("GPUSHARESAT_SCREEN")) screen_ = std::atoi(sc) != 0;
     if (policy_ != Policy::Host) {
         max_tags_ = 1 << 22;
         if (const char *t = std::getenv("GPUSHARESAT_RESIDENCY_MAX_TAGS"))
@@ -270,6 +273,7 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     d.valid = valid.data();
     d.last_pos = run.last_pos.data();
     d.records = clause_records_ ? GSS_RECORDS_CLAUSES : GSS_RECORDS_IDS;
+    d.flags = screen_ ? GSS_STEP_SCREEN : 0;
     check_rc(gss_step_submit(engine_, &d), "gss_step_submit");
     for (size_t i = 0; i < rows.size(); i++)
         if (rows[i] != b.rows[i])

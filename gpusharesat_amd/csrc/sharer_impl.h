@@ -294,6 +294,7 @@ private:
     enum class Policy { Host, Device, Learned };
     Policy policy_ = Policy::Host;
     bool clause_records_ = true;  // GPUSHARESAT_RECORDS: clauses (default) or ids
+    bool screen_ = true;          // GPUSHARESAT_SCREEN (default 1): runs as GSS_STEP_SCREEN steps
     gss_res *res_ = nullptr;
     int64_t max_tags_ = 0;
     std::vector<uint32_t> free_tags_;

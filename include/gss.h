@@ -317,12 +317,15 @@ typedef struct gss_step_desc {
 #define GSS_RECORDS_IDS 1
 #define GSS_RECORDS_CLAUSES 2
 /* Screens every clause, one per lane, against an aggregate of the table
- * (per literal, a bit per group of words: non-false in every valid slot of
- * the group, from one read of the table after the step's updates) and runs
- * the exact check only on the clauses and word groups the screening leaves.
- * The records are the same with or without it; it pays where the slots of a
- * word group share most of their values (consecutive assignments of one
- * solver), and costs one read of the table per step where they do not. */
+ * (per literal, a bit per group of words: non-false in every slot of the
+ * group) and runs the exact check only on the clauses and word groups the
+ * screening leaves.  The records are the same with or without it, in any
+ * record form.  The aggregate is read from the whole table when the table
+ * changed otherwise since the last screened step (and on that step's word
+ * loads); else only the entries of the step's updates and patches are
+ * refreshed, after they are applied and again after the roll forward.  It
+ * pays where the slots of a word group share most of their values
+ * (consecutive assignments of one solver). */
 #define GSS_STEP_SCREEN 1
 int gss_step_submit(gss_engine *e, const gss_step_desc *d);
 /* (Re)creates nbases solver base planes over the current variable count,

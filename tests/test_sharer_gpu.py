@@ -626,11 +626,15 @@ def _clause_hits(clauses, snapshots):
     return out
 
 
-@pytest.mark.parametrize("solvers,seed", [(1, 0), (3, 1), (5, 2), (16, 3)])
-def test_snapshot_semantics_match_oracle(solvers, seed):
+@pytest.mark.parametrize("solvers,seed,screen", [(1, 0, "0"), (3, 1, "0"), (5, 2, "0"),
+                                                 (16, 3, "0"), (3, 4, "1"), (16, 5, "1")])
+def test_snapshot_semantics_match_oracle(monkeypatch, solvers, seed, screen):
     """Each solver mutates its assignment (set / unset), sends 0-3 snapshots
     per run, and pops everything after two runs; the popped clause ids must be
-    exactly the union of the oracle's hits over its snapshots of that run."""
+    exactly the union of the oracle's hits over its snapshots of that run.
+    With the runs screened against the table's aggregate (GPUSHARESAT_SCREEN
+    1, the default) and not (0)."""
+    monkeypatch.setenv("GPUSHARESAT_SCREEN", screen)
     rng = np.random.default_rng(seed)
     V = 60
     with sharer(V, solvers) as s:

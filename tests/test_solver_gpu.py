@@ -53,10 +53,11 @@ def pigeonhole(p, h):
     return p * h, cl
 
 
-def solve(path, solver="glucose-gpu", threads=4):
+def solve(path, solver="glucose-gpu", threads=4, env=None):
     r = subprocess.run([SOLVERS[solver], "-model", "-verb=1", f"-thread-count={threads}",
                         "-write-stats-period-sec=-1", path],
-                       capture_output=True, text=True, timeout=240)
+                       capture_output=True, text=True, timeout=240,
+                       env=None if env is None else {**os.environ, **env})
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     return r
 
@@ -79,12 +80,16 @@ def test_reference_solver_sat_instances_with_model(tmp_path, solver):
             assert any(l in model for l in c), c
 
 
-@pytest.mark.parametrize("solver", [pytest.param(n, marks=needs(n)) for n in SOLVERS])
-def test_reference_solver_unsat_pigeonhole(tmp_path, solver):
+@pytest.mark.parametrize("solver,screen",
+                         [pytest.param(n, "1", marks=needs(n)) for n in SOLVERS]
+                         + [pytest.param("glucose-gpu", "0", marks=needs("glucose-gpu"))])
+def test_reference_solver_unsat_pigeonhole(tmp_path, solver, screen):
+    # screen "1" (the default): every gpuRun screened against the table's
+    # aggregate; "0": checked exactly throughout
     n, cls = pigeonhole(8, 7)
     path = str(tmp_path / "php_8_7.cnf")
     write_cnf(path, n, cls)
-    r = solve(path, solver)
+    r = solve(path, solver, env={"GPUSHARESAT_SCREEN": screen})
     assert r.returncode == 20, r.returncode
     assert "s UNSATISFIABLE" in r.stdout
     if solver != "glucose-gpu":

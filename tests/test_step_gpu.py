@@ -96,7 +96,7 @@ class Plain:
 
 def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipelined,
               grow_at=None, eager=True, tuning=(), page_limit=None, screen=False,
-              uniform=False, read_in_flight=False):
+              uniform=False, read_in_flight=False, uniform_init=False):
     rng = np.random.default_rng(seed)
     a, b = engine_factory(), engine_factory()
     for key, val in tuning:
@@ -106,8 +106,12 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
         b.set_page_locked_limit(page_limit)
     a.reset_assignments(V, W)
     b.reset_assignments(V, W)
-    d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
-    t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
+    if uniform_init:  # every word's 64 slots alike
+        d0 = np.where(rng.random((V, W)) < 0.6, ALL, np.uint64(0))
+        t0 = np.where(rng.random((V, W)) < 0.5, ALL, np.uint64(0)) & d0
+    else:
+        d0 = workloads.bernoulli_words(rng, V, W * 64, 0.5)
+        t0 = workloads.bernoulli_words(rng, V, W * 64, 0.5) & d0
     a.upload(d0, t0)
     b.upload(d0, t0)
     plain = Plain(b)
@@ -139,7 +143,8 @@ def run_steps(engine_factory, seed, V, W, nsteps, n_new, maxlen, n_ups, pipeline
                                                      uniform=uniform)
         print(f"step {k}: {nn} clauses, {ups.size} updates, {V} vars, {W} words", flush=True)
         expected.append(plain.step(lits, lens, ups, valid, last_pos))
-        rows = a.step_enqueue(lits, lens, ups, valid, last_pos, screen=screen)
+        rows = a.step_enqueue(lits, lens, ups, valid, last_pos,
+                              screen=screen(k) if callable(screen) else screen)
         lens_a = np.concatenate([lens_a, lens])
         rows_a = np.concatenate([rows_a, rows])
         if not pipelined:
@@ -282,6 +287,19 @@ def test_overflowing_steps_are_re_run(engine_factory, W, read_in_flight):
     run_steps(engine_factory, seed=1700 + W + read_in_flight, V=2000, W=W, nsteps=8,
               n_new=600, maxlen=30, n_ups=900, pipelined=True,
               tuning=((TUNE_MAX_HIT_CHUNKS, 1),), read_in_flight=read_in_flight)
+
+
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_screened_steps_keep_the_aggregate_current(engine_factory, pipelined):
+    """Consecutive screened steps refresh the aggregate only where their
+    updates changed the table (after the updates, and again after the roll
+    forward); a step that is not screened, a plain read between steps or a
+    table regrowth in between makes the next screened step rebuild it.  The
+    slots of a word are alike, so most entries are set and a stale one would
+    drop records."""
+    run_steps(engine_factory, seed=1800 + pipelined, V=1500, W=16, nsteps=12, n_new=500,
+              maxlen=25, n_ups=900, pipelined=pipelined, screen=lambda k: k % 5 != 3,
+              uniform=True, uniform_init=True, grow_at=8)
 
 
 def test_overflowing_screened_steps_are_re_run(engine_factory):
@@ -427,14 +445,18 @@ def test_id_records_are_the_full_records_first_words(engine_factory):
     assert np.array_equal(ids, np.sort(full.view(np.uint64).reshape(-1, 3)[:, 0]))
 
 
-@pytest.mark.parametrize("W, max_chunks", [(1, 0), (4, 0), (16, 0), (64, 0), (16, 1), (64, 1)])
-def test_clause_records_are_the_reports_by_clause(engine_factory, W, max_chunks):
+@pytest.mark.parametrize("W, max_chunks, screen", [
+    (1, 0, False), (4, 0, False), (16, 0, False), (64, 0, False), (16, 1, False), (64, 1, False),
+    (4, 0, True), (16, 0, True), (64, 0, True), (64, 1, True)])
+def test_clause_records_are_the_reports_by_clause(engine_factory, W, max_chunks, screen):
     """A step submitted with GSS_RECORDS_CLAUSES records one gss_hit_clause
     per clause with a report in some word: its row and length, and exactly
     the words of the oracle's reports for that clause; for steps that append,
     update and roll forward in between, at every width (sub-lanes below 16
     words, several words per lane above), also when one hit chunk per step
-    makes every step overflow and re-run; the other readers refuse it."""
+    makes every step overflow and re-run, and for screened steps (whose
+    slots here are drawn independently, so most clauses pass the screen);
+    the other readers refuse it."""
     from gpusharesat_amd import serving as sv
     rng = np.random.default_rng(730 + W)
     V = 1500
@@ -454,7 +476,7 @@ def test_clause_records_are_the_reports_by_clause(engine_factory, W, max_chunks)
                                                      600)
         exp = plain.step(lits, lens, ups, valid, last_pos)
         sv.submit(a, valid, lits=lits, lens=lens, ups=ups, last_pos=last_pos,
-                  records=sv.RECORDS_CLAUSES)
+                  records=sv.RECORDS_CLAUSES, flags=sv.STEP_SCREEN if screen else 0)
         n, _ = a.step_wait()
         cls = a.step_clauses_view().copy()
         with pytest.raises(GssError, match="ids or clauses only"):
