@@ -232,8 +232,8 @@ struct StepSlot {
     DevBuf<gss_hit> d_dense;     // compacted records of the last waited step
     DevBuf<uint64_t> d_offs;     // record offset of each chunk (the graph's compaction)
     uint64_t dense_want = 0;     // records the graph's compaction should hold
-    uint64_t graph_dense = 0;    // records the captured graph compacts into d_dense (0 = none)
-    uint64_t step_dense = 0;     // the same, for the last submitted step
+    uint64_t step_dense = 0;     // records the last submitted step's graph compacts into
+                                 // d_dense (0 = none)
     // The records of a waited step go to a pinned host buffer on the slot's
     // own stream as soon as gss_step_wait has seen the step end (one DMA copy
     // when the graph compacted them); the slot's next graph waits for that
@@ -259,12 +259,28 @@ struct StepSlot {
     uint64_t worst_chunks = 0;
     bool settled = false;       // the step in flight is known to have kept its records
     bool records_lost = false;  // its recovery could not grow the buffer
-    // launch settings baked into the graph, for a recovery's re-run
+    // launch settings of the last submitted step's graph, for a recovery's
+    // re-run of its tail
     int g_blocks = 0, g_tc = 0, g_fuse = 0, g_unroll = 0;
     bool g_serve = false, g_screen = false;
     bool g_agg_full = false;  // screened: the aggregate is rebuilt from the table
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // Graphs captured over this slot's buffers, one per key (everything baked
+    // in); a caller alternating between step forms (screened or not, rebuilt
+    // aggregate or refreshed) replays them without capturing again.  The
+    // least recently used one goes when a fifth is needed.
+    struct StepGraph {
+        std::vector<const void *> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int blocks = 0, tc = 0, fuse = 0, unroll = 0;
+        bool serve = false, screen = false, agg_full = false;
+        uint64_t dense = 0;  // records it compacts into d_dense (0 = none)
+        uint64_t used = 0;   // submit count at its last use
+    };
+    static constexpr size_t kGraphsPerSlot = 4;
+    std::vector<StepGraph> graphs;
+    int cur = -1;  // graph of the slot's last submitted step
+    uint64_t submits = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool busy = false;
     bool serve = false;    // the slot's last step admitted requests from the serving ring
@@ -281,7 +297,6 @@ struct StepSlot {
     int rec_words = 3;     // record form of the slot's last submitted step (3: gss_hit,
                            // 1: gss_hit_id, 2: gss_hit_clause)
     int res_words = 3;     // the same, of its last waited step (the results)
-    std::vector<const void *> key;  // everything baked into the captured graph
 };
 
 }  // namespace
@@ -627,8 +642,10 @@ int gss_engine_destroy(gss_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (StepSlot &sl : e->slots) {
         if (sl.copy) (void)hipStreamSynchronize(sl.copy);
-        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        for (StepSlot::StepGraph &g : sl.graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
         if (sl.t0) (void)hipEventDestroy(sl.t0);
         if (sl.t1) (void)hipEventDestroy(sl.t1);
         if (sl.ev_copied) (void)hipEventDestroy(sl.ev_copied);
@@ -1687,13 +1704,15 @@ static hipError_t launch_step_tail(gss_engine *e, StepSlot &sl, hipStream_t s) {
     return he;
 }
 
-// Captures the step's fixed launch sequence over this slot's buffers.
-static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int fuse,
-                            bool serve, bool screen, bool agg_full) {
-    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-    if (sl.graph) (void)hipGraphDestroy(sl.graph);
-    sl.exec = nullptr;
-    sl.graph = nullptr;
+// Captures the step's fixed launch sequence over this slot's buffers into
+// graph entry g (whose previous graph, if any, is destroyed first).
+static int build_step_graph(gss_engine *e, StepSlot &sl, StepSlot::StepGraph &gr, int blocks,
+                            int tc, int fuse, bool serve, bool screen, bool agg_full) {
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+    if (gr.graph) (void)hipGraphDestroy(gr.graph);
+    gr.exec = nullptr;
+    gr.graph = nullptr;
+    gr.key.clear();
     sl.g_blocks = blocks;
     sl.g_tc = tc;
     sl.g_fuse = fuse;
@@ -1763,8 +1782,16 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, int blocks, int tc, int
         return set_err(GSS_E_HIP, "step capture failed: %s", hipGetErrorString(he));
     }
     if (he2 != hipSuccess) return set_err(GSS_E_HIP, "step capture: %s", hipGetErrorString(he2));
-    sl.graph = g;
-    HIP_TRY(hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0));
+    gr.graph = g;
+    HIP_TRY(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+    gr.blocks = blocks;
+    gr.tc = tc;
+    gr.fuse = fuse;
+    gr.unroll = e->tune_unroll;
+    gr.serve = serve;
+    gr.screen = screen;
+    gr.agg_full = agg_full;
+    gr.dense = sl.d_dense.cap;
     e->graph_captures.fetch_add(1, std::memory_order_relaxed);
     return GSS_OK;
 }
@@ -1815,7 +1842,7 @@ static int recover_step(gss_engine *e, size_t at, uint32_t used) {
     for (size_t i = at + 1; i < e->inflight.size(); i++) {
         StepSlot &nx = e->slots[e->inflight[i]];
         HIP_TRY(hipEventRecord(nx.t0, es));
-        HIP_TRY(hipGraphLaunch(nx.exec, es));
+        HIP_TRY(hipGraphLaunch(nx.graphs[nx.cur].exec, es));
         HIP_TRY(hipEventRecord(nx.t1, es));
         nx.settled = false;
     }
@@ -2238,13 +2265,33 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
         use_agg ? (const void *)e->serve_agg.p : nullptr,
         use_agg ? (const void *)e->serve_agg_words.p : nullptr,
         (const void *)(intptr_t)(screen ? (agg_full ? 1 : 2) : 0)};
-    if (!sl.exec || key != sl.key) {
-        sl.key.clear();
-        if ((rc = build_step_graph(e, sl, blocks, tc, fuse, serve, screen, agg_full))) return rc;
-        sl.key = key;
-        sl.graph_dense = sl.d_dense.cap;
+    int gi = -1;
+    for (size_t i = 0; i < sl.graphs.size(); i++)
+        if (sl.graphs[i].exec && sl.graphs[i].key == key) gi = (int)i;
+    if (gi < 0) {
+        if (sl.graphs.size() < StepSlot::kGraphsPerSlot) {
+            sl.graphs.emplace_back();
+            gi = (int)sl.graphs.size() - 1;
+        } else {
+            gi = 0;
+            for (size_t i = 1; i < sl.graphs.size(); i++)
+                if (sl.graphs[i].used < sl.graphs[gi].used) gi = (int)i;
+        }
+        StepSlot::StepGraph &gr = sl.graphs[gi];
+        if ((rc = build_step_graph(e, sl, gr, blocks, tc, fuse, serve, screen, agg_full))) return rc;
+        gr.key = key;
     }
-    sl.step_dense = sl.graph_dense;
+    StepSlot::StepGraph &gr = sl.graphs[gi];
+    gr.used = ++sl.submits;
+    sl.cur = gi;
+    sl.g_blocks = gr.blocks;
+    sl.g_tc = gr.tc;
+    sl.g_fuse = gr.fuse;
+    sl.g_unroll = gr.unroll;
+    sl.g_serve = gr.serve;
+    sl.g_screen = gr.screen;
+    sl.g_agg_full = gr.agg_full;
+    sl.step_dense = gr.dense;
     // ---- stage this step (the slot is idle) and commit it to the DB: one
     // record per clause (the device copies it to its bucket row)
     {
@@ -2320,7 +2367,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     e->state_version++;
     e->table_agg_at = screen ? e->state_version : 0;
     HIP_TRY(hipEventRecord(sl.t0, e->stream));
-    HIP_TRY(hipGraphLaunch(sl.exec, e->stream));
+    HIP_TRY(hipGraphLaunch(sl.graphs[sl.cur].exec, e->stream));
     HIP_TRY(hipEventRecord(sl.t1, e->stream));
     sl.busy = true;
     sl.settled = false;

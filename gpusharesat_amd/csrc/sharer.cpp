@@ -112,9 +112,17 @@ EngineClauseSharer::EngineClauseSharer(GpuClauseSharerOptions opts,
         else if (!v.empty() && v != "clauses")
             throw std::invalid_argument("GPUSHARESAT_RECORDS must be clauses or ids");
     }
-    // runs screened against the table's aggregate first (same reports; a
-    // solver's slots share most of their values, which is where it pays)
-    if (const char *sc = std::getenv("GPUSHARESAT_SCREEN")) screen_ = std::atoi(sc) != 0;
+    // runs screened against the table's aggregate first (same reports; it
+    // pays where a solver's slots share most of their values)
+    if (const char *sc = std::getenv("GPUSHARESAT_SCREEN")) {
+        const std::string v = sc;
+        if (v == "0") screen_cfg_ = Screen::Off;
+        else if (v == "1") screen_cfg_ = Screen::On;
+        else if (v.rfind("auto:", 0) == 0 && std::atoi(v.c_str() + 5) >= 8)
+            screen_period_ = std::atoi(v.c_str() + 5);
+        else if (!v.empty() && v != "auto")
+            throw std::invalid_argument("GPUSHARESAT_SCREEN must be 0, 1, auto or auto:N (N >= 8)");
+    }
     if (policy_ != Policy::Host) {
         max_tags_ = 1 << 22;
         if (const char *t = std::getenv("GPUSHARESAT_RESIDENCY_MAX_TAGS"))
@@ -273,7 +281,8 @@ bool EngineClauseSharer::start_run(InFlight &run) {
     d.valid = valid.data();
     d.last_pos = run.last_pos.data();
     d.records = clause_records_ ? GSS_RECORDS_CLAUSES : GSS_RECORDS_IDS;
-    d.flags = screen_ ? GSS_STEP_SCREEN : 0;
+    run.screened = choose_screen();
+    d.flags = run.screened ? GSS_STEP_SCREEN : 0;
     check_rc(gss_step_submit(engine_, &d), "gss_step_submit");
     for (size_t i = 0; i < rows.size(); i++)
         if (rows[i] != b.rows[i])
@@ -319,6 +328,33 @@ void EngineClauseSharer::collect(InFlight &run) {
     check_rc(gss_step_wait(engine_, &n, &ms), "gss_step_wait");
     run.records = n;
     if (opts_.quickProf) global_stats_[timeSpentTestingClauses] += (unsigned long)(ms * 1000.0f);
+    note_screen_time(run.screened, ms, run.clause_count);
+}
+
+bool EngineClauseSharer::choose_screen() {
+    if (screen_cfg_ != Screen::Auto) return screen_cfg_ == Screen::On;
+    const int64_t k = screen_runs_++;
+    if (k > 0 && k % screen_period_ == 0) {
+        // a period (and its probe of the other form) ended
+        const int b = screen_best_ ? 1 : 0;
+        if (screen_ema_[b] > 0.0 && screen_ema_[1 - b] > 0.0 &&
+            screen_ema_[1 - b] < 0.9 * screen_ema_[b])
+            screen_best_ = !screen_best_;
+        log(2, "screening: device ns per clause " + std::to_string(screen_ema_[1] * 1e6) +
+                   " screened, " + std::to_string(screen_ema_[0] * 1e6) + " not; runs " +
+                   (screen_best_ ? "screened" : "not screened"));
+    }
+    const bool probe = k % screen_period_ >= screen_period_ - kScreenProbe;
+    return probe ? !screen_best_ : screen_best_;
+}
+
+void EngineClauseSharer::note_screen_time(bool screened, float ms, long clauses) {
+    const int f = screened ? 1 : 0;
+    const bool steady = last_screened_ == f;
+    last_screened_ = f;
+    if (!steady || clauses <= 0 || !(ms > 0.f)) return;
+    const double x = (double)ms / (double)clauses;
+    screen_ema_[f] = screen_ema_[f] > 0.0 ? 0.75 * screen_ema_[f] + 0.25 * x : x;
 }
 
 // Learned policy: every 16 runs a snapshot of up to 8192 resident clauses'

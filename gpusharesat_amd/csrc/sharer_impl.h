@@ -248,6 +248,7 @@ private:
         bool plain = false;   // a plain sweep (device policies) instead of a step
         bool swept = false;   // plain: a sweep was launched (some slot was valid)
         bool idle = false;    // nothing to append, apply or check: no device work
+        bool screened = false;  // the step was submitted with GSS_STEP_SCREEN
     };
 
     void run_once(bool can_start);
@@ -294,7 +295,24 @@ private:
     enum class Policy { Host, Device, Learned };
     Policy policy_ = Policy::Host;
     bool clause_records_ = true;  // GPUSHARESAT_RECORDS: clauses (default) or ids
-    bool screen_ = true;          // GPUSHARESAT_SCREEN (default 1): runs as GSS_STEP_SCREEN steps
+    // Runs as GSS_STEP_SCREEN steps (same reports): GPUSHARESAT_SCREEN 1 =
+    // always, 0 = never, unset = chosen from the runs' device times.  In
+    // that mode the last kScreenProbe of every screen_period_ runs use the
+    // other form; each form keeps a moving average of device ms per resident
+    // clause over its runs that followed a run of the same form (a switch
+    // makes the next screened step rebuild the aggregate), and at the end of
+    // a period the other form takes over if it was at least 10 % faster.
+    // Screened first.
+    enum class Screen { Off, On, Auto };
+    Screen screen_cfg_ = Screen::Auto;
+    static constexpr int kScreenProbe = 4;
+    int screen_period_ = 256;  // GPUSHARESAT_SCREEN=auto:N sets it (N >= 8)
+    int64_t screen_runs_ = 0;
+    bool screen_best_ = true;
+    double screen_ema_[2] = {0.0, 0.0};  // [unscreened, screened]
+    int last_screened_ = -1;             // form of the last step run (-1: none yet)
+    bool choose_screen();
+    void note_screen_time(bool screened, float ms, long clauses);
     gss_res *res_ = nullptr;
     int64_t max_tags_ = 0;
     std::vector<uint32_t> free_tags_;

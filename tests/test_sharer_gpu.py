@@ -626,14 +626,17 @@ def _clause_hits(clauses, snapshots):
     return out
 
 
-@pytest.mark.parametrize("solvers,seed,screen", [(1, 0, "0"), (3, 1, "0"), (5, 2, "0"),
-                                                 (16, 3, "0"), (3, 4, "1"), (16, 5, "1")])
-def test_snapshot_semantics_match_oracle(monkeypatch, solvers, seed, screen):
+@pytest.mark.parametrize("solvers,seed,screen,rounds", [
+    (1, 0, "0", 12), (3, 1, "0", 12), (5, 2, "0", 12), (16, 3, "0", 12), (3, 4, "1", 12),
+    (16, 5, "1", 12), (3, 6, "auto:8", 40)])
+def test_snapshot_semantics_match_oracle(monkeypatch, solvers, seed, screen, rounds):
     """Each solver mutates its assignment (set / unset), sends 0-3 snapshots
     per run, and pops everything after two runs; the popped clause ids must be
     exactly the union of the oracle's hits over its snapshots of that run.
     With the runs screened against the table's aggregate (GPUSHARESAT_SCREEN
-    1, the default) and not (0)."""
+    1), not (0), and chosen by their times (auto, the default, here with
+    the other form for 4 of every 8 runs instead of 256, so that the forms
+    alternate many times)."""
     monkeypatch.setenv("GPUSHARESAT_SCREEN", screen)
     rng = np.random.default_rng(seed)
     V = 60
@@ -641,7 +644,7 @@ def test_snapshot_semantics_match_oracle(monkeypatch, solvers, seed, screen):
         clauses = []
         ids = []
         state = [np.full(V, 2, np.int64) for _ in range(solvers)]
-        for rnd in range(12):
+        for rnd in range(rounds):
             for _ in range(int(rng.integers(5, 40))):
                 L = int(rng.integers(1, 6))
                 c = [int(x) for x in rng.integers(0, 2 * V, size=L)]

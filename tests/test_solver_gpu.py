@@ -81,11 +81,12 @@ def test_reference_solver_sat_instances_with_model(tmp_path, solver):
 
 
 @pytest.mark.parametrize("solver,screen",
-                         [pytest.param(n, "1", marks=needs(n)) for n in SOLVERS]
-                         + [pytest.param("glucose-gpu", "0", marks=needs("glucose-gpu"))])
+                         [pytest.param(n, "auto", marks=needs(n)) for n in SOLVERS]
+                         + [pytest.param("glucose-gpu", s, marks=needs("glucose-gpu"))
+                            for s in ("0", "1")])
 def test_reference_solver_unsat_pigeonhole(tmp_path, solver, screen):
-    # screen "1" (the default): every gpuRun screened against the table's
-    # aggregate; "0": checked exactly throughout
+    # gpuRun screened against the table's aggregate: "auto" (the default)
+    # picks the form by the runs' device times, "1" always, "0" never
     n, cls = pigeonhole(8, 7)
     path = str(tmp_path / "php_8_7.cnf")
     write_cnf(path, n, cls)
