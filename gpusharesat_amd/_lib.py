@@ -69,7 +69,7 @@ SIGNATURES: dict[str, list] = {
     "gss_step_hits_map": [c_p, c_p, c_p],
     "gss_step_ids_map": [c_p, c_p, c_p],
     "gss_step_clauses_map": [c_p, c_p, c_p],
-    "gss_step_stats": [c_p, c_p, c_p],
+    "gss_step_stats": [c_p, c_p],
     "gss_db_reserve": [c_p, c_i64, c_p],
     "gss_server_default_config": [c_p],
     "gss_server_create": [c_int, c_p, c_p],

@@ -20,6 +20,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <thread>
+#include <deque>
+#include <condition_variable>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -156,7 +160,96 @@ struct Pinned {
         cap = 0;
         paged = false;
     }
+    // exchanges the allocations (not the page limit)
+    void swap_with(Pinned &o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(paged, o.paged);
+    }
     ~Pinned() { release(); }
+};
+
+// Allocates and frees pinned host record buffers on a thread of its own, so
+// that the thread waiting for a step does not stall for the milliseconds a
+// large pinned allocation (or its release) takes: a buffer's replacement is
+// allocated ahead of the wait that needs it.
+class HostPrefetch {
+  public:
+    explicit HostPrefetch(int device) : device_(device), th_([this] { run(); }) {}
+    ~HostPrefetch() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    // dst gets at least n records; *state goes from 1 to 2 when it is ready
+    // (0 if the allocation failed)
+    void allocate(Pinned<gss_hit> *dst, std::atomic<int> *state, size_t n) {
+        state->store(1, std::memory_order_release);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            jobs_.push_back({dst, state, n, nullptr, 0, false});
+        }
+        cv_.notify_all();
+    }
+    // takes over buf's allocation and releases it
+    void release(Pinned<gss_hit> &buf) {
+        if (!buf.p) return;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            jobs_.push_back({nullptr, nullptr, 0, buf.p, buf.cap, buf.paged});
+        }
+        buf.p = nullptr;
+        buf.cap = 0;
+        buf.paged = false;
+        cv_.notify_all();
+    }
+    void wait(const std::atomic<int> *state) {
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return state->load(std::memory_order_acquire) != 1; });
+    }
+
+  private:
+    struct Job {
+        Pinned<gss_hit> *dst;
+        std::atomic<int> *state;
+        size_t n;
+        gss_hit *old;  // release job: the allocation to free
+        size_t old_cap;
+        bool old_paged;
+    };
+    void run() {
+        (void)hipSetDevice(device_);
+        for (;;) {
+            Job j;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty()) return;  // stopping, nothing left
+                j = jobs_.front();
+                jobs_.pop_front();
+            }
+            if (j.dst) {
+                const int rc = j.dst->reserve(j.n);
+                {
+                    std::lock_guard<std::mutex> g(mu_);
+                    j.state->store(rc == GSS_OK ? 2 : 0, std::memory_order_release);
+                }
+                done_.notify_all();
+            } else if (j.old) {
+                if (j.old_paged) std::free(j.old);
+                else (void)hipHostFree(j.old);
+            }
+        }
+    }
+    int device_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::deque<Job> jobs_;
+    bool stop_ = false;
+    std::thread th_;
 };
 
 template <typename T>
@@ -242,6 +335,10 @@ struct StepSlot {
     // With h_n > 1 buffers (serving engines) the waits rotate through them,
     // and a step's records stay in place for h_n waits of the slot.
     Pinned<gss_hit> h_ring[kMaxHostRecordBuffers];
+    // replacements of h_ring[i] allocated ahead by the engine's HostPrefetch
+    // (state 0: none, 1: being allocated, 2: ready)
+    Pinned<gss_hit> h_spare[kMaxHostRecordBuffers];
+    std::atomic<int> spare_state[kMaxHostRecordBuffers] = {};
     int h_at = 0, h_n = 1;
     Pinned<gss_hit> &h_hits() { return h_ring[h_at]; }
     hipStream_t copy = nullptr;
@@ -439,6 +536,10 @@ struct gss_engine {
     uint64_t step_hit_want = 0;
     std::atomic<int64_t> step_recoveries{0};
     std::atomic<int64_t> step_hit_chunks{0};  // chunks the two slots' hit buffers hold
+    // pinned host record buffers grown in gss_step_wait, and the time taken
+    std::atomic<int64_t> host_grows{0}, host_grow_us{0}, host_grow_us_max{0};
+    uint64_t records_hw = 0;  // the most records a waited step has made
+    std::unique_ptr<HostPrefetch> prefetch;  // created at the first record-buffer growth
     // host threads for the staging copies of large steps
     // (GPUSHARESAT_STAGE_THREADS, default 4; 0 or 1 = the calling thread only)
     int stage_threads = 4;
@@ -631,6 +732,8 @@ int gss_engine_create(int device, gss_engine **out) {
     e->h_ups.page_limit = &e->page_limit;
     for (StepSlot &sl : e->slots)
         for (auto &h : sl.h_ring) h.page_limit = &e->page_limit;
+    for (StepSlot &sl : e->slots)
+        for (auto &h : sl.h_spare) h.page_limit = &e->page_limit;
     if ((rc = gss_assig_reset(e, 0, 1))) return fail(rc);
     *out = e;
     return GSS_OK;
@@ -640,6 +743,7 @@ int gss_engine_destroy(gss_engine *e) {
     if (!e) return GSS_OK;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    e->prefetch.reset();  // its thread may be filling a slot's spare buffer
     for (StepSlot &sl : e->slots) {
         if (sl.copy) (void)hipStreamSynchronize(sl.copy);
         for (StepSlot::StepGraph &g : sl.graphs) {
@@ -1596,23 +1700,61 @@ static int step_blocks(gss_engine *e, int mode) {
 // slot's pinned h_hits on the slot's stream: one DMA copy when the graph made
 // them dense, else a compaction first (and the next capture's dense buffer is
 // sized for this volume).
-static int queue_records_copy(StepSlot &sl, int64_t total, uint32_t used) {
+static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32_t used) {
     int rc;
     const int rw = sl.res_words;  // 8-byte words per record
     if (!sl.copy) HIP_TRY(hipStreamCreateWithFlags(&sl.copy, hipStreamNonBlocking));
     if (!sl.ev_copied) HIP_TRY(hipEventCreateWithFlags(&sl.ev_copied, hipEventDisableTiming));
     hipStream_t cs = sl.copy;
     sl.h_at = (sl.h_at + 1) % sl.h_n;  // the buffer of the slot's h_n-th last wait
+    const int j = sl.h_at;
+    e->records_hw = std::max<uint64_t>(e->records_hw, (uint64_t)total);
     if (2 * (size_t)total > sl.h_hits().cap) {
         // grown four-fold once the records pass half of it, to at least 1M
-        // records (a pinned allocation stalls the caller for milliseconds
-        // per 100 MB: growing well ahead of the volume keeps that rare); the
-        // previous copy into this slot's pinned buffer must end before the
-        // buffer is replaced
+        // records -- from the replacement allocated ahead when there is one
+        // (a pinned allocation stalls its thread for milliseconds per 100
+        // MB); the previous copy into this slot's pinned memory must end
+        // before a buffer is replaced
+        const auto g0 = std::chrono::steady_clock::now();
         if (sl.copy_queued) HIP_TRY(hipEventSynchronize(sl.ev_copied));
         sl.copy_queued = false;
-        if ((rc = sl.h_hits().reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
-            return rc;
+        int st = sl.spare_state[j].load(std::memory_order_acquire);
+        if (st == 1) {
+            e->prefetch->wait(&sl.spare_state[j]);
+            st = sl.spare_state[j].load(std::memory_order_acquire);
+        }
+        if (st == 2 && sl.h_spare[j].cap >= 2 * (size_t)total) {
+            sl.h_hits().swap_with(sl.h_spare[j]);
+            e->prefetch->release(sl.h_spare[j]);
+        } else {
+            if (st == 2) e->prefetch->release(sl.h_spare[j]);
+            if ((rc = sl.h_hits().reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
+                return rc;
+        }
+        sl.spare_state[j].store(0, std::memory_order_release);
+        const int64_t us = std::chrono::duration_cast<std::chrono::microseconds>(
+                               std::chrono::steady_clock::now() - g0)
+                               .count();
+        e->host_grows.fetch_add(1, std::memory_order_relaxed);
+        e->host_grow_us.fetch_add(us, std::memory_order_relaxed);
+        if (us > e->host_grow_us_max.load(std::memory_order_relaxed))
+            e->host_grow_us_max.store(us, std::memory_order_relaxed);
+    }
+    // the slot's next wait uses the next buffer: its replacement, if it holds
+    // less than twice the most records a step has made, is allocated now
+    const int jn = (j + 1) % sl.h_n;
+    if (2 * e->records_hw > sl.h_ring[jn].cap &&
+        sl.spare_state[jn].load(std::memory_order_acquire) == 0) {
+        if (!e->prefetch) {
+            try {
+                e->prefetch.reset(new HostPrefetch(e->device));
+            } catch (...) {
+                e->prefetch.reset();  // no thread: the wait allocates as before
+            }
+        }
+        if (e->prefetch)
+            e->prefetch->allocate(&sl.h_spare[jn], &sl.spare_state[jn],
+                                  std::max<size_t>(4 * (size_t)e->records_hw, (size_t)1 << 20));
     }
     if ((uint64_t)total > sl.step_dense) {
         if ((rc = sl.d_dense.reserve_async((size_t)total, cs)) ||
@@ -1913,10 +2055,13 @@ int gss_step_submit(gss_engine *e, const gss_step_desc *desc) {
     return gss::step_submit_ex(e, desc, false);
 }
 
-int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks) {
-    if (!e) return set_err(GSS_E_INVAL, "null engine");
-    if (reruns) *reruns = e->step_recoveries.load(std::memory_order_relaxed);
-    if (hit_chunks) *hit_chunks = e->step_hit_chunks.load(std::memory_order_relaxed);
+int gss_step_stats(gss_engine *e, gss_step_counters *out) {
+    if (!e || !out) return set_err(GSS_E_INVAL, "null argument");
+    out->reruns = e->step_recoveries.load(std::memory_order_relaxed);
+    out->hit_chunks = e->step_hit_chunks.load(std::memory_order_relaxed);
+    out->host_grows = e->host_grows.load(std::memory_order_relaxed);
+    out->host_grow_ms = 1e-3 * (double)e->host_grow_us.load(std::memory_order_relaxed);
+    out->host_grow_ms_max = 1e-3 * (double)e->host_grow_us_max.load(std::memory_order_relaxed);
     return GSS_OK;
 }
 
@@ -2433,7 +2578,7 @@ int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms) {
     e->last_step_hits = tot;
     e->last_step_chunks = used;
     e->last_step_ms = ms;
-    if (tot > 0 && (rc = queue_records_copy(sl, tot, used))) return rc;
+    if (tot > 0 && (rc = queue_records_copy(e, sl, tot, used))) return rc;
     sl.results = true;
     if (n_hits) *n_hits = tot;
     if (step_ms) *step_ms = ms;

@@ -340,11 +340,11 @@ class Engine:
         return np.frombuffer(buf, dtype=HIT_CLAUSE_DTYPE)
 
     def step_stats(self) -> dict:
-        """Steps re-run because their records overflowed the hit buffer, and
-        the chunks of 1024 records the step hit buffers hold."""
-        r, c = ctypes.c_int64(), ctypes.c_int64()
-        _lib.call("gss_step_stats", self._h, ctypes.byref(r), ctypes.byref(c))
-        return {"reruns": r.value, "hit_chunks": c.value}
+        """gss_step_counters of the engine: steps re-run because their records
+        overflowed the hit buffer, chunks of 1024 records the step hit
+        buffers hold, pinned record-buffer growths in step waits and their
+        time."""
+        return step_counters(self._h)
 
     def step_hits(self) -> np.ndarray:
         n = ctypes.c_int64()
@@ -371,3 +371,16 @@ class ClauseMap:
 
     def ids(self, hits: np.ndarray) -> np.ndarray:
         return self.table[self.base[hits["len"].astype(np.int64)] + hits["row"].astype(np.int64)]
+
+
+class _StepCounters(ctypes.Structure):
+    _fields_ = [("reruns", ctypes.c_int64), ("hit_chunks", ctypes.c_int64),
+                ("host_grows", ctypes.c_int64), ("host_grow_ms", ctypes.c_double),
+                ("host_grow_ms_max", ctypes.c_double)]
+
+
+def step_counters(handle) -> dict:
+    """gss_step_stats of an engine handle, as a dict."""
+    c = _StepCounters()
+    _lib.call("gss_step_stats", handle, ctypes.byref(c))
+    return {name: getattr(c, name) for name, _ in _StepCounters._fields_}

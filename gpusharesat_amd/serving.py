@@ -274,11 +274,10 @@ class GpuServer:
             _lib.call("gss_set_tuning", e, TUNE_KEYS[name], int(value))
 
     def step_stats(self) -> dict:
-        """The engine's step re-runs (records that overflowed a step's hit
-        buffer) and the chunks of 1024 records its hit buffers hold."""
-        r, c = ctypes.c_int64(), ctypes.c_int64()
-        _lib.call("gss_step_stats", self.engine_handle(), ctypes.byref(r), ctypes.byref(c))
-        return {"reruns": r.value, "hit_chunks": c.value}
+        """The engine's step counters (gss_step_stats): re-runs, hit-buffer
+        chunks, pinned record-buffer growths and their time."""
+        from .engine import step_counters
+        return step_counters(self.engine_handle())
 
     def update_base(self, ups: np.ndarray) -> None:
         ups = np.ascontiguousarray(ups, BASE_UP_DTYPE)

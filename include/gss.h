@@ -352,10 +352,16 @@ int gss_bases_reset(gss_engine *e, int nbases);
  * buffer (Reporter.cuh:46-48) and doubles it for later runs
  * (GpuRunner.cu:363-365). */
 int gss_step_wait(gss_engine *e, int64_t *n_hits, float *step_ms);
-/* Steps re-run because their records overflowed the hit buffer, and the
- * chunks of 1024 records the step hit buffers of the engine hold (also from
- * a thread other than the one stepping the engine, e.g. a server's). */
-int gss_step_stats(gss_engine *e, int64_t *reruns, int64_t *hit_chunks);
+/* Counters of the engine's batched steps, readable from any thread (also
+ * one other than the thread stepping the engine, e.g. a server's). */
+typedef struct gss_step_counters {
+    int64_t reruns;          /* steps re-run: their records overflowed the hit buffer */
+    int64_t hit_chunks;      /* chunks of 1024 records the step hit buffers hold */
+    int64_t host_grows;      /* pinned host record buffers grown by gss_step_wait */
+    double host_grow_ms;     /* time those allocations held the waiting thread, in all */
+    double host_grow_ms_max; /* the longest of them */
+} gss_step_counters;
+int gss_step_stats(gss_engine *e, gss_step_counters *out);
 /* Copy the hit records of the last waited step to host (up to cap; *n =
  * total), waiting for their transfer if it is still running.  Valid until
  * the next gss_step_wait of that step's slot (two waits later), also when

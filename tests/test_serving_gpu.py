@@ -13,8 +13,6 @@ loads and patches) and the native serving loop (include/gss_serve.h).
   growth) for two seconds: sampled requests' records equal the oracle's on
   the DB and base their step had.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
@@ -288,10 +286,8 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
         st = srv.stats()
         assert st["requests_done"] == 1 + len(burst) and st["requests_pending"] == 0
         # (the serving thread is idle: every request is done)
-        reruns, chunks = ctypes.c_int64(), ctypes.c_int64()
-        _lib.call("gss_step_stats", srv.engine_handle(), ctypes.byref(reruns),
-                  ctypes.byref(chunks))
-        assert (reruns.value > 0) == bool(max_chunks), reruns.value
+        reruns = srv.step_stats()["reruns"]
+        assert (reruns > 0) == bool(max_chunks), reruns
 
 
 @pytest.mark.parametrize("records", [sv.RECORDS_IDS, sv.RECORDS_FULL])
