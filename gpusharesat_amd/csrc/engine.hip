@@ -1709,9 +1709,12 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
     sl.h_at = (sl.h_at + 1) % sl.h_n;  // the buffer of the slot's h_n-th last wait
     const int j = sl.h_at;
     e->records_hw = std::max<uint64_t>(e->records_hw, (uint64_t)total);
-    if (2 * (size_t)total > sl.h_hits().cap) {
-        // grown four-fold once the records pass half of it, to at least 1M
-        // records -- from the replacement allocated ahead when there is one
+    // the ring's buffers count 24-byte units: n records of this form take
+    auto units = [rw](uint64_t n) { return (size_t)((n * (uint64_t)rw + 2) / 3); };
+    const size_t need = units((uint64_t)total);
+    if (2 * need > sl.h_hits().cap) {
+        // grown four-fold once the records pass half of it, to at least 24
+        // MB -- from the replacement allocated ahead when there is one
         // (a pinned allocation stalls its thread for milliseconds per 100
         // MB); the previous copy into this slot's pinned memory must end
         // before a buffer is replaced
@@ -1723,12 +1726,12 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
             e->prefetch->wait(&sl.spare_state[j]);
             st = sl.spare_state[j].load(std::memory_order_acquire);
         }
-        if (st == 2 && sl.h_spare[j].cap >= 2 * (size_t)total) {
+        if (st == 2 && sl.h_spare[j].cap >= 2 * need) {
             sl.h_hits().swap_with(sl.h_spare[j]);
             e->prefetch->release(sl.h_spare[j]);
         } else {
             if (st == 2) e->prefetch->release(sl.h_spare[j]);
-            if ((rc = sl.h_hits().reserve(std::max<size_t>(4 * (size_t)total, (size_t)1 << 20))))
+            if ((rc = sl.h_hits().reserve(std::max<size_t>(4 * need, (size_t)1 << 20))))
                 return rc;
         }
         sl.spare_state[j].store(0, std::memory_order_release);
@@ -1743,8 +1746,8 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
     // the slot's next wait uses the next buffer: its replacement, if it holds
     // less than twice the most records a step has made, is allocated now
     const int jn = (j + 1) % sl.h_n;
-    if (2 * e->records_hw > sl.h_ring[jn].cap &&
-        sl.spare_state[jn].load(std::memory_order_acquire) == 0) {
+    const size_t hw = units(e->records_hw);
+    if (2 * hw > sl.h_ring[jn].cap && sl.spare_state[jn].load(std::memory_order_acquire) == 0) {
         if (!e->prefetch) {
             try {
                 e->prefetch.reset(new HostPrefetch(e->device));
@@ -1754,7 +1757,7 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
         }
         if (e->prefetch)
             e->prefetch->allocate(&sl.h_spare[jn], &sl.spare_state[jn],
-                                  std::max<size_t>(4 * (size_t)e->records_hw, (size_t)1 << 20));
+                                  std::max<size_t>(4 * hw, (size_t)1 << 20));
     }
     if ((uint64_t)total > sl.step_dense) {
         if ((rc = sl.d_dense.reserve_async((size_t)total, cs)) ||
