@@ -38,14 +38,15 @@ struct SweepArgs {
     const int32_t *dyn;      // if set, {nbuckets, ntasks} are read here (graph replays)
     int rec_words;           // 8-byte words per record: 3 (gss_hit), 1 (gss_hit_id) or 2
                              // (gss_hit_clause: one record per clause, its words as a mask)
-    // Serving steps (null otherwise): bit b of agg[l] is set only if literal
-    // l is non-false in every valid slot of every request of group b (the
-    // step's admitted requests r with r % 8 = b); 2 * vars + 2 bytes (padded
-    // to 4), agg[2 * vars] = 0 for the sentinel.  agg_words[b] = the table
-    // words of group b's requests.  In a group where two literal occurrences
-    // of a clause are so marked no slot can report, so the exact check reads
-    // table rows only for the clause's other words.  4 MB at 2M variables: the
-    // votes are served from L2.
+    // Serving and screened batched steps (null otherwise): bit b of agg[l]
+    // is set only if literal l is non-false in every valid slot of group b --
+    // serving: the step's admitted requests r with r % 8 = b; screened steps:
+    // word group b, the row all ones over its slots (launch_table_agg);
+    // 2 * vars + 2 bytes (padded to 4).  agg_words[b] = the table words of
+    // group b (serving: of its requests; screened: with a valid slot).  In a
+    // group where two literal occurrences of a clause are so marked no slot
+    // can report, so the exact check reads table rows only for the clause's
+    // other words.  4 MB at 2M variables: the votes are served from L2.
     const uint8_t *agg;
     const uint64_t *agg_words;  // [8]
     int agg_vote;               // aggregate entries read together per vote: 1, 2, 4 or 8
