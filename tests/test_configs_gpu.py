@@ -37,12 +37,18 @@ def assert_same(got, exp, what):
         assert np.array_equal(got[f], exp[f]), (what, f)
 
 
-@pytest.mark.parametrize("screen", [False, True])
-def test_batched16_steps_match_plain_and_oracle(screen):
-    """batched-16 itself: six graph steps, each equal to the plain sequence
-    and to the oracle on the table at sweep time; with GSS_STEP_SCREEN the
-    steps screen clauses against the table's aggregate first."""
-    gen = workloads.SolverStreams(seed=3, var_count=1_000_000)
+@pytest.mark.parametrize("screen, signs, nsteps", [(False, "planted", 6), (True, "planted", 6),
+                                                  (True, "random", 3)])
+def test_batched16_steps_match_plain_and_oracle(screen, signs, nsteps):
+    """batched-16 itself: graph steps, each equal to the plain sequence and
+    to the oracle on the table at sweep time; with GSS_STEP_SCREEN the steps
+    screen clauses against the table's aggregate first (refreshed from each
+    step's updates after the first).  Also with random clause signs and
+    assigned values (40 % assigned), where short clauses report in nearly
+    every word and the screening drops almost nothing."""
+    gen = workloads.SolverStreams(seed=3, var_count=1_000_000,
+                                  **({} if signs == "planted" else
+                                     dict(p_assigned=0.4, agree=0.5, lit_true=0.5)))
     d0, t0 = gen.initial_table()
     lits, lens = gen.initial_clauses(1_000_000)
     print(f"batched-16: 1M vars, {lens.size} initial clauses ({lits.size} literals), "
@@ -54,7 +60,7 @@ def test_batched16_steps_match_plain_and_oracle(screen):
             e.upload(d0, t0)
         del d0, t0
         all_lits, all_lens, rows = [lits], [lens], [base_rows]
-        for k in range(6):
+        for k in range(nsteps):
             st = gen.step()
             print(f"step {k}: {st.lens.size} clauses, {st.ups.size} updates, "
                   f"{st.assignments} assignments", flush=True)
