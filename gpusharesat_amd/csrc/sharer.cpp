@@ -335,14 +335,17 @@ bool EngineClauseSharer::choose_screen() {
     if (screen_cfg_ != Screen::Auto) return screen_cfg_ == Screen::On;
     const int64_t k = screen_runs_++;
     if (k > 0 && k % screen_period_ == 0) {
-        // a period (and its probe of the other form) ended
+        // a period (and its probe of the other form) ended: the averages of
+        // its runs decide, and the next period averages afresh
+        const double m[2] = {screen_n_[0] ? screen_sum_[0] / (double)screen_n_[0] : 0.0,
+                             screen_n_[1] ? screen_sum_[1] / (double)screen_n_[1] : 0.0};
         const int b = screen_best_ ? 1 : 0;
-        if (screen_ema_[b] > 0.0 && screen_ema_[1 - b] > 0.0 &&
-            screen_ema_[1 - b] < 0.9 * screen_ema_[b])
-            screen_best_ = !screen_best_;
-        log(2, "screening: device ns per clause " + std::to_string(screen_ema_[1] * 1e6) +
-                   " screened, " + std::to_string(screen_ema_[0] * 1e6) + " not; runs " +
+        if (m[b] > 0.0 && m[1 - b] > 0.0 && m[1 - b] < 0.95 * m[b]) screen_best_ = !screen_best_;
+        log(2, "screening: device ns per clause " + std::to_string(m[1] * 1e6) + " screened, " +
+                   std::to_string(m[0] * 1e6) + " not; runs " +
                    (screen_best_ ? "screened" : "not screened"));
+        screen_sum_[0] = screen_sum_[1] = 0.0;
+        screen_n_[0] = screen_n_[1] = 0;
     }
     const bool probe = k % screen_period_ >= screen_period_ - kScreenProbe;
     return probe ? !screen_best_ : screen_best_;
@@ -353,8 +356,8 @@ void EngineClauseSharer::note_screen_time(bool screened, float ms, long clauses)
     const bool steady = last_screened_ == f;
     last_screened_ = f;
     if (!steady || clauses <= 0 || !(ms > 0.f)) return;
-    const double x = (double)ms / (double)clauses;
-    screen_ema_[f] = screen_ema_[f] > 0.0 ? 0.75 * screen_ema_[f] + 0.25 * x : x;
+    screen_sum_[f] += (double)ms / (double)clauses;
+    screen_n_[f]++;
 }
 
 // Learned policy: every 16 runs a snapshot of up to 8192 resident clauses'

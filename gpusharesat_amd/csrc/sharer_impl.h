@@ -298,10 +298,10 @@ private:
     // Runs as GSS_STEP_SCREEN steps (same reports): GPUSHARESAT_SCREEN 1 =
     // always, 0 = never, unset = chosen from the runs' device times.  In
     // that mode the last kScreenProbe of every screen_period_ runs use the
-    // other form; each form keeps a moving average of device ms per resident
-    // clause over its runs that followed a run of the same form (a switch
+    // other form; each form averages the device ms per resident clause of
+    // its runs in the period that followed a run of the same form (a switch
     // makes the next screened step rebuild the aggregate), and at the end of
-    // a period the other form takes over if it was at least 10 % faster.
+    // a period the other form takes over if it was at least 5 % faster.
     // Screened first.
     enum class Screen { Off, On, Auto };
     Screen screen_cfg_ = Screen::Auto;
@@ -309,7 +309,8 @@ private:
     int screen_period_ = 256;  // GPUSHARESAT_SCREEN=auto:N sets it (N >= 8)
     int64_t screen_runs_ = 0;
     bool screen_best_ = true;
-    double screen_ema_[2] = {0.0, 0.0};  // [unscreened, screened]
+    double screen_sum_[2] = {0.0, 0.0};  // [unscreened, screened] over the period
+    int64_t screen_n_[2] = {0, 0};
     int last_screened_ = -1;             // form of the last step run (-1: none yet)
     bool choose_screen();
     void note_screen_time(bool screened, float ms, long clauses);
