@@ -1900,9 +1900,14 @@ static int build_step_graph(gss_engine *e, StepSlot &sl, StepSlot::StepGraph &gr
     if (he == hipSuccess && serve)
         he = launch_serve_admit(e->serve, dh, sl.d_loads.p, sl.h_report.p, e->words, cs);
     if (he == hipSuccess && serve) he = launch_serve_stage_patches(e->serve, dh, cs);
+    // a serving step's words follow its loads and the rest are invalid in it,
+    // so it writes whole table lines
     if (he == hipSuccess && sl.g_bases)
-        he = launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
-                                   e->table.p, e->words, e->vars, cs);
+        he = serve && e->words % 2 == 0
+                 ? launch_words_load_lines(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
+                                           e->table.p, e->words, e->vars, cs)
+                 : launch_words_load_dyn(sl.d_loads.p, &dh->nloads, e->bases.p, e->bases_vw,
+                                         e->table.p, e->words, e->vars, cs);
     if (he == hipSuccess && agg && serve)
         he = launch_serve_agg(e->serve, dh, sl.d_loads.p, e->bases.p, e->bases_vw, e->vars, agg,
                               e->serve_agg_words.p, cs);
@@ -2226,7 +2231,7 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
     for (int64_t i = 0; i < n_patches; i++)
         if (desc->patches[i].var >= (uint64_t)e->vars || desc->patches[i].word >= (uint32_t)W)
             return set_err(GSS_E_INVAL, "patch %lld out of range", (long long)i);
-    if ((n_bups || n_loads) && e->bases_vw != (e->vars + 63) / 64)
+    if ((n_bups || n_loads || serve) && e->bases_vw != (e->vars + 63) / 64)
         return set_err(GSS_E_STATE, "the table changed size: reset the bases");
     for (int64_t i = 0; i < n_bups; i++)
         if (desc->base_ups[i].var >= (uint64_t)e->vars || desc->base_ups[i].base >= e->nbases)

@@ -265,6 +265,13 @@ hipError_t launch_base_apply_dyn(const gss_base_update *u, const int64_t *n, uin
 hipError_t launch_words_load_dyn(const gss_word_load *loads, const int64_t *n,
                                  const uint64_t *bases, int64_t vw, uint64_t *table, int words,
                                  int64_t vars, hipStream_t s);
+// Serving steps (loads[k].word == k, words the step does not load invalid in
+// it): sets words [0, n) of every variable's two rows from their loads and
+// words [n, L) to all ones, L = n rounded up to 16 (at most words), so rows
+// are written in whole lines.  words even; vw = (vars + 63) / 64.
+hipError_t launch_words_load_lines(const gss_word_load *loads, const int64_t *n,
+                                   const uint64_t *bases, int64_t vw, uint64_t *table, int words,
+                                   int64_t vars, hipStream_t s);
 hipError_t launch_assig_patch_dyn(const gss_var_patch *p, const int64_t *n, uint64_t *table,
                                   int words, hipStream_t s);
 
