@@ -1742,12 +1742,25 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
         e->host_grow_us.fetch_add(us, std::memory_order_relaxed);
         if (us > e->host_grow_us_max.load(std::memory_order_relaxed))
             e->host_grow_us_max.store(us, std::memory_order_relaxed);
+    } else if (sl.spare_state[j].load(std::memory_order_acquire) == 2) {
+        // a replacement allocated ahead is ready: it takes the buffer's place
+        // now, before a step needs it (with one buffer, once the copy into it
+        // has ended)
+        if (sl.h_n == 1 && sl.copy_queued) {
+            HIP_TRY(hipEventSynchronize(sl.ev_copied));
+            sl.copy_queued = false;
+        }
+        if (sl.h_spare[j].cap > sl.h_hits().cap) sl.h_hits().swap_with(sl.h_spare[j]);
+        e->prefetch->release(sl.h_spare[j]);
+        sl.spare_state[j].store(0, std::memory_order_release);
     }
-    // the slot's next wait uses the next buffer: its replacement, if it holds
-    // less than twice the most records a step has made, is allocated now
-    const int jn = (j + 1) % sl.h_n;
+    // every buffer of the slot that holds less than three times the most
+    // records a step has made gets a replacement of six times that, allocated
+    // ahead: ready long before a step's records pass half of the buffer
     const size_t hw = units(e->records_hw);
-    if (2 * hw > sl.h_ring[jn].cap && sl.spare_state[jn].load(std::memory_order_acquire) == 0) {
+    for (int b = 0; b < sl.h_n; b++) {
+        if (3 * hw <= sl.h_ring[b].cap || sl.spare_state[b].load(std::memory_order_acquire) != 0)
+            continue;
         if (!e->prefetch) {
             try {
                 e->prefetch.reset(new HostPrefetch(e->device));
@@ -1755,9 +1768,9 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
                 e->prefetch.reset();  // no thread: the wait allocates as before
             }
         }
-        if (e->prefetch)
-            e->prefetch->allocate(&sl.h_spare[jn], &sl.spare_state[jn],
-                                  std::max<size_t>(4 * hw, (size_t)1 << 20));
+        if (!e->prefetch) break;
+        e->prefetch->allocate(&sl.h_spare[b], &sl.spare_state[b],
+                              std::max<size_t>(6 * hw, (size_t)1 << 20));
     }
     if ((uint64_t)total > sl.step_dense) {
         if ((rc = sl.d_dense.reserve_async((size_t)total, cs)) ||
