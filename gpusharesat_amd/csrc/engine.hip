@@ -231,8 +231,13 @@ class HostPrefetch {
                 j = jobs_.front();
                 jobs_.pop_front();
             }
+            const auto t0 = std::chrono::steady_clock::now();
             if (j.dst) {
                 const int rc = j.dst->reserve(j.n);
+                gss::log_stall("record buffer allocated ahead",
+                          std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0)
+                              .count());
                 {
                     std::lock_guard<std::mutex> g(mu_);
                     j.state->store(rc == GSS_OK ? 2 : 0, std::memory_order_release);
@@ -241,6 +246,10 @@ class HostPrefetch {
             } else if (j.old) {
                 if (j.old_paged) std::free(j.old);
                 else (void)hipHostFree(j.old);
+                gss::log_stall("record buffer freed",
+                          std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - t0)
+                              .count());
             }
         }
     }
@@ -401,6 +410,18 @@ struct StepSlot {
 int gss::record_error(int code, const char *msg) {
     g_err = msg ? msg : "";
     return code;
+}
+
+void gss::log_stall(const char *what, double ms) {
+    static const double limit = [] {
+        const char *v = std::getenv("GPUSHARESAT_LOG_STALLS_MS");
+        return v ? std::atof(v) : -1.0;
+    }();
+    static const auto t0 = std::chrono::steady_clock::now();
+    if (limit < 0 || ms < limit) return;
+    const double at =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[gpusharesat] %.3f s: %s took %.1f ms\n", at, what, ms);
 }
 
 struct gss_engine {
@@ -1738,6 +1759,7 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
         const int64_t us = std::chrono::duration_cast<std::chrono::microseconds>(
                                std::chrono::steady_clock::now() - g0)
                                .count();
+        gss::log_stall("record buffer grown in a wait", us / 1000.0);
         e->host_grows.fetch_add(1, std::memory_order_relaxed);
         e->host_grow_us.fetch_add(us, std::memory_order_relaxed);
         if (us > e->host_grow_us_max.load(std::memory_order_relaxed))

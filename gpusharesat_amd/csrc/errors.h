@@ -5,4 +5,8 @@
 namespace gss {
 // Records msg as the calling thread's last error and returns code.
 int record_error(int code, const char *msg);
+// Writes "what took ms" to stderr, with the seconds since the library's
+// first such check, when GPUSHARESAT_LOG_STALLS_MS is set and ms reaches it
+// (host calls that held a serving thread or an engine's allocator thread).
+void log_stall(const char *what, double ms);
 }  // namespace gss

@@ -209,6 +209,7 @@ void launch(gss_server *s, std::unique_lock<std::mutex> &lk) {
     lk.lock();
     s->submit_ms_total += sub_ms;
     s->submit_ms_max = std::max(s->submit_ms_max, sub_ms);
+    gss::log_stall("serving step submit", sub_ms);
     if (rc) {
         set_failed(s, rc, "serving step");
         return;
@@ -341,6 +342,7 @@ void complete_oldest(gss_server *s, std::unique_lock<std::mutex> &lk) {
     lk.lock();
     if (grouped_here) s->host_grouped++;
     s->wait_ms_max = std::max(s->wait_ms_max, wait_ms);
+    gss::log_stall("serving step wait", wait_ms);
     const StepInfo si = s->inflight.front();
     s->inflight.pop_front();
     if (rc) {

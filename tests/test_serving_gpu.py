@@ -220,8 +220,9 @@ def _same(got, exp, what):
         assert np.array_equal(got[f], exp[f]), (what, f)
 
 
-@pytest.mark.parametrize("prefilter, max_chunks", [("1", 0), ("0", 0), ("1", 1)])
-def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilter, max_chunks):
+@pytest.mark.parametrize("prefilter, max_chunks, W", [("1", 0, 16), ("0", 0, 16), ("1", 1, 16),
+                                                      ("1", 0, 8), ("0", 0, 64)])
+def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilter, max_chunks, W):
     """Through the C ABI: request R alone, then copies of R inside a burst
     of other requests (admitted together on the device, at other word
     offsets): every copy's records equal R alone, and the oracle's on R's
@@ -231,10 +232,14 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
     prefilter's rows are cleared by patches as well as set from the bases.
     With one hit chunk per step every step with records overflows: its wait
     re-runs its sweep and replays the step launched behind it, which found
-    the engine halted and admitted nothing."""
+    the engine halted and admitted nothing.  A serving step writes its loaded
+    words in whole lines of the table (the first L words of each row, L =
+    the loaded words rounded up to 16, at most the table's words): with 8
+    words a variable's two rows share one line, with 64 a burst's words span
+    several lines of each row."""
     monkeypatch.setenv("GPUSHARESAT_SERVE_PREFILTER", prefilter)
     rng = np.random.default_rng(21)
-    V, W, N = 500, 16, 4000
+    V, N = 500, 4000
     lens = rng.integers(1, 13, size=N).astype(np.int32)
     lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
     off = np.zeros(N + 1, np.int64)
