@@ -628,6 +628,7 @@ int ensure_bucket_cap(gss_engine *e, int len, int64_t need, bool shrink = false)
     // Stream-ordered: the new array is allocated, filled and the old one
     // released in the engine stream's order, after every queued sweep or step
     // that still reads the old one, so growth never stalls the host.
+    const auto t0 = std::chrono::steady_clock::now();
     uint32_t *np = nullptr, *nt = nullptr;
     hipError_t he = hipMallocAsync((void **)&np, (size_t)nc * len * sizeof(uint32_t), e->stream);
     if (he == hipSuccess && e->tagged) {
@@ -651,6 +652,9 @@ int ensure_bucket_cap(gss_engine *e, int len, int64_t need, bool shrink = false)
     b.d = np;
     b.tags = nt;
     b.cap = nc;
+    gss::log_stall("clause bucket growth",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                       .count());
     return GSS_OK;
 }
 
@@ -2466,7 +2470,11 @@ int gss::step_submit_ex(gss_engine *e, const gss_step_desc *desc, bool serve) {
                 if (sl.graphs[i].used < sl.graphs[gi].used) gi = (int)i;
         }
         StepSlot::StepGraph &gr = sl.graphs[gi];
+        const auto t0 = std::chrono::steady_clock::now();
         if ((rc = build_step_graph(e, sl, gr, blocks, tc, fuse, serve, screen, agg_full))) return rc;
+        gss::log_stall("step graph capture", std::chrono::duration<double, std::milli>(
+                                                  std::chrono::steady_clock::now() - t0)
+                                                  .count());
         gr.key = key;
     }
     StepSlot::StepGraph &gr = sl.graphs[gi];
