@@ -99,7 +99,9 @@ int gss_server_create(int device, const gss_server_config *c, gss_server **out);
 /* Stops the thread once the steps in flight are done; requests not yet
  * admitted are dropped. */
 int gss_server_destroy(gss_server *s);
-/* The server's engine, for tuning (gss_set_tuning) before the first request. */
+/* The server's engine, for tuning (gss_set_tuning) before the first request.
+ * Its table is the serving steps' scratch: a step writes the words it loads
+ * and sets the rest of their 16-word groups to all ones. */
 int gss_server_engine(gss_server *s, gss_engine **out);
 /* Base updates (a later one for the same (base, var) wins): the next step the
  * serving thread launches applies them, before it loads its requests' words.
