@@ -1770,9 +1770,9 @@ static int queue_records_copy(gss_engine *e, StepSlot &sl, int64_t total, uint32
             e->host_grow_us_max.store(us, std::memory_order_relaxed);
     } else if (sl.spare_state[j].load(std::memory_order_acquire) == 2) {
         // a replacement allocated ahead is ready: it takes the buffer's place
-        // now, before a step needs it (with one buffer, once the copy into it
-        // has ended)
-        if (sl.h_n == 1 && sl.copy_queued) {
+        // now, before a step needs it, once the slot's copies have ended (the
+        // last one, and so every earlier one on the slot's stream)
+        if (sl.copy_queued) {
             HIP_TRY(hipEventSynchronize(sl.ev_copied));
             sl.copy_queued = false;
         }
