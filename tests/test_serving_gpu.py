@@ -295,6 +295,43 @@ def test_native_loop_request_alone_equals_in_mix_and_oracle(monkeypatch, prefilt
         assert (reruns > 0) == bool(max_chunks), reruns
 
 
+def test_native_loop_one_word_table():
+    """A one-word table: each step admits one request of at most 64
+    assignments and loads its word variable by variable (the whole-line
+    form needs an even word count); every request equals the oracle on its
+    explicit assignments."""
+    rng = np.random.default_rng(29)
+    V, N = 300, 2000
+    lens = rng.integers(1, 10, size=N).astype(np.int32)
+    lits = rng.integers(0, 2 * V, size=int(lens.sum())).astype(np.uint32)
+    off = np.zeros(N + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    bd, bt = random_bases(rng, 3, V)
+    with sv.GpuServer(V, 3, words=1) as srv:
+        assert srv.add_clauses(lits, lens) == 0
+        srv.update_base(base_updates(bd, bt))
+        sent = []
+        for i in range(8):
+            s_, k = i % 3, int(rng.integers(1, 65))
+            p = sv.slot_patches(rng, V, k, 6, 0.6)
+            p = p[p["word"] < 1]
+            p["mask"] &= ALL if k == 64 else np.uint64((1 << k) - 1)
+            p = p[p["mask"] != 0]
+            rid = srv.submit(s_, k, p)
+            assert rid is not None
+            sent.append((rid, s_, k, p))
+        got = {}
+        while len(got) < len(sent):
+            for rid in srv.poll(64, 2_000_000):
+                got[rid] = srv.result(rid)
+                srv.release(rid)
+        for rid, s_, k, p in sent:
+            meta, recs = got[rid]
+            assert meta["first_word"] == 0 and meta["clauses"] == N
+            d, t = _explicit_request(bd[s_], bt[s_], p, k)
+            _same(_as_oracle(recs), _oracle_request(off, lits, N, d, t, k), f"request {rid}")
+
+
 @pytest.mark.parametrize("records", [sv.RECORDS_IDS, sv.RECORDS_FULL])
 def test_result_held_across_many_steps_keeps_its_records(records):
     """A client holds a finished request's result unread while 24 more
